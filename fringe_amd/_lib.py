@@ -1,0 +1,106 @@
+"""ctypes binding of libfringe_amd.so (the C ABI in include/fringe_amd.h).
+
+The library is built in-tree (``python -m fringe_amd.build``).  Loading fails
+loudly when it is missing: there is no CPU fallback anywhere in this package.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfringe_amd.so")
+
+FRINGE_MODULE_EVD, FRINGE_MODULE_PHASE_LINK = 0, 1
+FRINGE_METHOD_MLE, FRINGE_METHOD_EVD, FRINGE_METHOD_STBAS = 0, 1, 2
+FRINGE_PREC_FP64, FRINGE_PREC_MIXED, FRINGE_PREC_MIXED_BF16 = 0, 1, 2
+METHODS = {"MLE": FRINGE_METHOD_MLE, "EVD": FRINGE_METHOD_EVD, "STBAS": FRINGE_METHOD_STBAS}
+PRECISIONS = {"fp64": FRINGE_PREC_FP64, "mixed": FRINGE_PREC_MIXED, "mixed_bf16": FRINGE_PREC_MIXED_BF16}
+
+# Symbols declared in include/fringe_amd.h
+EXPORTS = [
+    "fringe_evd_options_init", "fringe_evd_process", "fringe_phase_link_process",
+    "fringe_nmap_options_init", "fringe_nmap_process",
+    "fringe_link_workspace_bytes", "fringe_link_run",
+    "fringe_nmap_workspace_bytes", "fringe_nmap_run", "fringe_ks_table",
+    "fringe_mask_words", "fringe_version", "fringe_last_error",
+    "fringe_timing_enable", "fringe_timing_reset", "fringe_timing_read", "fringe_timing_stage_name",
+]
+
+
+class LinkTile(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_int) for f in (
+        "rows", "cols", "nbands", "Nx", "Ny", "module", "method", "min_neighbors",
+        "ref_index", "bandwidth", "precision", "row_begin", "row_end")]
+
+
+class NmapTile(ctypes.Structure):
+    _fields_ = [("rows", ctypes.c_int), ("cols", ctypes.c_int), ("nbands", ctypes.c_int),
+                ("Nx", ctypes.c_int), ("Ny", ctypes.c_int), ("prob", ctypes.c_double),
+                ("method", ctypes.c_int), ("row_begin", ctypes.c_int), ("row_end", ctypes.c_int)]
+
+
+class EvdOptions(ctypes.Structure):
+    _fields_ = [("inputDS", ctypes.c_char_p), ("wtsDS", ctypes.c_char_p),
+                ("compSlc", ctypes.c_char_p), ("outputCompressedSlcFolder", ctypes.c_char_p),
+                ("outputFolder", ctypes.c_char_p), ("blocksize", ctypes.c_int),
+                ("memsize", ctypes.c_int), ("Nx", ctypes.c_int), ("Ny", ctypes.c_int),
+                ("minNeighbors", ctypes.c_int), ("miniStackCount", ctypes.c_int),
+                ("bandWidth", ctypes.c_int), ("method", ctypes.c_char_p),
+                ("precision", ctypes.c_int), ("verbose", ctypes.c_int)]
+
+
+class NmapOptions(ctypes.Structure):
+    _fields_ = [("inputDS", ctypes.c_char_p), ("maskDS", ctypes.c_char_p),
+                ("wtsDS", ctypes.c_char_p), ("ncountDS", ctypes.c_char_p),
+                ("method", ctypes.c_char_p), ("noGPU", ctypes.c_int), ("blocksize", ctypes.c_int),
+                ("memsize", ctypes.c_int), ("Nx", ctypes.c_int), ("Ny", ctypes.c_int),
+                ("prob", ctypes.c_double), ("verbose", ctypes.c_int)]
+
+
+_LIB = None
+
+
+def lib():
+    """Load libfringe_amd.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m fringe_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    L.fringe_link_workspace_bytes.argtypes = [ctypes.POINTER(LinkTile)]
+    L.fringe_link_workspace_bytes.restype = sz
+    L.fringe_link_run.argtypes = [ctypes.POINTER(LinkTile), vp, vp, vp, vp, vp, vp, sz, vp]
+    L.fringe_nmap_workspace_bytes.argtypes = [ctypes.POINTER(NmapTile)]
+    L.fringe_nmap_workspace_bytes.restype = sz
+    L.fringe_nmap_run.argtypes = [ctypes.POINTER(NmapTile), vp, vp, vp, vp, vp, vp, sz, vp]
+    L.fringe_ks_table.argtypes = [ctypes.c_int, ctypes.c_double, vp]
+    L.fringe_ks_table.restype = ctypes.c_double
+    L.fringe_mask_words.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.fringe_version.restype = ctypes.c_char_p
+    L.fringe_last_error.restype = ctypes.c_char_p
+    L.fringe_timing_enable.argtypes = [ctypes.c_int]
+    L.fringe_timing_read.argtypes = [vp, vp, ctypes.c_int]
+    L.fringe_timing_stage_name.argtypes = [ctypes.c_int]
+    L.fringe_timing_stage_name.restype = ctypes.c_char_p
+    for name, st in (("fringe_evd_options_init", EvdOptions), ("fringe_evd_process", EvdOptions),
+                     ("fringe_phase_link_process", EvdOptions),
+                     ("fringe_nmap_options_init", NmapOptions), ("fringe_nmap_process", NmapOptions)):
+        if hasattr(L, name):
+            getattr(L, name).argtypes = [ctypes.POINTER(st)]
+    _LIB = L
+    return L
+
+
+def last_error():
+    return lib().fringe_last_error().decode()
+
+
+def stage_times():
+    """{stage: (ms, calls)} accumulated since the last reset (waits on events)."""
+    import numpy as np
+    L = lib()
+    ms = np.zeros(8, np.float64)
+    calls = np.zeros(8, np.int64)
+    k = L.fringe_timing_read(ms.ctypes.data, calls.ctypes.data, 8)
+    return {L.fringe_timing_stage_name(i).decode(): (float(ms[i]), int(calls[i])) for i in range(k)}

@@ -1,0 +1,231 @@
+// engine.hip -- host side of the tile engine: argument checks, workspace
+// layout, the KS decision table and the C ABI entry points declared in
+// include/fringe_amd.h.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "engine_internal.hpp"
+#include "link_solve.hpp"
+
+namespace fringe {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const char* what, hipError_t e) {
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+}
+void set_last_error(const std::string& what) { g_last_error = what; }
+
+static int device_lds_limit() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 65536;
+    static int cached[64] = {0};
+    if (dev >= 0 && dev < 64 && cached[dev]) return cached[dev];
+    hipDeviceProp_t prop;
+    int lim = 65536;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.sharedMemPerBlock > 0)
+        lim = (int)prop.sharedMemPerBlock;
+    if (dev >= 0 && dev < 64) cached[dev] = lim;
+    return lim;
+}
+
+// Kolmogorov distribution tail as KS2sample::KolmogorovProb evaluates it
+// (KS2sample.hpp:42-77): ROOT's piecewise approximation with sqrt(2 pi)
+// to 9 digits, -pi^2/8 * {1, 9, 25} and the series exponents -2 j^2.
+static double kolmogorov_prob(double z) {
+    const double w = 2.50662827;
+    const double c1 = -1.2337005501361697, c2 = -11.103304951225528, c3 = -30.842513753404244;
+    const double u = std::fabs(z);
+    if (u < 0.2) return 1.0;
+    if (u < 0.755) {
+        const double v = 1.0 / (u * u);
+        return 1.0 - w * (std::exp(c1 * v) + std::exp(c2 * v) + std::exp(c3 * v)) / u;
+    }
+    if (u < 6.8116) {
+        double r[4] = {0.0, 0.0, 0.0, 0.0};
+        const double v = u * u;
+        const int maxj = std::max(1, (int)std::round(3.0 / u));
+        for (int j = 0; j < maxj && j < 4; ++j) r[j] = std::exp(-2.0 * (j + 1) * (j + 1) * v);
+        return 2.0 * (r[0] - r[1] + r[2] - r[3]);
+    }
+    return 0.0;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// ---------------------------------------------------------------------------
+// phase linking
+// ---------------------------------------------------------------------------
+static int link_check(const fringe_link_tile* t) {
+    if (!t) return FRINGE_ERR_ARGUMENT;
+    if (t->rows <= 0 || t->cols <= 0 || t->nbands <= 0 || t->Nx < 0 || t->Ny < 0)
+        return FRINGE_ERR_ARGUMENT;
+    if (t->nbands > kMaxBandsLink) return FRINGE_ERR_UNSUPPORTED;
+    if (shp_words(t->Nx, t->Ny) > kMaxMaskWords) return FRINGE_ERR_UNSUPPORTED;
+    if (t->row_begin < 0 || t->row_end > t->rows || t->row_begin > t->row_end) return FRINGE_ERR_ARGUMENT;
+    if (t->ref_index < 0 || t->ref_index >= t->nbands) return FRINGE_ERR_ARGUMENT;
+    if (t->module != FRINGE_MODULE_EVD && t->module != FRINGE_MODULE_PHASE_LINK) return FRINGE_ERR_ARGUMENT;
+    if (t->method < FRINGE_METHOD_MLE || t->method > FRINGE_METHOD_STBAS) return FRINGE_ERR_ARGUMENT;
+    if (t->precision < FRINGE_PREC_FP64 || t->precision > FRINGE_PREC_MIXED_BF16) return FRINGE_ERR_ARGUMENT;
+    if (t->method == FRINGE_METHOD_STBAS && (t->bandwidth <= 0 || t->bandwidth >= t->nbands - 1))
+        return FRINGE_ERR_BANDWIDTH;   // evd.cpp:74-91
+    if ((long long)t->rows * t->cols >= (1ll << 31) - 1) return FRINGE_ERR_UNSUPPORTED;
+    // packed records are addressed with 32-bit word offsets
+    if (link_uses_mfma(t->nbands, t->precision) &&
+        ((long long)t->rows * t->cols + 1) * 2 * t->nbands >= (1ll << 32))
+        return FRINGE_ERR_UNSUPPORTED;
+    // LDS footprint of the solver (one wave per workgroup)
+    const int gpw = t->nbands <= 32 ? 2 : 1;
+    const size_t need = (t->precision == FRINGE_PREC_FP64 ? solve_group_bytes<double>(t->nbands)
+                                                          : solve_group_bytes<float>(t->nbands)) * gpw;
+    if (need > (size_t)device_lds_limit()) return FRINGE_ERR_UNSUPPORTED;
+    return 0;
+}
+
+static int link_chunk_pixels(const fringe_link_tile* t) {
+    const size_t per = link_gamma_bytes_per_pixel(t->nbands, t->precision);
+    const size_t target = (size_t)192 << 20;   // keep Gamma of one round near the last-level cache
+    long long rows = (long long)(target / (per * (size_t)t->cols));
+    if (rows < 1) rows = 1;
+    const long long out_rows = t->row_end - t->row_begin;
+    if (rows > out_rows) rows = out_rows > 0 ? out_rows : 1;
+    return (int)(rows * t->cols);
+}
+
+static size_t link_layout(const fringe_link_tile* t, size_t* off_gamma, size_t* off_npix,
+                          size_t* off_pack) {
+    const int chunk = link_chunk_pixels(t);
+    const int n = t->nbands;
+    const size_t e = link_uses_mfma(n, t->precision) ? 8 : 16;
+    size_t b = 0;
+    *off_gamma = b;
+    b += align256((size_t)chunk * n * n * e);
+    *off_npix = b;
+    b += align256((size_t)chunk * sizeof(int));
+    *off_pack = b;
+    if (link_uses_mfma(n, t->precision)) b += align256(mfma_pack_bytes(t->rows, t->cols, n));
+    return b;
+}
+
+}  // namespace fringe
+
+using namespace fringe;
+
+extern "C" {
+
+const char* fringe_version(void) { return "fringe_amd 0.1 (gfx950)"; }
+const char* fringe_last_error(void) { return g_last_error.c_str(); }
+int fringe_mask_words(int Nx, int Ny) { return shp_words(Nx, Ny); }
+
+size_t fringe_link_workspace_bytes(const fringe_link_tile* t) {
+    if (link_check(t) != 0) return 0;
+    size_t a, b, c;
+    return link_layout(t, &a, &b, &c);
+}
+
+int fringe_link_run(const fringe_link_tile* t, const void* d_slc, const uint32_t* d_wts,
+                    void* d_phase, float* d_tcorr, void* d_comp, void* d_work, size_t work_bytes,
+                    void* stream) {
+    const int st = link_check(t);
+    if (st != 0) return st;
+    if (!d_slc || !d_wts || !d_phase || !d_tcorr || !d_comp || !d_work) return FRINGE_ERR_ARGUMENT;
+    size_t og, on, op;
+    const size_t need = link_layout(t, &og, &on, &op);
+    if (work_bytes < need) return FRINGE_ERR_WORKSPACE;
+    if (t->row_end == t->row_begin) return 0;
+    LinkPlan P{};
+    P.rows = t->rows;
+    P.cols = t->cols;
+    P.n = t->nbands;
+    P.Nx = t->Nx;
+    P.Ny = t->Ny;
+    P.nul = shp_words(t->Nx, t->Ny);
+    P.module = t->module;
+    P.method = t->method;
+    // evd.cpp:566 ignores minNeighbors and requires 2; phase_link.cpp:524 honours it
+    P.min_nb = (t->module == FRINGE_MODULE_EVD) ? 2 : t->min_neighbors;
+    P.ref = t->ref_index;
+    P.bw = t->bandwidth;
+    P.precision = t->precision;
+    P.row_begin = t->row_begin;
+    P.row_end = t->row_end;
+    P.npx = (long long)t->rows * t->cols;
+    P.chunk = link_chunk_pixels(t);
+    P.d_gamma = (char*)d_work + og;
+    P.d_npix = (int*)((char*)d_work + on);
+    P.d_pack = (uint32_t*)((char*)d_work + op);
+    hipStream_t s = (hipStream_t)stream;
+    if (link_launch(P, (const float2*)d_slc, d_wts, (float2*)d_phase, d_tcorr, (float2*)d_comp, s) != 0) {
+        set_last_error("fringe_link_run", hipGetLastError());
+        return FRINGE_ERR_DEVICE;
+    }
+    return 0;
+}
+
+double fringe_ks_table(int n, double prob, uint8_t* h_table) {
+    const double rn = n;
+    const double scale = std::sqrt(rn * rn / (rn + rn));   // KS2sample.hpp:139
+    double margin = INFINITY;
+    for (int k = 0; k <= n; ++k) {
+        const double rd = (double)k / rn;
+        const double p = kolmogorov_prob(rd * scale);
+        if (h_table) h_table[k] = (p >= prob) ? 1 : 0;     // nmap.cpp:462
+        margin = std::min(margin, std::fabs(p - prob));
+    }
+    return margin;
+}
+
+size_t fringe_nmap_workspace_bytes(const fringe_nmap_tile* t) {
+    if (!t || t->rows <= 0 || t->cols <= 0 || t->nbands <= 0) return 0;
+    const size_t npx = (size_t)t->rows * t->cols;
+    return align256(npx * t->nbands * sizeof(float)) + align256(npx);
+}
+
+int fringe_nmap_run(const fringe_nmap_tile* t, const void* d_slc, const uint8_t* d_mask,
+                    const double* h_amp_const, uint32_t* d_wts, int32_t* d_count, void* d_work,
+                    size_t work_bytes, void* stream) {
+    if (!t || !d_slc || !d_wts || !d_count || !d_work) return FRINGE_ERR_ARGUMENT;
+    if (t->rows <= 0 || t->cols <= 0 || t->nbands <= 0 || t->Nx < 0 || t->Ny < 0)
+        return FRINGE_ERR_ARGUMENT;
+    if (t->row_begin < 0 || t->row_end > t->rows || t->row_begin > t->row_end) return FRINGE_ERR_ARGUMENT;
+    if (t->nbands > kMaxBandsNmap) return FRINGE_ERR_UNSUPPORTED;
+    if (shp_words(t->Nx, t->Ny) > kMaxMaskWords) return FRINGE_ERR_UNSUPPORTED;
+    if (t->method != 0) return FRINGE_ERR_UNSUPPORTED;   // AD2 is not built on the GPU yet
+    if ((long long)t->rows * t->cols >= (1ll << 31) - 1) return FRINGE_ERR_UNSUPPORTED;
+    if (work_bytes < fringe_nmap_workspace_bytes(t)) return FRINGE_ERR_WORKSPACE;
+    NmapPlan P{};
+    P.rows = t->rows;
+    P.cols = t->cols;
+    P.n = t->nbands;
+    P.Nx = t->Nx;
+    P.Ny = t->Ny;
+    P.nul = shp_words(t->Nx, t->Ny);
+    P.row_begin = t->row_begin;
+    P.row_end = t->row_end;
+    // alpha_b = amplitudeConstant_b / amplitudeConstant_1, missing or <= 0
+    // read as 1, alpha_1 forced to 1 (nmap.cpp:204-233)
+    for (int b = 0; b < t->nbands; ++b) {
+        const double c = h_amp_const ? h_amp_const[b] : 0.0;
+        P.alpha[b] = (c <= 0.0) ? 1.0 : c;
+    }
+    const double norm = P.alpha[0];
+    for (int b = 0; b < t->nbands; ++b) P.alpha[b] /= norm;
+    P.alpha[0] = 1.0;
+    uint8_t tab[kMaxBandsNmap + 1];
+    fringe_ks_table(t->nbands, t->prob, tab);
+    std::memset(P.table.bits, 0, sizeof(P.table.bits));
+    for (int k = 0; k <= t->nbands; ++k)
+        if (tab[k]) P.table.bits[k >> 5] |= (1u << (k & 31));
+    const size_t npx = (size_t)t->rows * t->cols;
+    P.d_amp = (float*)d_work;
+    P.d_valid = (uint8_t*)d_work + align256(npx * t->nbands * sizeof(float));
+    if (nmap_launch(P, (const float2*)d_slc, d_mask, d_wts, d_count, (hipStream_t)stream) != 0) {
+        set_last_error("fringe_nmap_run", hipGetLastError());
+        return FRINGE_ERR_DEVICE;
+    }
+    return 0;
+}
+
+}  // extern "C"

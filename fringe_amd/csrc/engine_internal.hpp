@@ -1,0 +1,52 @@
+// engine_internal.hpp -- plans shared between the host engine and the
+// kernel launchers (not part of the public C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/fringe_amd.h"
+
+namespace fringe {
+
+constexpr int kMaxBandsNmap = 128;   // nmap sorts at most this many dates
+constexpr int kMaxBandsLink = 128;   // phase-linking solver limit
+constexpr int kMaxMaskWords = 16;    // window up to 512 positions
+
+// Decision table of the integer KS statistic, one bit per k = D*n.
+struct KsTable {
+    uint32_t bits[(kMaxBandsNmap + 1 + 31) / 32];
+};
+
+struct NmapPlan {
+    int rows, cols, n, Nx, Ny, nul, row_begin, row_end;
+    double alpha[kMaxBandsNmap];  // amplitudeConstant_b / amplitudeConstant_1
+    KsTable table;
+    float* d_amp;      // [rows*cols][n] sorted amplitudes
+    uint8_t* d_valid;  // [rows*cols]
+};
+
+int nmap_launch(const NmapPlan& P, const float2* slc, const uint8_t* mask, uint32_t* wts,
+                int32_t* count, hipStream_t s);
+
+struct LinkPlan {
+    int rows, cols, n, Nx, Ny, nul;
+    int module, method, min_nb, ref, bw, precision;
+    int row_begin, row_end;
+    long long npx;
+    int chunk;          // pixels per Gram/solve round
+    void* d_gamma;      // [chunk][n][n] complex (double for FP64, float for mixed)
+    int* d_npix;        // [chunk]
+    uint32_t* d_pack;   // mixed precision: packed (hi|lo) records, [rows*cols+1][2n]
+};
+
+int link_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, float2* phase,
+                float* tcorr, float2* comp, hipStream_t s);
+size_t link_gamma_bytes_per_pixel(int n, int precision);
+size_t mfma_pack_bytes(int rows, int cols, int n);
+int pack_launch(const float2* slc, int rows, int cols, int n, bool f16, uint32_t* pk, hipStream_t s);
+int gram_mfma_launch(const LinkPlan& P, const uint32_t* pk, const uint32_t* wts, long long first,
+                     int slots, hipStream_t s);
+bool link_uses_mfma(int n, int precision);
+
+}  // namespace fringe

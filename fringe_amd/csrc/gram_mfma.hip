@@ -1,0 +1,384 @@
+// gram_mfma.hip -- masked-neighbourhood coherence on the MFMA units (gfx950).
+//
+// For a pixel with compacted SHP neighbour list k = 1..K (positions whose
+// mask bit is set and which lie inside the tile, evd.cpp:530-564), the
+// complex Gram G = sum_k z_k z_k^H is computed as the real Gram P = Y^T Y of
+// Y = [Re z | Im z] (K x 2n):
+//     Re G_ij = P(re_i, re_j) + P(im_i, im_j)
+//     Im G_ij = P(im_i, re_j) - P(re_i, im_j)
+// Each sample is split once into a 16-bit hi + lo pair (hi = h(x), lo =
+// h(x - hi)) and P is accumulated in FP32 by v_mfma_f32_32x32x16_{f16,bf16}
+// over hi*hi + hi*lo + lo*hi.
+//   * fp16 (default): every date is first scaled by an exact power of two so
+//     that its largest |Re|, |Im| over the tile lies in [2^14, 2^15); the pair
+//     then carries ~22 significant bits.  The coherence
+//     Gamma_ij = G_ij / sqrt(G_ii G_jj) is invariant to per-date scaling, so
+//     the scale never has to be undone.
+//   * bf16: the pair carries ~16 bits and keeps the FP32 exponent range, so
+//     no scaling is applied.
+//
+// Data flow:
+//   pack_hilo : the band-planar stack slc[t][row][col] is rewritten once as
+//               position-major packed words pk[pixel][re_0..re_{n-1},
+//               im_0..im_{n-1}] with (hi | lo << 16); one extra all-zero
+//               record at index rows*cols pads neighbour lists.
+//   gram_mfma : one wave per pixel at a time.  The wave compacts the mask
+//               into a list of pixel indices (padded to a multiple of 16) in
+//               LDS; per 16 positions lane l gathers component (l % 32) of 8
+//               positions for every 32-wide component block -- lanes 0..31
+//               read one contiguous 128-byte record slice, lanes 32..63
+//               another -- and issues 3 MFMAs per block pair.  The upper
+//               block triangle of P is folded into Gamma (the transposed
+//               cross block goes through a per-wave LDS scratch) and written.
+#include <algorithm>
+
+#include "device_common.hpp"
+#include "engine_internal.hpp"
+#include "../../include/fringe_amd.h"
+
+namespace fringe {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(8 * sizeof(_Float16)))) _Float16 f16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t u32x4;
+
+// Round-to-nearest-even float -> bf16 bits (finite input).
+__device__ inline uint32_t bf16_rne(float x) {
+    const uint32_t u = __float_as_uint(x);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ inline uint32_t split_hi_lo_bf16(float x) {
+    const uint32_t hi = bf16_rne(x);
+    const float rest = x - __uint_as_float(hi << 16);
+    const uint32_t lo = bf16_rne(rest);
+    return hi | (lo << 16);
+}
+__device__ inline uint32_t split_hi_lo_f16(float x) {
+    const _Float16 hi = (_Float16)x;
+    const _Float16 lo = (_Float16)(x - (float)hi);
+    return (uint32_t)__builtin_bit_cast(uint16_t, hi) | ((uint32_t)__builtin_bit_cast(uint16_t, lo) << 16);
+}
+
+// ---- per-date largest |Re|, |Im| over the tile ----------------------------
+// Non-negative floats order like their bit patterns, so the maximum is kept
+// as an unsigned word with atomicMax.
+__global__ void date_absmax_clear(uint32_t* dmax, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dmax[t] = 0u;
+}
+__global__ __launch_bounds__(256) void date_absmax(const float2* __restrict__ slc, long long npx,
+                                                   uint32_t* __restrict__ dmax) {
+    const int t = blockIdx.y;
+    float m = 0.0f;
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npx;
+         p += (long long)gridDim.x * blockDim.x) {
+        const float2 z = slc[(long long)t * npx + p];
+        m = fmaxf(m, fmaxf(fabsf(z.x), fabsf(z.y)));
+    }
+    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(dmax + t, __float_as_uint(m));
+}
+// Power-of-two exponent k with max * 2^k in [2^14, 2^15); 0 when the date is
+// all zero or not finite.
+__device__ inline int date_scale_exp(uint32_t bits) {
+    const float m = __uint_as_float(bits);
+    if (!(m > 0.0f) || !isfinite(m)) return 0;
+    int e;
+    frexpf(m, &e);          // m in [2^(e-1), 2^e)
+    return min(15 - e, 126);
+}
+
+// ---- pack: [t][pixel] complex64 -> [pixel][2n] (hi|lo) words -------------
+// 64 pixels per workgroup; the band-planar rows are read coalesced and the
+// records written contiguously through an LDS transpose.
+template <bool F16>
+__global__ __launch_bounds__(256) void pack_hilo(const float2* __restrict__ slc, int n,
+                                                 long long npx, const uint32_t* __restrict__ dmax,
+                                                 uint32_t* __restrict__ pk) {
+    extern __shared__ uint32_t st[];   // [64][2n]
+    const long long p0 = (long long)blockIdx.x * 64;
+    const int CS = 2 * n;
+    for (int e = threadIdx.x; e < 64 * n; e += blockDim.x) {
+        const int t = e / 64, i = e % 64;
+        const long long p = p0 + i;
+        float2 z = make_float2(0.0f, 0.0f);
+        if (p < npx) z = slc[(long long)t * npx + p];
+        if (F16) {
+            const int k = date_scale_exp(dmax[t]);
+            st[i * CS + t] = split_hi_lo_f16(ldexpf(z.x, k));
+            st[i * CS + n + t] = split_hi_lo_f16(ldexpf(z.y, k));
+        } else {
+            st[i * CS + t] = split_hi_lo_bf16(z.x);
+            st[i * CS + n + t] = split_hi_lo_bf16(z.y);
+        }
+    }
+    __syncthreads();
+    const long long words = (long long)min((long long)64, npx - p0) * CS;
+    for (long long e = threadIdx.x; e < words; e += blockDim.x) pk[p0 * CS + e] = st[e];
+    if (blockIdx.x == 0)
+        for (int e = threadIdx.x; e < CS; e += blockDim.x) pk[npx * CS + e] = 0u;
+}
+
+struct MfmaArgs {
+    const uint32_t* pk;   // packed records, [rows*cols + 1][2n]
+    const uint32_t* wts;
+    int rows, cols, n, Nx, Ny, nul, min_nb;
+    int row0;             // first output row of this chunk
+    int chunk_rows;
+    int TH, TW;           // output pixels per workgroup
+    float* gamma;         // [slots][n][n] complex float, interleaved
+    int* npix;            // [slots]
+    int list_cap;         // ints per wave list
+};
+
+// 32x32 accumulator layout: lane l, element v holds C[row][col] with
+// col = l % 32 and row = 8*(v/4) + 4*(l/32) + (v%4).
+__device__ inline int acc_row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
+
+template <int NBH> struct TileIdx {
+    static constexpr int NB = 2 * NBH;
+    // (bi, bj), bi <= bj < NB, in row-major upper order
+    static __device__ constexpr int idx(int bi, int bj) {
+        return bi * NB - (bi * (bi - 1)) / 2 + (bj - bi);
+    }
+    static constexpr int count = NB * (NB + 1) / 2;
+};
+
+__device__ inline void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool F16> struct Frag;
+template <> struct Frag<true> {
+    typedef f16x8 type;
+    static __device__ inline f32x16 mfma(type a, type b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Frag<false> {
+    typedef bf16x8 type;
+    static __device__ inline f32x16 mfma(type a, type b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+template <int NBH, int WAVES, bool F16>
+__global__ __launch_bounds__(64 * WAVES) void gram_mfma(MfmaArgs a) {
+    typedef typename Frag<F16>::type frag_t;
+    constexpr int NB = 2 * NBH;
+    constexpr int NT = TileIdx<NBH>::count;
+    extern __shared__ __align__(16) char smem[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const size_t wbytes = (size_t)a.list_cap * 4 + 32 * 33 * 4 + 64 * NBH * 4;
+    int* list = (int*)(smem + (size_t)wave * wbytes);
+    float* xs = (float*)(list + a.list_cap);   // 32 x 33 transpose scratch
+    float* dg = xs + 32 * 33;                  // diagonal of Re G
+
+    const int n = a.n, Nx = a.Nx, Ny = a.Ny;
+    const int CS = 2 * n;
+    const int npx_zero = a.rows * a.cols;      // index of the all-zero record
+    const int R0 = a.row0 + blockIdx.y * a.TH, C0 = blockIdx.x * a.TW;
+    const int row_end = a.row0 + a.chunk_rows;
+    const int Wx = 2 * Nx + 1, Wn = (2 * Ny + 1) * Wx;
+    const int m = lane & 31, half = lane >> 5;
+
+    uint32_t coff[NB];
+    bool cval[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int date = 32 * (b % NBH) + m;
+        cval[b] = date < n;
+        coff[b] = cval[b] ? (uint32_t)((b < NBH) ? date : n + date) : 0u;
+    }
+
+    for (int pid = wave; pid < a.TH * a.TW; pid += WAVES) {
+        const int r = R0 + pid / a.TW, c = C0 + pid % a.TW;
+        if (r >= row_end || r >= a.rows || c >= a.cols) continue;   // wave-uniform
+        const long long pix = (long long)r * a.cols + c;
+        const int slot = (r - a.row0) * a.cols + c;
+        const uint32_t* w = a.wts + pix * a.nul;
+        const bool cen = shp_get(w, shp_flat(0, 0, Nx, Ny));
+
+        // ---- neighbour list ----
+        int cnt = 0;
+        for (int b0 = 0; b0 < Wn; b0 += 64) {
+            const int f = b0 + lane;
+            bool take = false;
+            int q = 0;
+            if (cen && f < Wn) {
+                const int rr = r + f / Wx - Ny, cc = c + f % Wx - Nx;
+                if (rr >= 0 && rr < a.rows && cc >= 0 && cc < a.cols && shp_get(w, f)) {
+                    take = true;
+                    q = rr * a.cols + cc;
+                }
+            }
+            const uint64_t bal = __ballot(take);
+            if (take) {
+                const uint64_t below = (lane == 0) ? 0ull : (bal & ((~0ull) >> (64 - lane)));
+                list[cnt + __popcll(below)] = q;
+            }
+            cnt += __popcll(bal);
+        }
+        if (!(cen && cnt >= a.min_nb)) {
+            if (lane == 0) a.npix[slot] = -1;
+            continue;
+        }
+        const int steps = (cnt + 15) >> 4;
+        for (int e = cnt + lane; e < steps * 16; e += 64) list[e] = npx_zero;
+        wave_sync_lds();
+
+        // ---- MFMA accumulation ----
+        f32x16 acc[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[t][v] = 0.0f;
+
+        for (int s = 0; s < steps; ++s) {
+            const int* lp = list + 16 * s + 8 * half;
+            uint32_t base[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) base[e] = (uint32_t)lp[e] * (uint32_t)CS;
+            frag_t fh[NB], fl[NB];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                uint32_t wv[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) wv[e] = cval[b] ? a.pk[base[e] + coff[b]] : 0u;
+                u32x4 hv, lv;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    hv[e] = __builtin_amdgcn_perm(wv[2 * e + 1], wv[2 * e], 0x05040100u);
+                    lv[e] = __builtin_amdgcn_perm(wv[2 * e + 1], wv[2 * e], 0x07060302u);
+                }
+                fh[b] = __builtin_bit_cast(frag_t, hv);
+                fl[b] = __builtin_bit_cast(frag_t, lv);
+            }
+#pragma unroll
+            for (int bi = 0; bi < NB; ++bi)
+#pragma unroll
+                for (int bj = bi; bj < NB; ++bj) {
+                    f32x16& C = acc[TileIdx<NBH>::idx(bi, bj)];
+                    C = Frag<F16>::mfma(fh[bi], fh[bj], C);
+                    C = Frag<F16>::mfma(fh[bi], fl[bj], C);
+                    C = Frag<F16>::mfma(fl[bi], fh[bj], C);
+                }
+        }
+
+        // ---- fold P into Gamma ----
+#pragma unroll
+        for (int I = 0; I < NBH; ++I) {
+            const f32x16& A0 = acc[TileIdx<NBH>::idx(I, I)];
+            const f32x16& A1 = acc[TileIdx<NBH>::idx(NBH + I, NBH + I)];
+#pragma unroll
+            for (int v = 0; v < 16; ++v)
+                if (acc_row(v, lane) == m) dg[32 * I + m] = A0[v] + A1[v];
+        }
+        float* gout = a.gamma + (size_t)slot * n * n * 2;
+#pragma unroll
+        for (int I = 0; I < NBH; ++I)
+#pragma unroll
+            for (int J = I; J < NBH; ++J) {
+                const f32x16& X = acc[TileIdx<NBH>::idx(J, NBH + I)];   // P(re_J, im_I)
+                wave_sync_lds();
+#pragma unroll
+                for (int v = 0; v < 16; ++v) xs[acc_row(v, lane) * 33 + m] = X[v];
+                wave_sync_lds();
+                const f32x16& RR = acc[TileIdx<NBH>::idx(I, J)];
+                const f32x16& II = acc[TileIdx<NBH>::idx(NBH + I, NBH + J)];
+                const f32x16& RI = acc[TileIdx<NBH>::idx(I, NBH + J)];
+                const int j = 32 * J + m;
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int il = acc_row(v, lane);
+                    const int i = 32 * I + il;
+                    if (i >= n || j >= n || i > j) continue;
+                    float gr = 1.0f, gi = 0.0f;
+                    if (i != j) {
+                        const float re = RR[v] + II[v];
+                        const float im = xs[m * 33 + il] - RI[v];   // P(im_i, re_j) - P(re_i, im_j)
+                        const float sc = rsqrtf(dg[i] * dg[j]);
+                        gr = re * sc;
+                        gi = im * sc;
+                    }
+                    gout[2 * ((size_t)i * n + j)] = gr;
+                    gout[2 * ((size_t)i * n + j) + 1] = gi;
+                    if (i != j) {
+                        gout[2 * ((size_t)j * n + i)] = gr;
+                        gout[2 * ((size_t)j * n + i) + 1] = -gi;
+                    }
+                }
+            }
+        if (lane == 0) a.npix[slot] = cnt;
+        wave_sync_lds();
+    }
+}
+
+// packed records followed by the per-date maxima (256-byte aligned)
+size_t mfma_pack_bytes(int rows, int cols, int n) {
+    const size_t rec = ((size_t)rows * cols + 1) * 2 * n * sizeof(uint32_t);
+    return ((rec + 255) & ~(size_t)255) + (size_t)n * sizeof(uint32_t);
+}
+
+int pack_launch(const float2* slc, int rows, int cols, int n, bool f16, uint32_t* pk, hipStream_t s) {
+    const long long npx = (long long)rows * cols;
+    const size_t rec = ((size_t)npx + 1) * 2 * n * sizeof(uint32_t);
+    uint32_t* dmax = (uint32_t*)((char*)pk + ((rec + 255) & ~(size_t)255));
+    const unsigned blocks = (unsigned)((npx + 63) / 64);
+    if (f16) {
+        hipLaunchKernelGGL(date_absmax_clear, dim3((n + 63) / 64), dim3(64), 0, s, dmax, n);
+        const unsigned bx = (unsigned)std::min<long long>((npx + 255) / 256, 1024);
+        hipLaunchKernelGGL(date_absmax, dim3(bx, n), dim3(256), 0, s, slc, npx, dmax);
+        hipLaunchKernelGGL(pack_hilo<true>, dim3(blocks), dim3(256), (size_t)64 * 2 * n * 4, s, slc, n,
+                           npx, (const uint32_t*)dmax, pk);
+    } else {
+        hipLaunchKernelGGL(pack_hilo<false>, dim3(blocks), dim3(256), (size_t)64 * 2 * n * 4, s, slc, n,
+                           npx, (const uint32_t*)dmax, pk);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int NBH, int WAVES, bool F16>
+static void launch_mfma(MfmaArgs a, hipStream_t s) {
+    const size_t wbytes = (size_t)a.list_cap * 4 + 32 * 33 * 4 + 64 * NBH * 4;
+    const dim3 grid((unsigned)((a.cols + a.TW - 1) / a.TW),
+                    (unsigned)((a.chunk_rows + a.TH - 1) / a.TH));
+    hipLaunchKernelGGL((gram_mfma<NBH, WAVES, F16>), grid, dim3(64 * WAVES), wbytes * WAVES, s, a);
+}
+
+int gram_mfma_launch(const LinkPlan& P, const uint32_t* pk, const uint32_t* wts, long long first,
+                     int slots, hipStream_t s) {
+    if (P.n > 64 || first % P.cols != 0 || slots % P.cols != 0) return -1;
+    MfmaArgs a{};
+    a.pk = pk;
+    a.wts = wts;
+    a.rows = P.rows;
+    a.cols = P.cols;
+    a.n = P.n;
+    a.Nx = P.Nx;
+    a.Ny = P.Ny;
+    a.nul = P.nul;
+    a.min_nb = P.min_nb;
+    a.row0 = (int)(first / P.cols);
+    a.chunk_rows = slots / P.cols;
+    a.TH = 2;
+    a.TW = 8;
+    a.gamma = (float*)P.d_gamma;
+    a.npix = P.d_npix;
+    const int Wn = (2 * P.Ny + 1) * (2 * P.Nx + 1);
+    a.list_cap = ((Wn + 15) / 16) * 16 + 16;
+    const bool f16 = (P.precision != FRINGE_PREC_MIXED_BF16);
+    if (P.n <= 32) {
+        if (f16) launch_mfma<1, 4, true>(a, s);
+        else launch_mfma<1, 4, false>(a, s);
+    } else {
+        if (f16) launch_mfma<2, 4, true>(a, s);
+        else launch_mfma<2, 4, false>(a, s);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace fringe

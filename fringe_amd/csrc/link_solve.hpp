@@ -1,0 +1,612 @@
+// link_solve.hpp -- the per-pixel estimator chain that follows the Gram:
+// normalised coherence Gamma -> (|Gamma|^-1 o Gamma) -> extreme eigenpair ->
+// referenced phases, compressed SLC, temporal coherence.
+//
+// Lane groups: a wave holds 64/G pixels; lane g of a group owns matrix rows
+// g + G*q (q < R), so n <= G*R.  All control flow that contains a barrier is
+// uniform over the wave (it depends only on launch arguments or on wave-wide
+// votes); per-pixel decisions are carried as predicates.
+#pragma once
+#include "device_common.hpp"
+#include "../../include/fringe_amd.h"
+
+namespace fringe {
+
+struct SolveArgs {
+    const void* gamma;   // [slots][n][n] cx<Tg>, row-major, Hermitian
+    const int* npix;     // [slots]: neighbour count, -1 = pixel left untouched
+    const float2* slc;   // tile SLC stack [band][row][col] (centre pixel for comp)
+    long long npx;       // rows*cols of the tile
+    long long first;     // linear pixel index of slot 0
+    int slots;
+    int n, module, method, ref, bw;
+    size_t group_bytes;
+    float2* phase;       // [band][row][col]
+    float* tcorr;        // [row][col]
+    float2* comp;        // [row][col]
+};
+
+template <typename T> struct Lim;
+template <> struct Lim<float> {
+    static __device__ constexpr float eps() { return 5.9604645e-8f; }
+    static __device__ constexpr float tiny() { return 1.0e-30f; }
+    static __device__ constexpr int iters(int G) { return G >= 64 ? 6 : 7; }
+};
+template <> struct Lim<double> {
+    static __device__ constexpr double eps() { return 1.1102230246251565e-16; }
+    static __device__ constexpr double tiny() { return 1.0e-290; }
+    static __device__ constexpr int iters(int G) { return G >= 64 ? 10 : 12; }
+};
+
+// Per-group LDS carve-up.
+template <typename T> struct SolveLds {
+    cx<T>* A;        // n x S (S = n|1), also hosts the FP64 |Gamma| inverse
+    cx<T>* V;        // reflector being applied
+    cx<T>* W;
+    cx<T>* TAU;      // reflector scalars
+    cx<T>* Y;        // eigenvector, for the reference lookup
+    T *D, *E, *E2;   // tridiagonal T
+    T *X, *DD, *DU, *DU2, *DL;  // inverse-iteration work
+    cx<float>* OUT;  // referenced unit phasors
+};
+
+__host__ __device__ inline size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
+
+template <typename T> __host__ __device__ inline size_t solve_group_bytes(int n) {
+    const int S = n | 1;
+    size_t b = align16((size_t)n * S * sizeof(cx<T>));
+    b += 5 * align16((size_t)n * sizeof(cx<T>));
+    b += 8 * align16((size_t)n * sizeof(T));
+    b += align16((size_t)n * sizeof(cx<float>));
+    return b;
+}
+
+template <typename T> __device__ inline SolveLds<T> solve_carve(char* base, int n) {
+    SolveLds<T> L;
+    const int S = n | 1;
+    char* p = base;
+    L.A = (cx<T>*)p;   p += align16((size_t)n * S * sizeof(cx<T>));
+    L.V = (cx<T>*)p;   p += align16((size_t)n * sizeof(cx<T>));
+    L.W = (cx<T>*)p;   p += align16((size_t)n * sizeof(cx<T>));
+    L.TAU = (cx<T>*)p; p += align16((size_t)n * sizeof(cx<T>));
+    L.Y = (cx<T>*)p;   p += align16((size_t)n * sizeof(cx<T>));
+    p += align16((size_t)n * sizeof(cx<T>));   // spare
+    L.D = (T*)p;   p += align16((size_t)n * sizeof(T));
+    L.E = (T*)p;   p += align16((size_t)n * sizeof(T));
+    L.E2 = (T*)p;  p += align16((size_t)n * sizeof(T));
+    L.X = (T*)p;   p += align16((size_t)n * sizeof(T));
+    L.DD = (T*)p;  p += align16((size_t)n * sizeof(T));
+    L.DU = (T*)p;  p += align16((size_t)n * sizeof(T));
+    L.DU2 = (T*)p; p += align16((size_t)n * sizeof(T));
+    L.DL = (T*)p;  p += align16((size_t)n * sizeof(T));
+    L.OUT = (cx<float>*)p;
+    return L;
+}
+
+template <int G> __device__ inline bool group_all(bool b) { return group_ballot<G>(!b) == 0ull; }
+
+// ---- load Gamma (optionally banded for STBAS) into A --------------------
+template <typename T, typename Tg, int G, int R>
+__device__ bool load_gamma(cx<T>* A, int S, const cx<Tg>* gam, int n, bool act, int band) {
+    const int g = threadIdx.x % G;
+    bool fin = true;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int r = g + G * q;
+        if (r >= n) continue;
+        for (int c = 0; c < n; ++c) {
+            cx<Tg> v = act ? gam[(size_t)r * n + c] : cx<Tg>{Tg(r == c ? 1 : 0), Tg(0)};
+            if (band >= 0 && (c - r > band || r - c > band)) v = cx<Tg>{Tg(0), Tg(0)};
+            fin = fin && isfinite(v.re) && isfinite(v.im);
+            A[r * S + c] = cx<T>{(T)v.re, (T)v.im};
+        }
+    }
+    __syncthreads();
+    return group_all<G>(fin);
+}
+
+// ---- |Gamma| (unit diagonal, minus shift) as FP64 in A's storage ---------
+// M row r starts where A row r starts; MS is the row stride in doubles.
+template <typename Tg, int G, int R>
+__device__ void load_abs_gamma(double* M, int MS, const cx<Tg>* gam, int n, bool act, double shift) {
+    const int g = threadIdx.x % G;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int r = g + G * q;
+        if (r >= n) continue;
+        for (int c = 0; c < n; ++c) {
+            double m;
+            if (c == r) {
+                m = 1.0 - shift;
+            } else if (act) {
+                const cx<Tg> v = gam[(size_t)r * n + c];
+                m = hypot((double)v.re, (double)v.im);
+            } else {
+                m = 0.0;
+            }
+            M[r * MS + c] = m;
+        }
+    }
+    __syncthreads();
+}
+
+// ---- in-place inverse of a symmetric positive definite matrix ------------
+// Gauss-Jordan sweep without pivoting.  The pivot met at step j is the
+// Schur complement M_jj - M_j,<j M_<j,<j^-1 M_<j,j, i.e. the square of the
+// j-th Cholesky diagonal, so the matrix is declared not positive definite
+// exactly when a Cholesky factorisation would be (a pivot <= 0 or NaN).
+template <int G, int R>
+__device__ bool sweep_inverse(double* M, int MS, int n) {
+    const int g = threadIdx.x % G;
+    bool ok = true;
+    for (int j = 0; j < n; ++j) {
+        const double p = M[j * MS + j];
+        ok = ok && (p > 0.0);
+        const double piv = 1.0 / p;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int c = g + G * q;
+            if (c < n) M[j * MS + c] = (c == j) ? piv : M[j * MS + c] * piv;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int r = g + G * q;
+            if (r >= n || r == j) continue;
+            const double f = M[r * MS + j];
+            for (int c = 0; c < n; ++c)
+                if (c != j) M[r * MS + c] -= f * M[j * MS + c];
+            M[r * MS + j] = -f * piv;
+        }
+        __syncthreads();
+    }
+    // mirror the upper triangle into the lower one (zpotri leaves only the
+    // upper triangle and EigenLapack.hpp:240-248 mirrors it the same way)
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int r = g + G * q;
+        if (r >= n) continue;
+        for (int c = 0; c < r; ++c) M[r * MS + c] = M[c * MS + r];
+    }
+    __syncthreads();
+    return group_all<G>(ok);
+}
+
+// ---- H = M o Gamma into A (decreasing columns keep FP64 M intact until read)
+template <typename T, typename Tg, int G, int R>
+__device__ bool form_hadamard(cx<T>* A, int S, const double* M, int MS, const cx<Tg>* gam, int n,
+                              bool act) {
+    const int g = threadIdx.x % G;
+    bool fin = true;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int r = g + G * q;
+        if (r >= n) continue;
+        for (int c = n - 1; c >= 0; --c) {
+            const double m = M[r * MS + c];
+            const cx<Tg> v = act ? gam[(size_t)r * n + c] : cx<Tg>{Tg(r == c ? 1 : 0), Tg(0)};
+            const double hr = m * (double)v.re, hi = m * (double)v.im;
+            fin = fin && isfinite(hr) && isfinite(hi);
+            A[r * S + c] = cx<T>{(T)hr, (T)hi};
+        }
+    }
+    __syncthreads();
+    return group_all<G>(fin);
+}
+
+// ---- Householder tridiagonalisation A = Q T Q^H ---------------------------
+// Reflector j maps column j below the diagonal onto beta*e1 with beta real,
+// so T = tridiag(D, E) is real.  v_j is stored in A[r][j] for r > j+1
+// (v_{j+1} = 1 implicit) and tau_j in TAU[j].
+template <typename T, int G, int R>
+__device__ void tridiagonalize(SolveLds<T>& L, int S, int n) {
+    const int g = threadIdx.x % G;
+    cx<T>* A = L.A;
+    for (int j = 0; j < n - 1; ++j) {
+        const cx<T> alpha = A[(j + 1) * S + j];
+        T xn2 = T(0);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int r = g + G * q;
+            if (r > j + 1 && r < n) xn2 += cabs2(A[r * S + j]);
+        }
+        xn2 = gsum<G>(xn2);
+        const T ar = alpha.re, ai = alpha.im;
+        cx<T> tau = {T(0), T(0)}, scal = {T(0), T(0)};
+        T beta = ar;
+        if (xn2 != T(0) || ai != T(0)) {
+            const T nrm = sqrt(ar * ar + ai * ai + xn2);
+            beta = (ar >= T(0)) ? -nrm : nrm;
+            tau = cx<T>{(beta - ar) / beta, -ai / beta};
+            scal = crecip(cx<T>{ar - beta, ai});
+        }
+        if (g == 0) {
+            L.D[j] = A[j * S + j].re;
+            L.E[j] = beta;
+            L.TAU[j] = tau;
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int r = g + G * q;
+            if (r >= n) continue;
+            cx<T> v = {T(0), T(0)};
+            if (r == j + 1) {
+                v = cx<T>{T(1), T(0)};
+            } else if (r > j + 1) {
+                v = cmul(A[r * S + j], scal);
+                A[r * S + j] = v;
+            }
+            L.V[r] = v;
+        }
+        __syncthreads();
+        // p = tau * A22 v ; w = p - (tau/2)(p^H v) v
+        cx<T> p[R];
+        cx<T> s = {T(0), T(0)};
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int r = g + G * q;
+            p[q] = cx<T>{T(0), T(0)};
+            if (r > j && r < n) {
+                for (int c = j + 1; c < n; ++c) cfma(p[q], A[r * S + c], L.V[c]);
+                p[q] = cmul(tau, p[q]);
+                const cx<T> vr = L.V[r];
+                s = cadd(s, cmulc(vr, p[q]));
+            }
+        }
+        s = gsum<G>(s);
+        const cx<T> alpha2 = cscale(cmul(tau, s), T(-0.5));
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int r = g + G * q;
+            if (r > j && r < n) L.W[r] = cadd(p[q], cmul(alpha2, L.V[r]));
+        }
+        __syncthreads();
+        // A22 -= v w^H + w v^H
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int r = g + G * q;
+            if (r <= j || r >= n) continue;
+            const cx<T> vr = L.V[r], wr = L.W[r];
+            for (int c = j + 1; c < n; ++c) {
+                cx<T> a = A[r * S + c];
+                const cx<T> vc = L.V[c], wc = L.W[c];
+                a = csub(a, cmulc(vr, wc));
+                a = csub(a, cmulc(wr, vc));
+                A[r * S + c] = a;
+            }
+        }
+        __syncthreads();
+    }
+    if (g == 0) L.D[n - 1] = A[(n - 1) * S + (n - 1)].re;
+    __syncthreads();
+}
+
+// Number of eigenvalues of tridiag(D, E) below sigma (LDL^T inertia).
+template <typename T>
+__device__ inline int sturm_count(const T* D, const T* E2, int n, T sigma, T pivmin) {
+    T q = D[0] - sigma;
+    if (fabs(q) < pivmin) q = -pivmin;
+    int cnt = (q < T(0)) ? 1 : 0;
+    for (int i = 1; i < n; ++i) {
+        q = (D[i] - sigma) - E2[i - 1] / q;
+        if (fabs(q) < pivmin) q = -pivmin;
+        cnt += (q < T(0)) ? 1 : 0;
+    }
+    return cnt;
+}
+
+// k-th smallest eigenvalue (0-based) by (G+1)-section: lane g tests the
+// g-th interior point of the current interval.  tnorm receives max|D|+|E|.
+template <typename T, int G>
+__device__ T select_eigenvalue(SolveLds<T>& L, int n, int k, T& tnorm) {
+    const int g = threadIdx.x % G;
+    T lo = T(3.0e38), hi = T(-3.0e38), em2 = T(0), tn = T(0);
+    for (int i = g; i < n; i += G) {
+        const T a = (i > 0) ? fabs(L.E[i - 1]) : T(0);
+        const T b = (i < n - 1) ? fabs(L.E[i]) : T(0);
+        lo = min(lo, L.D[i] - a - b);
+        hi = max(hi, L.D[i] + a + b);
+        tn = max(tn, fabs(L.D[i]) + a + b);
+        if (i < n - 1) {
+            L.E2[i] = L.E[i] * L.E[i];
+            em2 = max(em2, L.E2[i]);
+        }
+    }
+    lo = gmin<G>(lo);
+    hi = gmax<G>(hi);
+    em2 = gmax<G>(em2);
+    tn = gmax<G>(tn);
+    __syncthreads();
+    tnorm = tn;
+    const T pivmin = Lim<T>::tiny() * max(T(1), em2);
+    const T pad = T(4) * Lim<T>::eps() * tn + T(2) * pivmin;
+    lo -= pad;
+    hi += pad;
+    const int iters = Lim<T>::iters(G);
+    for (int it = 0; it < iters; ++it) {
+        const T w = hi - lo;
+        const T sig = lo + (w * T(g + 1)) / T(G + 1);
+        const int cnt = sturm_count(L.D, L.E2, n, sig, pivmin);
+        const uint64_t b = group_ballot<G>(cnt > k);
+        const int f = b ? (int)__builtin_ctzll(b) : G;
+        const T nhi = (f < G) ? lo + (w * T(f + 1)) / T(G + 1) : hi;
+        const T nlo = (f > 0) ? lo + (w * T(f)) / T(G + 1) : lo;
+        lo = nlo;
+        hi = nhi;
+    }
+    return T(0.5) * (lo + hi);
+}
+
+// Inverse iteration on tridiag(D, E) - lam I (Gaussian elimination with
+// partial pivoting, small pivots lifted to eps*|T|); run by one lane.
+template <typename T>
+__device__ void tridiag_inverse_iteration(SolveLds<T>& L, int n, T lam, T tnorm) {
+    T* X = L.X;
+    T *DD = L.DD, *DU = L.DU, *DU2 = L.DU2, *DL = L.DL;
+    const T pert = max(Lim<T>::eps() * tnorm, Lim<T>::tiny());
+    for (int i = 0; i < n; ++i) X[i] = T(0.5) + T((i * 37 + 11) & 63) * T(1.0 / 64.0);
+    for (int it = 0; it < 3; ++it) {
+        for (int i = 0; i < n; ++i) {
+            DD[i] = L.D[i] - lam;
+            if (i < n - 1) {
+                DU[i] = L.E[i];
+                DL[i] = L.E[i];
+            }
+        }
+        for (int i = 0; i < n - 1; ++i) {
+            if (fabs(DD[i]) >= fabs(DL[i])) {
+                if (fabs(DD[i]) < pert) DD[i] = (DD[i] < T(0)) ? -pert : pert;
+                const T f = DL[i] / DD[i];
+                DD[i + 1] -= f * DU[i];
+                X[i + 1] -= f * X[i];
+                DU2[i] = T(0);
+            } else {
+                const T f = DD[i] / DL[i];
+                DD[i] = DL[i];
+                const T tmp = DD[i + 1];
+                DD[i + 1] = DU[i] - f * tmp;
+                DU[i] = tmp;
+                if (i < n - 2) {
+                    DU2[i] = DU[i + 1];
+                    DU[i + 1] = -f * DU2[i];
+                } else {
+                    DU2[i] = T(0);
+                }
+                const T tx = X[i];
+                X[i] = X[i + 1];
+                X[i + 1] = tx - f * X[i];
+            }
+        }
+        if (fabs(DD[n - 1]) < pert) DD[n - 1] = (DD[n - 1] < T(0)) ? -pert : pert;
+        X[n - 1] /= DD[n - 1];
+        if (n > 1) X[n - 2] = (X[n - 2] - DU[n - 2] * X[n - 1]) / DD[n - 2];
+        for (int i = n - 3; i >= 0; --i)
+            X[i] = (X[i] - DU[i] * X[i + 1] - DU2[i] * X[i + 2]) / DD[i];
+        T mx = T(0);
+        for (int i = 0; i < n; ++i) mx = max(mx, fabs(X[i]));
+        const T sc = (mx > T(0) && isfinite(mx)) ? T(1) / mx : T(1);
+        for (int i = 0; i < n; ++i) X[i] *= sc;
+    }
+}
+
+// Eigenvector of A for eigenvalue index k (0-based ascending) after
+// tridiagonalize(): returns the eigenvalue, y holds the lane's rows.
+template <typename T, int G, int R>
+__device__ T extreme_eigenpair(SolveLds<T>& L, int S, int n, int k, cx<T> (&y)[R]) {
+    const int g = threadIdx.x % G;
+    T tnorm;
+    const T lam = select_eigenvalue<T, G>(L, n, k, tnorm);
+    if (g == 0) tridiag_inverse_iteration<T>(L, n, lam, tnorm);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int r = g + G * q;
+        y[q] = cx<T>{(r < n) ? L.X[r] : T(0), T(0)};
+    }
+    // y <- Q y = H_0 H_1 ... H_{n-2} y
+    for (int j = n - 2; j >= 0; --j) {
+        cx<T> s = {T(0), T(0)};
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int r = g + G * q;
+            if (r >= j + 1 && r < n) {
+                const cx<T> v = (r == j + 1) ? cx<T>{T(1), T(0)} : L.A[r * S + j];
+                s = cadd(s, cmulc(y[q], v));
+            }
+        }
+        s = gsum<G>(s);
+        const cx<T> ts = cmul(L.TAU[j], s);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int r = g + G * q;
+            if (r >= j + 1 && r < n) {
+                const cx<T> v = (r == j + 1) ? cx<T>{T(1), T(0)} : L.A[r * S + j];
+                y[q] = csub(y[q], cmul(v, ts));
+            }
+        }
+    }
+    __syncthreads();
+    return lam;
+}
+
+template <typename T> __device__ inline cx<T> unit_or(cx<T> u, cx<T> fallback, bool& ok) {
+    const T a = sqrt(cabs2(u));
+    ok = a > T(0);
+    return ok ? cscale(u, T(1) / a) : fallback;
+}
+
+// ---------------------------------------------------------------------------
+template <typename T, typename Tg, int G, int R>
+__global__ __launch_bounds__(64) void solve_kernel(SolveArgs a) {
+    constexpr int GPW = 64 / G;
+    const int grp = threadIdx.x / G, g = threadIdx.x % G;
+    const int slot = blockIdx.x * GPW + grp;
+    const int n = a.n, S = n | 1;
+    const int MS = S * (int)(sizeof(cx<T>) / sizeof(double));
+    extern __shared__ __align__(16) char smem[];
+    SolveLds<T> L = solve_carve<T>(smem + grp * a.group_bytes, n);
+    double* M = (double*)L.A;
+
+    const bool inslot = slot < a.slots;
+    const int np = inslot ? a.npix[slot] : -1;
+    const bool act = np >= 0;
+    const cx<Tg>* gam = (const cx<Tg>*)a.gamma + (size_t)(inslot ? slot : 0) * n * n;
+    const long long pix = a.first + (inslot ? slot : 0);
+
+    float code = 0.0f;
+    bool failed = false;
+    cx<T> y[R];
+
+    if (a.module == FRINGE_MODULE_EVD && a.method == FRINGE_METHOD_MLE) {
+        // evd.cpp:584-688: lambda_min(Gamma) (-1/-2), lambda_min(|Gamma|) (-3/-4),
+        // inv(|Gamma|) (-5), smallest eigenpair of inv(|Gamma|) o Gamma (-6/-7).
+        const bool fin = load_gamma<T, Tg, G, R>(L.A, S, gam, n, act, -1);
+        tridiagonalize<T, G, R>(L, S, n);
+        T tn;
+        const T lmin = select_eigenvalue<T, G>(L, n, 0, tn);
+        if (!fin) { code = -1.0f; failed = true; }
+        else if (!(lmin >= T(1.0e-6))) { code = -2.0f; failed = true; }
+        load_abs_gamma<Tg, G, R>(M, MS, gam, n, act, 1.0e-6);
+        const bool pd_shift = sweep_inverse<G, R>(M, MS, n);
+        if (!failed && !pd_shift) { code = -4.0f; failed = true; }
+        load_abs_gamma<Tg, G, R>(M, MS, gam, n, act, 0.0);
+        const bool pd = sweep_inverse<G, R>(M, MS, n);
+        if (!failed && !pd) { code = -5.0f; failed = true; }
+        const bool hfin = form_hadamard<T, Tg, G, R>(L.A, S, M, MS, gam, n, act);
+        tridiagonalize<T, G, R>(L, S, n);
+        const T lam = extreme_eigenpair<T, G, R>(L, S, n, 0, y);
+        if (!failed && !hfin) { code = -6.0f; failed = true; }
+        if (!failed && !(lam >= T(1.0e-6))) { code = -7.0f; failed = true; }
+    } else if (a.module == FRINGE_MODULE_EVD) {
+        // EVD / STBAS (evd.cpp:689-732)
+        const int band = (a.method == FRINGE_METHOD_STBAS) ? a.bw : -1;
+        const bool fin = load_gamma<T, Tg, G, R>(L.A, S, gam, n, act, band);
+        tridiagonalize<T, G, R>(L, S, n);
+        const T lam = extreme_eigenpair<T, G, R>(L, S, n, n - 1, y);
+        if (!fin) { code = -6.0f; failed = true; }
+        else if (!(lam >= T(1.0e-6))) { code = -7.0f; failed = true; }
+    } else {
+        // phase_link EMI (phase_link.cpp:540-618)
+        bool fin = true;
+        load_abs_gamma<Tg, G, R>(M, MS, gam, n, act, 0.0);
+        {
+            // Gamma must be finite for the reference's first zheevr to succeed (-1)
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int r = g + G * q;
+                if (r >= n) continue;
+                for (int c = 0; c < n; ++c) fin = fin && isfinite(M[r * MS + c]);
+            }
+            fin = group_all<G>(fin);
+        }
+        const bool pd = sweep_inverse<G, R>(M, MS, n);
+        const bool hfin = form_hadamard<T, Tg, G, R>(L.A, S, M, MS, gam, n, act);
+        bool run_evd = !pd || !hfin;
+        tridiagonalize<T, G, R>(L, S, n);
+        extreme_eigenpair<T, G, R>(L, S, n, 0, y);
+        if (__any(act && fin && run_evd)) {
+            // EVD fallback: largest eigenvector of Gamma
+            load_gamma<T, Tg, G, R>(L.A, S, gam, n, act, -1);
+            tridiagonalize<T, G, R>(L, S, n);
+            cx<T> ye[R];
+            extreme_eigenpair<T, G, R>(L, S, n, n - 1, ye);
+            if (run_evd) {
+#pragma unroll
+                for (int q = 0; q < R; ++q) y[q] = ye[q];
+            }
+        }
+        if (!fin) { code = -1.0f; failed = true; }
+    }
+
+    // ---- referenced phases (evd.cpp:738-749) ------------------------------
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int r = g + G * q;
+        if (r < n) L.Y[r] = y[q];
+    }
+    __syncthreads();
+    const cx<T> vref = L.Y[a.ref];
+    bool okr;
+    const cx<T> cref = unit_or(cconj(vref), cx<T>{T(1), T(0)}, okr);
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int r = g + G * q;
+        if (r >= n) continue;
+        bool oku, okv;
+        const cx<T> u = unit_or(cmulc(y[q], vref), cx<T>{T(0), T(0)}, oku);
+        cx<T> o;
+        if (oku) o = u;
+        else if (okr) o = cref;
+        else o = unit_or(y[q], cx<T>{T(1), T(0)}, okv);
+        if (r == a.ref) o = cx<T>{T(1), T(0)};
+        L.OUT[r] = cx<float>{(float)o.re, (float)o.im};
+    }
+    __syncthreads();
+
+    const bool write_ok = act && !failed;
+    // ---- compressed SLC from the centre pixel (evd.cpp:755-762) ----------
+    cx<double> cs = {0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int r = g + G * q;
+        if (r >= a.ref && r < n && write_ok) {
+            const float2 z = a.slc[(long long)r * a.npx + pix];
+            const cx<float> o = L.OUT[r];
+            const float pr = z.x * o.re + z.y * o.im, pi = z.y * o.re - z.x * o.im;
+            cs.re += (double)pr;
+            cs.im += (double)pi;
+        }
+    }
+    cs = gsum<G>(cs);
+    // ---- temporal coherence (evd.cpp:770-786) -----------------------------
+    const bool stbas = (a.method == FRINGE_METHOD_STBAS);
+    cx<double> tc = {0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int i = g + G * q;
+        if (i >= n || !write_ok) continue;
+        const int ulim = stbas ? min(n, i + a.bw + 1) : n;
+        bool okx;
+        const cx<float> oi = L.OUT[i];
+        const cx<double> ci = unit_or(cx<double>{(double)oi.re, -(double)oi.im}, cx<double>{1.0, 0.0}, okx);
+        for (int j = i + 1; j < ulim; ++j) {
+            const cx<Tg> gv = gam[(size_t)i * n + j];
+            const cx<double> ph = unit_or(cx<double>{(double)gv.re, (double)gv.im}, cx<double>{1.0, 0.0}, okx);
+            const cx<float> oj = L.OUT[j];
+            const cx<double> cj = unit_or(cx<double>{(double)oj.re, (double)oj.im}, cx<double>{1.0, 0.0}, okx);
+            tc = cadd(tc, cmul(cmul(ph, ci), cj));
+        }
+    }
+    tc = gsum<G>(tc);
+
+    if (!inslot) return;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int r = g + G * q;
+        if (r >= n) continue;
+        const cx<float> o = write_ok ? L.OUT[r] : cx<float>{0.0f, 0.0f};
+        a.phase[(long long)r * a.npx + pix] = make_float2(o.re, o.im);
+    }
+    if (g == 0) {
+        long long counter = 0;
+        if (stbas) {
+            for (int i = 0; i < n; ++i) counter += min(n, i + a.bw + 1) - i - 1;
+        } else {
+            counter = (long long)n * (n - 1) / 2;
+        }
+        float tco = 0.0f;
+        float2 co = make_float2(0.0f, 0.0f);
+        if (act && failed) {
+            tco = code;
+        } else if (write_ok) {
+            tco = (float)(sqrt(tc.re * tc.re + tc.im * tc.im) / (double)counter);
+            const double inv = 1.0 / (double)(n - a.ref);
+            co = make_float2((float)(cs.re * inv), (float)(cs.im * inv));
+        }
+        a.tcorr[pix] = tco;
+        a.comp[pix] = co;
+    }
+}
+
+}  // namespace fringe

@@ -1,0 +1,120 @@
+"""Tile engine on device-resident buffers (torch tensors used as HBM storage).
+
+``link`` and ``nmap`` call the C ABI of libfringe_amd.so on a HIP stream;
+torch only allocates device memory and provides the stream.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+_DTYPE_U32 = torch.int32   # bitmask words are stored as int32 bit patterns
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream(stream):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed with code {rc}: {_lib.last_error()}")
+
+
+def mask_words(Nx, Ny):
+    return _lib.lib().fringe_mask_words(Nx, Ny)
+
+
+def ks_table(n, prob):
+    """(table[k] for k = 0..n, smallest |p_k - prob|)."""
+    tab = np.zeros(n + 1, np.uint8)
+    margin = _lib.lib().fringe_ks_table(n, prob, tab.ctypes.data)
+    return tab, margin
+
+
+def link_params(rows, cols, nbands, Nx, Ny, module="phase_link", method="MLE",
+                min_neighbors=2, ref_index=0, bandwidth=-1, precision="mixed",
+                row_begin=0, row_end=None):
+    t = _lib.LinkTile()
+    t.rows, t.cols, t.nbands, t.Nx, t.Ny = rows, cols, nbands, Nx, Ny
+    t.module = _lib.FRINGE_MODULE_EVD if module == "evd" else _lib.FRINGE_MODULE_PHASE_LINK
+    t.method = _lib.METHODS[method]
+    t.min_neighbors = min_neighbors
+    t.ref_index = ref_index
+    t.bandwidth = bandwidth
+    t.precision = _lib.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    t.row_begin = row_begin
+    t.row_end = rows if row_end is None else row_end
+    return t
+
+
+class LinkRunner:
+    """Preallocated outputs + workspace for repeated phase linking of one
+    tile shape (the request loop and the benchmark reuse it)."""
+
+    def __init__(self, params, device="cuda"):
+        self.p = params
+        L = _lib.lib()
+        self.work_bytes = L.fringe_link_workspace_bytes(ctypes.byref(params))
+        if self.work_bytes == 0:
+            raise RuntimeError("invalid or unsupported link tile parameters")
+        dev = torch.device(device)
+        n, r, c = params.nbands, params.rows, params.cols
+        self.work = torch.empty(self.work_bytes, dtype=torch.uint8, device=dev)
+        self.phase = torch.zeros((n, r, c), dtype=torch.complex64, device=dev)
+        self.tcorr = torch.zeros((r, c), dtype=torch.float32, device=dev)
+        self.comp = torch.zeros((r, c), dtype=torch.complex64, device=dev)
+
+    def run(self, slc, wts, stream=None):
+        assert slc.is_contiguous() and slc.dtype == torch.complex64
+        assert wts.is_contiguous() and wts.dtype == _DTYPE_U32
+        p = self.p
+        assert tuple(slc.shape) == (p.nbands, p.rows, p.cols)
+        assert wts.numel() == p.rows * p.cols * mask_words(p.Nx, p.Ny)
+        rc = _lib.lib().fringe_link_run(ctypes.byref(p), _ptr(slc), _ptr(wts), _ptr(self.phase),
+                                        _ptr(self.tcorr), _ptr(self.comp), _ptr(self.work),
+                                        self.work_bytes, _stream(stream))
+        _check(rc, "fringe_link_run")
+        return self.phase, self.tcorr, self.comp
+
+
+def link(slc, wts, Nx, Ny, **kw):
+    """Phase-link a device tile.  slc: complex64 [n, rows, cols] on the GPU;
+    wts: int32 [rows, cols, nulong] on the GPU.  Returns (phase, tcorr, comp)."""
+    n, rows, cols = slc.shape
+    runner = LinkRunner(link_params(rows, cols, n, Nx, Ny, **kw), device=slc.device)
+    return runner.run(slc, wts)
+
+
+def nmap(slc, Nx, Ny, prob=0.05, mask=None, amp_const=None, row_begin=0, row_end=None,
+         stream=None):
+    """KS2 SHP map of a device tile.  Returns (wts int32 [rows, cols, nulong],
+    count int32 [rows, cols])."""
+    n, rows, cols = slc.shape
+    t = _lib.NmapTile()
+    t.rows, t.cols, t.nbands, t.Nx, t.Ny = rows, cols, n, Nx, Ny
+    t.prob, t.method = prob, 0
+    t.row_begin, t.row_end = row_begin, rows if row_end is None else row_end
+    L = _lib.lib()
+    wb = L.fringe_nmap_workspace_bytes(ctypes.byref(t))
+    work = torch.empty(max(wb, 1), dtype=torch.uint8, device=slc.device)
+    nul = mask_words(Nx, Ny)
+    wts = torch.zeros((rows, cols, nul), dtype=_DTYPE_U32, device=slc.device)
+    cnt = torch.zeros((rows, cols), dtype=torch.int32, device=slc.device)
+    ac = None
+    if amp_const is not None:
+        ac = np.ascontiguousarray(amp_const, np.float64)
+    m = None
+    if mask is not None:
+        m = mask.to(device=slc.device, dtype=torch.uint8).contiguous()
+    rc = L.fringe_nmap_run(ctypes.byref(t), _ptr(slc.contiguous()), _ptr(m),
+                           None if ac is None else ctypes.c_void_p(ac.ctypes.data),
+                           _ptr(wts), _ptr(cnt), _ptr(work), wb, _stream(stream))
+    _check(rc, "fringe_nmap_run")
+    return wts, cnt

@@ -1,0 +1,180 @@
+/* fringe_amd.h -- C ABI of the MI355X phase-linking library (libfringe_amd.so).
+ *
+ * Two levels:
+ *   1. Module entry points that replace the reference's per-module
+ *      processors one for one (same options, defaults, files and return
+ *      codes), taking plain-old-data option structs instead of the
+ *      std::string-bearing evdOptions / nmapOptions.
+ *   2. A tile engine on device-resident buffers (HBM pointers plus a HIP
+ *      stream), used by the module entry points, the sequential estimator and
+ *      the request loop.
+ *
+ * All pointers named d_* are device pointers; h_* are host pointers.
+ * Streams are passed as void* (a hipStream_t; NULL selects the null stream).
+ */
+#ifndef FRINGE_AMD_H
+#define FRINGE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes --------------------------------------------------------
+ * 0 and 101..121 keep the meaning they have in the reference modules
+ * (src/evd/evd.cpp:54-857, src/phase_link/phase_link.cpp, src/nmap/nmap.cpp).
+ * 2xx are raised by this library only. */
+#define FRINGE_OK 0
+#define FRINGE_ERR_BANDWIDTH 101    /* STBAS bandwidth missing or too large */
+#define FRINGE_ERR_OPEN_INPUT 102   /* input stack cannot be opened */
+#define FRINGE_ERR_OPEN_WTS 105     /* evd: weights cannot be opened; nmap: cannot create wts */
+#define FRINGE_ERR_WIDTH 106
+#define FRINGE_ERR_LENGTH 107
+#define FRINGE_ERR_NULONG 108
+#define FRINGE_ERR_HALFX 109
+#define FRINGE_ERR_HALFY 110
+#define FRINGE_ERR_NMAP_WRITE 111
+#define FRINGE_ERR_DATE 112
+#define FRINGE_ERR_OUT_DIR_EXISTS 113
+#define FRINGE_ERR_OUT_IS_FILE 114
+#define FRINGE_ERR_MKDIR 115
+#define FRINGE_ERR_CREATE_SLC 116
+#define FRINGE_ERR_CREATE_AUX 117
+#define FRINGE_ERR_READ_SLC 118
+#define FRINGE_ERR_READ_WTS 119
+#define FRINGE_ERR_WRITE_SLC 120
+#define FRINGE_ERR_WRITE_AUX 121
+#define FRINGE_ERR_ARGUMENT 201     /* invalid tile geometry or option */
+#define FRINGE_ERR_UNSUPPORTED 202  /* e.g. more dates than the kernels hold */
+#define FRINGE_ERR_DEVICE 203       /* HIP runtime error */
+#define FRINGE_ERR_WORKSPACE 204    /* workspace too small */
+
+/* ---- estimator selection ------------------------------------------------- */
+#define FRINGE_MODULE_EVD 0         /* semantics of src/evd/evd.cpp */
+#define FRINGE_MODULE_PHASE_LINK 1  /* semantics of src/phase_link/phase_link.cpp */
+#define FRINGE_METHOD_MLE 0
+#define FRINGE_METHOD_EVD 1
+#define FRINGE_METHOD_STBAS 2
+#define FRINGE_PREC_FP64 0          /* FP64 Gram, FP64 eigensolver ("plain" path) */
+#define FRINGE_PREC_MIXED 1         /* fp16 hi/lo MFMA Gram (per-date power-of-two scaled),
+                                       FP32 eigensolver, FP64 |Gamma| inverse and refinement */
+#define FRINGE_PREC_MIXED_BF16 2    /* as MIXED with bf16 hi/lo samples (no scaling needed) */
+
+/* ---- module level: replaces evd_process (evd.cpp:16) and
+ *      evd_process (phase_link.cpp:17); fields follow evdOptions
+ *      (evd.hpp:19-68, phase_link.hpp:20-69). */
+typedef struct fringe_evd_options {
+    const char* inputDS;                   /* stack VRT with SLCs as bands */
+    const char* wtsDS;                     /* SHP bitmask written by nmap */
+    const char* compSlc;                   /* name of the compressed SLC */
+    const char* outputCompressedSlcFolder;
+    const char* outputFolder;
+    int blocksize;                         /* 64   */
+    int memsize;                           /* 2048 (MB) */
+    int Nx;                                /* 5 */
+    int Ny;                                /* 5 */
+    int minNeighbors;                      /* 2 */
+    int miniStackCount;                    /* 1 */
+    int bandWidth;                         /* -1 */
+    const char* method;                    /* "MLE" | "EVD" | "STBAS" */
+    int precision;                         /* FRINGE_PREC_* (extension; default MIXED) */
+    int verbose;                           /* print options and progress */
+} fringe_evd_options;
+
+void fringe_evd_options_init(fringe_evd_options* o);
+int fringe_evd_process(const fringe_evd_options* o);        /* evd.cpp:16 */
+int fringe_phase_link_process(const fringe_evd_options* o); /* phase_link.cpp:17 */
+
+/* ---- module level: replaces nmap_process (nmap.cpp:18); fields follow
+ *      nmapOptions (nmap.hpp:23-65). */
+typedef struct fringe_nmap_options {
+    const char* inputDS;
+    const char* maskDS;    /* "" or "None": no mask */
+    const char* wtsDS;     /* output bitmask (ENVI BIP UInt32, nulong bands) */
+    const char* ncountDS;  /* output count (ENVI Int16) */
+    const char* method;    /* "KS2" | "AD2" */
+    int noGPU;             /* accepted for option compatibility; this library only runs on the GPU */
+    int blocksize;         /* 64  */
+    int memsize;           /* 512 */
+    int Nx;                /* 5 */
+    int Ny;                /* 5 */
+    double prob;           /* 0.05 */
+    int verbose;
+} fringe_nmap_options;
+
+void fringe_nmap_options_init(fringe_nmap_options* o);
+int fringe_nmap_process(const fringe_nmap_options* o);      /* nmap.cpp:18 */
+
+/* ---- tile engine: phase linking ----------------------------------------
+ * A tile is rows x cols pixels of nbands SLCs, band-planar complex64
+ * d_slc[band][row][col], with the SHP bitmask d_wts[row][col][nulong]
+ * (nulong = ceil((2Ny+1)(2Nx+1)/32)).  Windows are truncated at the tile
+ * edges, so a tile cut from a larger image must carry Ny halo rows above and
+ * below the output rows [row_begin, row_end) (the reference's row blocks do
+ * the same, evd.cpp:399-443).  Every pixel of the output rows is written:
+ * d_phase[band][row][col] (complex64), d_tcorr[row][col], d_comp[row][col]
+ * (complex64); untouched pixels get zeros, failed pixels a negative code in
+ * tcorr (evd.cpp:606-728, phase_link.cpp:547,612). */
+typedef struct fringe_link_tile {
+    int rows, cols, nbands;
+    int Nx, Ny;
+    int module;          /* FRINGE_MODULE_* */
+    int method;          /* FRINGE_METHOD_* */
+    int min_neighbors;   /* honoured by phase_link; evd uses 2 (evd.cpp:566) */
+    int ref_index;       /* miniStackCount - 1 */
+    int bandwidth;       /* STBAS */
+    int precision;       /* FRINGE_PREC_* */
+    int row_begin, row_end;
+} fringe_link_tile;
+
+size_t fringe_link_workspace_bytes(const fringe_link_tile* t);
+int fringe_link_run(const fringe_link_tile* t, const void* d_slc, const uint32_t* d_wts,
+                    void* d_phase, float* d_tcorr, void* d_comp, void* d_work,
+                    size_t work_bytes, void* stream);
+
+/* ---- tile engine: SHP map (KS2 two-sample test) -------------------------
+ * d_mask (optional, uint8 [row][col]) marks usable pixels.  h_amp_const holds
+ * the per-band amplitudeConstant metadata (<= 0 or NULL: missing), normalised
+ * by band 1 as nmap.cpp:204-233 does.  Output rows [row_begin, row_end) of
+ * d_wts[row][col][nulong] and d_count[row][col] (int32) are written. */
+typedef struct fringe_nmap_tile {
+    int rows, cols, nbands;
+    int Nx, Ny;
+    double prob;
+    int method;          /* 0 = KS2, 1 = AD2 */
+    int row_begin, row_end;
+} fringe_nmap_tile;
+
+size_t fringe_nmap_workspace_bytes(const fringe_nmap_tile* t);
+int fringe_nmap_run(const fringe_nmap_tile* t, const void* d_slc, const uint8_t* d_mask,
+                    const double* h_amp_const, uint32_t* d_wts, int32_t* d_count,
+                    void* d_work, size_t work_bytes, void* stream);
+
+/* KS decision table used by the nmap kernel: h_table[k] = 1 when
+ * KolmogorovProb((k/n) sqrt(n/2)) >= prob, k = 0..n (KS2sample.hpp:91-144,
+ * nmap.cpp:462).  Returns the smallest distance |p_k - prob| over k, so a
+ * caller can see how far every decision is from the threshold. */
+double fringe_ks_table(int n, double prob, uint8_t* h_table);
+
+/* ---- stage timing (no counterpart in the reference; for profiling) ------
+ * When enabled, every engine stage (0 pack, 1 gram, 2 solve, 3 nmap_prep,
+ * 4 nmap_ks) is bracketed by HIP events on the stream it runs on.
+ * fringe_timing_read waits for the recorded events and returns the number
+ * of stages; ms[i] and calls[i] receive the accumulated time and count. */
+void fringe_timing_enable(int on);
+void fringe_timing_reset(void);
+int fringe_timing_read(double* ms, long long* calls, int max_stages);
+const char* fringe_timing_stage_name(int stage);
+
+/* ---- utilities ---------------------------------------------------------- */
+int fringe_mask_words(int Nx, int Ny);
+const char* fringe_version(void);
+/* Last HIP error text seen by the library on this thread (or ""). */
+const char* fringe_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FRINGE_AMD_H */

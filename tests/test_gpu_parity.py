@@ -1,0 +1,76 @@
+"""GPU parity: libfringe_amd.so (through its C ABI) against the CPU oracle.
+
+Sizes are small enough for the FP64 LAPACK oracle to finish in seconds.
+"""
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+from helpers import compare_link, wts_dev, to_dev, PHASE_TOL, TCORR_TOL
+
+pytestmark = pytest.mark.gpu
+
+if not gpu_available():
+    pytest.skip("needs a GPU", allow_module_level=True)
+
+import torch  # noqa: E402
+
+import oracle  # noqa: E402
+from fringe_amd import synthetic, tile  # noqa: E402
+
+
+def _run_both(slc, wts, Nx, Ny, module, method, precision, min_nb=2, ref=0, bw=-1, want_gap=True):
+    r = oracle.link(slc, wts, Nx, Ny, module=module, method=method, min_neighbors=min_nb,
+                    mini_stack_count=ref + 1, bandwidth=bw, want_gap=want_gap)
+    ph, tc, co = tile.link(to_dev(slc, torch), wts_dev(wts, torch), Nx, Ny, module=module,
+                           method=method, min_neighbors=min_nb, ref_index=ref, bandwidth=bw,
+                           precision=precision)
+    torch.cuda.synchronize()
+    g = (ph.cpu().numpy(), tc.cpu().numpy(), co.cpu().numpy())
+    return g, r
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("module,method", [("phase_link", "MLE"), ("evd", "EVD"), ("evd", "MLE")])
+def test_link_full_mask_small(precision, module, method):
+    n, rows, cols, Nx, Ny = 10, 20, 24, 3, 2
+    print(f"case n={n} {rows}x{cols} win {Nx}x{Ny} {module}/{method} {precision}", flush=True)
+    slc = synthetic.stack(n, rows, cols, seed=101)
+    wts = synthetic.full_mask(rows, cols, Nx, Ny)
+    g, r = _run_both(slc, wts, Nx, Ny, module, method, precision)
+    st = compare_link(g, r, r["gap"])
+    print(st)
+    assert st["untouched_equal"]
+    assert st["n_well"] > 0.9 * rows * cols
+    assert st["phase_max"] <= PHASE_TOL
+    assert st["tcorr_max"] <= TCORR_TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_link_n30_nmap_mask(precision):
+    n, rows, cols, Nx, Ny = 30, 24, 40, 7, 5
+    print(f"case n={n} nmap mask {precision}", flush=True)
+    slc = synthetic.stack(n, rows, cols, seed=102, patches=8)
+    wts, cnt = oracle.nmap(slc, Nx, Ny, 0.05)
+    g, r = _run_both(slc, wts, Nx, Ny, "phase_link", "MLE", precision, min_nb=5)
+    st = compare_link(g, r, r["gap"])
+    print(st)
+    assert st["untouched_equal"]
+    assert st["phase_max"] <= PHASE_TOL
+    assert st["tcorr_max"] <= TCORR_TOL
+
+
+@pytest.mark.parametrize("n", [8, 10, 30, 64])
+def test_nmap_bit_exact(n):
+    rows, cols, Nx, Ny = 20, 28, 5, 3
+    print(f"case nmap n={n}", flush=True)
+    slc = synthetic.stack(n, rows, cols, seed=200 + n, patches=6)
+    if n == 10:   # quantised amplitudes exercise the tie branch of the KS walk
+        slc = (np.round(np.abs(slc) * 4) / 4 * np.exp(1j * np.angle(slc))).astype(np.complex64)
+    wr, cr = oracle.nmap(slc, Nx, Ny, 0.05)
+    wg, cg = tile.nmap(to_dev(slc, torch), Nx, Ny, 0.05)
+    torch.cuda.synchronize()
+    wg = wg.cpu().numpy().view(np.uint32)
+    cg = cg.cpu().numpy()
+    assert np.array_equal(wg, wr)
+    assert np.array_equal(cg, cr)

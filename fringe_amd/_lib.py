@@ -23,6 +23,7 @@ EXPORTS = [
     "fringe_nmap_workspace_bytes", "fringe_nmap_run", "fringe_ks_table",
     "fringe_mask_words", "fringe_version", "fringe_last_error",
     "fringe_timing_enable", "fringe_timing_reset", "fringe_timing_read", "fringe_timing_stage_name",
+    "fringe_raster_info", "fringe_raster_metadata", "fringe_raster_read",
 ]
 
 
@@ -83,6 +84,10 @@ def lib():
     L.fringe_timing_read.argtypes = [vp, vp, ctypes.c_int]
     L.fringe_timing_stage_name.argtypes = [ctypes.c_int]
     L.fringe_timing_stage_name.restype = ctypes.c_char_p
+    L.fringe_raster_info.argtypes = [ctypes.c_char_p, vp, vp, vp]
+    L.fringe_raster_metadata.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p,
+                                         ctypes.c_char_p, ctypes.c_int]
+    L.fringe_raster_read.argtypes = [ctypes.c_char_p] + [ctypes.c_int] * 6 + [vp]
     for name, st in (("fringe_evd_options_init", EvdOptions), ("fringe_evd_process", EvdOptions),
                      ("fringe_phase_link_process", EvdOptions),
                      ("fringe_nmap_options_init", NmapOptions), ("fringe_nmap_process", NmapOptions)):

@@ -5,7 +5,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
-#include "../../include/fringe_amd.h"
+#include "fringe_amd.h"
 
 namespace fringe {
 

@@ -34,7 +34,7 @@
 
 #include "device_common.hpp"
 #include "engine_internal.hpp"
-#include "../../include/fringe_amd.h"
+#include "fringe_amd.h"
 
 namespace fringe {
 

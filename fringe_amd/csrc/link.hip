@@ -16,6 +16,7 @@
 #include "device_common.hpp"
 #include "engine_internal.hpp"
 #include "link_solve.hpp"
+#include "solve32.hpp"
 #include "timing.hpp"
 
 namespace fringe {
@@ -189,7 +190,10 @@ static void launch_solve(SolveArgs a, hipStream_t s) {
 
 template <typename T, typename Tg>
 static void dispatch_solve(const SolveArgs& a, hipStream_t s) {
-    if (a.n <= 32) launch_solve<T, Tg, 32, 1>(a, s);
+    if (a.n <= 32) {
+        const unsigned blocks = (unsigned)((a.slots + 1) / 2);
+        hipLaunchKernelGGL((solve32_kernel<T, Tg>), dim3(blocks), dim3(64), 0, s, a);
+    }
     else if (a.n <= 64) launch_solve<T, Tg, 64, 1>(a, s);
     else launch_solve<T, Tg, 64, 2>(a, s);
 }

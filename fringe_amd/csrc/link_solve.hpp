@@ -8,7 +8,7 @@
 // votes); per-pixel decisions are carried as predicates.
 #pragma once
 #include "device_common.hpp"
-#include "../../include/fringe_amd.h"
+#include "fringe_amd.h"
 
 namespace fringe {
 

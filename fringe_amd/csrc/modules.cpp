@@ -1,0 +1,447 @@
+// modules.cpp -- file-level module processors: drop-in replacements of
+// evd_process (src/evd/evd.cpp:16), evd_process (src/phase_link/phase_link.cpp:17)
+// and nmap_process (src/nmap/nmap.cpp:18).
+//
+// Inputs are opened by path (stack VRT, SHP bitmask, optional byte mask),
+// processed in the reference's memsize-driven row blocks with Ny rows of
+// overlap (evd.cpp:170-187, :399-443, :861-868), each block run on the GPU
+// by the tile engine, and written to the same ENVI rasters, with the same
+// return codes, as the reference.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fringe_amd.h"
+#include "raster.hpp"
+
+using fringe::io::DType;
+using fringe::io::EnviWriter;
+using fringe::io::Raster;
+
+namespace {
+
+const char* nz(const char* s) { return s ? s : ""; }
+
+int ceil_words(int Nx, int Ny) { return (int)std::ceil(((2 * Ny + 1) * (2 * Nx + 1)) / 32.0); }
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    bool alloc(size_t bytes) {
+        if (bytes <= n && p) return true;
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+        if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) return false;
+        n = bytes;
+        return true;
+    }
+    ~DevBuf() {
+        if (p) hipFree(p);
+    }
+};
+
+// The reference's row-block schedule: the first block writes [0, inysize-Ny)
+// and rolls back Ny rows, middle blocks write [Ny, inysize-Ny), the last
+// block writes [Ny, inysize), a single block writes everything.
+struct BlockPlan {
+    int yoff, inysize, first, lines, rollback;
+};
+
+int block_rows(int memsize, int blocksize, int cols, int rows, int per_pixel_units) {
+    int boxes = (int)((memsize * 1.0e6) / cols) / (blocksize * per_pixel_units);
+    int b = boxes * blocksize;
+    if (b < blocksize) b = blocksize;
+    if (b > rows) b = rows;
+    return b;
+}
+
+bool next_block(int& yoff, int& count, int blockysize, int rows, int Ny, BlockPlan& bp) {
+    if (yoff >= rows) return false;
+    ++count;
+    bp.yoff = yoff;
+    bp.inysize = std::min(blockysize, rows - yoff);
+    if (count == 1) {
+        bp.first = 0;
+        bp.lines = bp.inysize - Ny;
+        bp.rollback = Ny;
+        if (yoff + blockysize >= rows) {
+            bp.lines = bp.inysize;
+            bp.rollback = 0;
+        }
+    } else if (yoff + blockysize >= rows) {
+        bp.first = Ny;
+        bp.lines = bp.inysize - Ny;
+        bp.rollback = 0;
+    } else {
+        bp.first = Ny;
+        bp.lines = bp.inysize - 2 * Ny;
+        bp.rollback = 0;
+    }
+    return true;
+}
+
+void advance(int& yoff, int blockysize, int rows, const BlockPlan& bp) {
+    if (yoff + blockysize < rows) yoff += bp.lines - bp.rollback;
+    else yoff = rows;
+}
+
+int link_module(const fringe_evd_options* o, int module) {
+    const bool verbose = o->verbose != 0;
+    const std::string method = nz(o->method);
+    const int Nx = o->Nx, Ny = o->Ny;
+    if (verbose) {
+        std::printf("Input: %s\nWeights: %s\nOutput folder: %s\nCompressed SLC: %s/%s\n", nz(o->inputDS),
+                    nz(o->wtsDS), nz(o->outputFolder), nz(o->outputCompressedSlcFolder), nz(o->compSlc));
+        std::printf("Method: %s  half window %d x %d  minNeighbors %d  miniStackCount %d  bandWidth %d\n",
+                    method.c_str(), Nx, Ny, o->minNeighbors, o->miniStackCount, o->bandWidth);
+    }
+    int meth;
+    if (method == "MLE") meth = FRINGE_METHOD_MLE;
+    else if (method == "EVD") meth = FRINGE_METHOD_EVD;
+    else if (method == "STBAS") meth = FRINGE_METHOD_STBAS;
+    else {
+        std::printf("Unknown method %s\n", method.c_str());
+        return FRINGE_ERR_ARGUMENT;
+    }
+    const int nulong = ceil_words(Nx, Ny);
+
+    std::string err;
+    auto in = fringe::io::open_raster(nz(o->inputDS), &err);
+    if (!in) {
+        std::printf("Cannot open stack file { %s } (%s)\nExiting with error code .... (102)\n", nz(o->inputDS), err.c_str());
+        return FRINGE_ERR_OPEN_INPUT;
+    }
+    const int cols = in->xsize, rows = in->ysize, nbands = (int)in->bands.size();
+    if (verbose) std::printf("Number of rows  = %d\nNumber of cols  = %d\nNumber of bands = %d\n", rows, cols, nbands);
+
+    if (meth == FRINGE_METHOD_STBAS && (o->bandWidth <= 0 || o->bandWidth >= nbands - 1)) {
+        std::printf("Requested STBAS bandwidth %d is not in [1, %d]\n", o->bandWidth, nbands - 2);
+        return FRINGE_ERR_BANDWIDTH;
+    }
+
+    auto wts = fringe::io::open_raster(nz(o->wtsDS), &err);
+    if (!wts) {
+        std::printf("Could not open weights dataset {%s}\nExiting with non-zero error code ... 105\n", nz(o->wtsDS));
+        return FRINGE_ERR_OPEN_WTS;
+    }
+    {
+        int code = 0;
+        if (wts->xsize != cols) code = FRINGE_ERR_WIDTH;
+        if (wts->ysize != rows) code = FRINGE_ERR_LENGTH;
+        if ((int)wts->bands.size() != nulong) code = FRINGE_ERR_NULONG;
+        const int inNx = std::atoi(wts->metadata_item("HALFWINDOWX", "ENVI").c_str());
+        const int inNy = std::atoi(wts->metadata_item("HALFWINDOWY", "ENVI").c_str());
+        if (inNx != Nx || inNx == 0) code = FRINGE_ERR_HALFX;
+        if (inNy != Ny || inNy == 0) code = FRINGE_ERR_HALFY;
+        if (code != 0) {
+            std::printf("Weights do not match the stack or the window (HALFWINDOWX=%d HALFWINDOWY=%d)\n"
+                        "Exiting with error code ....(%d)\n", inNx, inNy, code);
+            return code;
+        }
+    }
+
+    int blockysize = block_rows(o->memsize, o->blocksize, cols, rows, nbands * 20 + 4 + nulong);
+    // a block must be taller than its two halo bands, or the schedule stalls
+    if (blockysize < rows && blockysize <= 2 * Ny) blockysize = std::min(rows, 2 * Ny + 1);
+    if (verbose) std::printf("Block size = %d lines\n", blockysize);
+
+    std::vector<std::string> dates(nbands);
+    for (int b = 0; b < nbands; ++b) {
+        dates[b] = in->bands[b].metadata_item("Date", "slc");
+        if (dates[b].size() != 8) {
+            std::printf("Band %d does not appear to have Date information in slc metadata domain\n", b + 1);
+            return FRINGE_ERR_DATE;
+        }
+    }
+    {
+        bool isdir = false;
+        if (fringe::io::path_exists(nz(o->outputFolder), &isdir)) {
+            std::printf("Output folder : %s already exists\n", nz(o->outputFolder));
+            return isdir ? FRINGE_ERR_OUT_DIR_EXISTS : FRINGE_ERR_OUT_IS_FILE;
+        }
+        if (!fringe::io::make_dir(nz(o->outputFolder))) {
+            std::printf("Could not create output folder: %s\n", nz(o->outputFolder));
+            return FRINGE_ERR_MKDIR;
+        }
+    }
+    std::vector<EnviWriter> outs(nbands);
+    for (int b = 0; b < nbands; ++b) {
+        const std::string fname = fringe::io::join_path(nz(o->outputFolder), dates[b] + ".slc");
+        if (!outs[b].create(fname, cols, rows, 1, DType::CFloat32)) {
+            std::printf("Could not create output SLC: %s\n", fname.c_str());
+            return FRINGE_ERR_CREATE_SLC;
+        }
+    }
+    EnviWriter corr, comp;
+    if (!corr.create(fringe::io::join_path(nz(o->outputFolder), "tcorr.bin"), cols, rows, 1, DType::Float32)) {
+        std::printf("Could not create temporal correlation file\n");
+        return FRINGE_ERR_CREATE_AUX;
+    }
+    const std::string compname = fringe::io::join_path(nz(o->outputCompressedSlcFolder), nz(o->compSlc));
+    if (!comp.create(compname, cols, rows, 1, DType::CFloat32)) {
+        std::printf("Could not create compressed SLC file: %s\n", compname.c_str());
+        return FRINGE_ERR_CREATE_AUX;
+    }
+
+    // device buffers sized for the tallest block
+    const size_t px = (size_t)blockysize * cols;
+    DevBuf d_slc, d_wts, d_ph, d_tc, d_co, d_work;
+    fringe_link_tile t{};
+    t.rows = blockysize;
+    t.cols = cols;
+    t.nbands = nbands;
+    t.Nx = Nx;
+    t.Ny = Ny;
+    t.module = module;
+    t.method = meth;
+    t.min_neighbors = o->minNeighbors;
+    t.ref_index = o->miniStackCount - 1;
+    t.bandwidth = o->bandWidth;
+    t.precision = o->precision;
+    t.row_begin = 0;
+    t.row_end = blockysize;
+    const size_t wb = fringe_link_workspace_bytes(&t);
+    if (wb == 0 || !d_slc.alloc(px * nbands * 8) || !d_wts.alloc(px * nulong * 4) || !d_ph.alloc(px * nbands * 8) ||
+        !d_tc.alloc(px * 4) || !d_co.alloc(px * 8) || !d_work.alloc(wb)) {
+        std::printf("Cannot set up the GPU tile engine (%s)\n", fringe_last_error());
+        return wb == 0 ? FRINGE_ERR_UNSUPPORTED : FRINGE_ERR_DEVICE;
+    }
+    hipStream_t stream;
+    if (hipStreamCreate(&stream) != hipSuccess) return FRINGE_ERR_DEVICE;
+
+    std::vector<std::complex<float>> h_slc(px * nbands), h_ph(px * nbands), h_co(px), row_buf;
+    std::vector<uint32_t> h_wts(px * nulong), band_buf(px);
+    std::vector<float> h_tc(px);
+    int yoff = 0, count = 0, status = 0;
+    BlockPlan bp;
+    while (status == 0 && next_block(yoff, count, blockysize, rows, Ny, bp)) {
+        const size_t bpx = (size_t)bp.inysize * cols;
+        for (int b = 0; b < nbands && status == 0; ++b)
+            if (!fringe::io::read_window(*in, b + 1, 0, bp.yoff, cols, bp.inysize, h_slc.data() + (size_t)b * bpx))
+                status = FRINGE_ERR_READ_SLC;
+        for (int w = 0; w < nulong && status == 0; ++w) {
+            if (!fringe::io::read_window(*wts, w + 1, 0, bp.yoff, cols, bp.inysize, band_buf.data())) {
+                status = FRINGE_ERR_READ_WTS;
+                break;
+            }
+            for (size_t p = 0; p < bpx; ++p) h_wts[p * nulong + w] = band_buf[p];
+        }
+        if (status) break;
+        t.rows = bp.inysize;
+        t.row_begin = bp.first;
+        t.row_end = bp.first + bp.lines;
+        if (hipMemcpyAsync(d_slc.p, h_slc.data(), bpx * nbands * 8, hipMemcpyHostToDevice, stream) != hipSuccess ||
+            hipMemcpyAsync(d_wts.p, h_wts.data(), bpx * nulong * 4, hipMemcpyHostToDevice, stream) != hipSuccess) {
+            status = FRINGE_ERR_DEVICE;
+            break;
+        }
+        const int rc = fringe_link_run(&t, d_slc.p, (const uint32_t*)d_wts.p, d_ph.p, (float*)d_tc.p, d_co.p,
+                                       d_work.p, d_work.n, stream);
+        if (rc != 0) {
+            std::printf("tile engine failed (%d): %s\n", rc, fringe_last_error());
+            status = rc;
+            break;
+        }
+        if (hipMemcpyAsync(h_ph.data(), d_ph.p, bpx * nbands * 8, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+            hipMemcpyAsync(h_tc.data(), d_tc.p, bpx * 4, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+            hipMemcpyAsync(h_co.data(), d_co.p, bpx * 8, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+            hipStreamSynchronize(stream) != hipSuccess) {
+            status = FRINGE_ERR_DEVICE;
+            break;
+        }
+        const size_t off = (size_t)bp.first * cols;
+        for (int b = 0; b < nbands && status == 0; ++b)
+            if (!outs[b].write_rows(bp.yoff + bp.first, bp.lines, h_ph.data() + (size_t)b * bpx + off))
+                status = FRINGE_ERR_WRITE_SLC;
+        if (status == 0 && (!corr.write_rows(bp.yoff + bp.first, bp.lines, h_tc.data() + off) ||
+                            !comp.write_rows(bp.yoff + bp.first, bp.lines, h_co.data() + off)))
+            status = FRINGE_ERR_WRITE_AUX;
+        advance(yoff, blockysize, rows, bp);
+        if (verbose) std::printf("\rprogress %5.1f%%", 100.0 * yoff / rows), std::fflush(stdout);
+    }
+    if (verbose) std::printf("\n");
+    hipStreamDestroy(stream);
+    return status;
+}
+
+}  // namespace
+
+extern "C" {
+
+void fringe_evd_options_init(fringe_evd_options* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->inputDS = "";
+    o->wtsDS = "";
+    o->compSlc = "compslc.bin";
+    o->outputCompressedSlcFolder = "";
+    o->outputFolder = "";
+    o->blocksize = 64;
+    o->memsize = 2048;
+    o->Nx = 5;
+    o->Ny = 5;
+    o->minNeighbors = 2;
+    o->miniStackCount = 1;
+    o->bandWidth = -1;
+    o->method = "MLE";
+    o->precision = FRINGE_PREC_MIXED;
+    o->verbose = 1;
+}
+
+int fringe_evd_process(const fringe_evd_options* o) {
+    return o ? link_module(o, FRINGE_MODULE_EVD) : FRINGE_ERR_ARGUMENT;
+}
+
+int fringe_phase_link_process(const fringe_evd_options* o) {
+    return o ? link_module(o, FRINGE_MODULE_PHASE_LINK) : FRINGE_ERR_ARGUMENT;
+}
+
+void fringe_nmap_options_init(fringe_nmap_options* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->inputDS = "";
+    o->maskDS = "";
+    o->wtsDS = "";
+    o->ncountDS = "";
+    o->method = "KS2";
+    o->noGPU = 0;
+    o->blocksize = 64;
+    o->memsize = 512;
+    o->Nx = 5;
+    o->Ny = 5;
+    o->prob = 0.05;
+    o->verbose = 1;
+}
+
+int fringe_nmap_process(const fringe_nmap_options* o) {
+    if (!o) return FRINGE_ERR_ARGUMENT;
+    const bool verbose = o->verbose != 0;
+    const std::string method = nz(o->method);
+    int meth;
+    if (method == "KS2") meth = 0;
+    else if (method == "AD2") meth = 1;
+    else {
+        std::printf("Statistics method can be KS2 or AD2\nUnknown method: %s\n", method.c_str());
+        return 1;   // nmap.cpp:36-42
+    }
+    const int Nx = o->Nx, Ny = o->Ny;
+    const int nulong = ceil_words(Nx, Ny);
+    std::string err;
+    auto in = fringe::io::open_raster(nz(o->inputDS), &err);
+    if (!in) {
+        std::printf("Cannot open stack file { %s } (%s)\nExiting with error code .... (102)\n", nz(o->inputDS), err.c_str());
+        return FRINGE_ERR_OPEN_INPUT;
+    }
+    const int cols = in->xsize, rows = in->ysize, nbands = (int)in->bands.size();
+    std::shared_ptr<Raster> msk;
+    const std::string mname = nz(o->maskDS);
+    if (!mname.empty() && strcasecmp(mname.c_str(), "None") != 0) {
+        msk = fringe::io::open_raster(mname, &err);
+        if (!msk) {
+            std::printf("Cannot open mask file { %s }\nExiting with error code .... (102)\n", mname.c_str());
+            return FRINGE_ERR_OPEN_INPUT;
+        }
+        int code = 0;
+        if (msk->xsize != cols) code = 104;
+        if (msk->ysize != rows) code = 105;
+        if (msk->bands.size() != 1) code = 106;
+        if (code) {
+            std::printf("Mask file does not match the stack\nExiting with error code .... (%d)\n", code);
+            return code;
+        }
+    }
+    int blockysize = block_rows(o->memsize, o->blocksize, cols, rows, 4 * (nbands + 2 + nulong));
+    if (blockysize < rows && blockysize <= 2 * Ny) blockysize = std::min(rows, 2 * Ny + 1);
+
+    std::vector<double> aconst(nbands, 0.0);
+    for (int b = 0; b < nbands; ++b) {
+        const std::string s = in->bands[b].metadata_item("amplitudeConstant", "slc");
+        aconst[b] = s.empty() ? 0.0 : std::atof(s.c_str());
+    }
+    EnviWriter wcount, wwts;
+    wcount.meta["HALFWINDOWX"] = std::to_string(Nx);
+    wcount.meta["HALFWINDOWY"] = std::to_string(Ny);
+    wwts.meta = wcount.meta;
+    if (!wcount.create(nz(o->ncountDS), cols, rows, 1, DType::Int16)) {
+        std::printf("Could not create count dataset {%s}\n", nz(o->ncountDS));
+        return 104;
+    }
+    if (!wwts.create(nz(o->wtsDS), cols, rows, nulong, DType::UInt32)) {
+        std::printf("Could not create weights dataset {%s}\n", nz(o->wtsDS));
+        return 105;
+    }
+    if (meth == 1) {
+        std::printf("AD2 is not available on the GPU in this build\n");
+        return FRINGE_ERR_UNSUPPORTED;
+    }
+    fringe_nmap_tile t{};
+    t.rows = blockysize;
+    t.cols = cols;
+    t.nbands = nbands;
+    t.Nx = Nx;
+    t.Ny = Ny;
+    t.prob = o->prob;
+    t.method = meth;
+    const size_t px = (size_t)blockysize * cols;
+    DevBuf d_slc, d_mask, d_wts, d_cnt, d_work;
+    const size_t wb = fringe_nmap_workspace_bytes(&t);
+    if (wb == 0 || !d_slc.alloc(px * nbands * 8) || !d_mask.alloc(px) || !d_wts.alloc(px * nulong * 4) ||
+        !d_cnt.alloc(px * 4) || !d_work.alloc(wb))
+        return FRINGE_ERR_DEVICE;
+    hipStream_t stream;
+    if (hipStreamCreate(&stream) != hipSuccess) return FRINGE_ERR_DEVICE;
+    std::vector<std::complex<float>> h_slc(px * nbands);
+    std::vector<uint8_t> h_mask(px);
+    std::vector<uint32_t> h_wts(px * nulong);
+    std::vector<int32_t> h_cnt(px);
+    std::vector<int16_t> cnt16(px);
+    int yoff = 0, count = 0, status = 0;
+    BlockPlan bp;
+    while (status == 0 && next_block(yoff, count, blockysize, rows, Ny, bp)) {
+        const size_t bpx = (size_t)bp.inysize * cols;
+        if (msk && !fringe::io::read_window(*msk, 1, 0, bp.yoff, cols, bp.inysize, h_mask.data())) status = 108;
+        for (int b = 0; b < nbands && status == 0; ++b)
+            if (!fringe::io::read_window(*in, b + 1, 0, bp.yoff, cols, bp.inysize, h_slc.data() + (size_t)b * bpx))
+                status = 108;
+        if (status) break;
+        t.rows = bp.inysize;
+        t.row_begin = bp.first;
+        t.row_end = bp.first + bp.lines;
+        if (hipMemcpyAsync(d_slc.p, h_slc.data(), bpx * nbands * 8, hipMemcpyHostToDevice, stream) != hipSuccess ||
+            (msk && hipMemcpyAsync(d_mask.p, h_mask.data(), bpx, hipMemcpyHostToDevice, stream) != hipSuccess)) {
+            status = FRINGE_ERR_DEVICE;
+            break;
+        }
+        const int rc = fringe_nmap_run(&t, d_slc.p, msk ? (const uint8_t*)d_mask.p : nullptr, aconst.data(),
+                                       (uint32_t*)d_wts.p, (int32_t*)d_cnt.p, d_work.p, d_work.n, stream);
+        if (rc != 0) {
+            status = rc;
+            break;
+        }
+        if (hipMemcpyAsync(h_wts.data(), d_wts.p, bpx * nulong * 4, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+            hipMemcpyAsync(h_cnt.data(), d_cnt.p, bpx * 4, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+            hipStreamSynchronize(stream) != hipSuccess) {
+            status = FRINGE_ERR_DEVICE;
+            break;
+        }
+        const size_t off = (size_t)bp.first * cols;
+        for (size_t p = 0; p < (size_t)bp.lines * cols; ++p) {
+            const int32_t v = h_cnt[off + p];   // Int32 buffer written to an Int16 file (nmap.cpp:535-539)
+            cnt16[p] = (int16_t)(v > 32767 ? 32767 : v < -32768 ? -32768 : v);
+        }
+        if (!wcount.write_rows(bp.yoff + bp.first, bp.lines, cnt16.data()) ||
+            !wwts.write_rows(bp.yoff + bp.first, bp.lines, h_wts.data() + off * nulong))
+            status = 111;
+        advance(yoff, blockysize, rows, bp);
+        if (verbose) std::printf("\rprogress %5.1f%%", 100.0 * yoff / rows), std::fflush(stdout);
+    }
+    if (verbose) std::printf("\n");
+    hipStreamDestroy(stream);
+    return status;
+}
+
+}  // extern "C"

@@ -1,0 +1,463 @@
+// solve32.hpp -- register-resident per-pixel estimator for n <= 32 dates.
+//
+// A wave holds two pixels; lane r of a 32-lane half owns row r of every
+// matrix of its pixel in registers.  LDS (about 3 KB per pixel) is used only
+// to publish one row or one vector to the other lanes, so many waves fit on
+// a CU and the arithmetic is not serialised behind LDS round trips.
+//
+// Stages (same semantics as the generic LDS solver in link_solve.hpp):
+//   * inverse of the real |Gamma| by a Gauss-Jordan sweep in FP64.  The
+//     sweep keeps the invariant M_rc = s_r s_c M_cr with s_k = -1 for pivots
+//     already swept, so the multiplier M_rj a lane needs at step j is read
+//     from the published pivot row instead of from a runtime register index;
+//   * H = |Gamma|^-1 o Gamma (or Gamma itself for EVD) into registers;
+//   * Householder tridiagonalisation: the owner of row j publishes it, the
+//     reflector v_j = conj(row j) * scal_j is applied, and row j is never
+//     touched again, so it still holds v_j for the back-transformation;
+//   * eigenvalue by 33-section of Sturm counts, eigenvector of T by inverse
+//     iteration (every lane runs the same pivoted elimination chain on
+//     broadcast operands), back-transformation by the stored reflectors;
+//   * referenced phases, compressed SLC and temporal coherence.
+#pragma once
+#include "link_solve.hpp"
+
+namespace fringe {
+
+template <typename T> struct Lds32 {
+    cx<T> R[32];      // published row
+    cx<T> V[32];
+    cx<T> W[32];
+    cx<T> TAU[32];
+    cx<T> SCAL[32];
+    cx<T> YREF;
+    T D[32], E[32], E2[32];
+    T B[32];          // inverse-iteration right-hand side
+    T FP[32], FQ[32], FS[32], FX[32];   // eliminated rows
+    double TR[32];    // sweep: scaled pivot row
+    double PV;        // sweep: pivot
+    cx<float> OUT[32];
+};
+
+__device__ inline void wsync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T> __device__ inline T fast_rcp(T x) { return T(1) / x; }
+template <> __device__ inline float fast_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// ---- FP64 sweep inverse of |Gamma| - shift*I, rows in registers ----------
+// m[c] (c != r) holds M_rc, dr holds M_rr; m[r] is a free slot.
+template <typename Tg>
+__device__ inline void abs_gamma_row(double (&m)[32], double& dr, const cx<Tg>* grow, int n, int r,
+                                     bool act, double shift) {
+#pragma unroll
+    for (int c = 0; c < 32; ++c) {
+        double v = 0.0;
+        if (act && c < n && r < n && c != r) {
+            const cx<Tg> g = grow[c];
+            v = sqrt((double)g.re * (double)g.re + (double)g.im * (double)g.im);
+        }
+        m[c] = v;
+    }
+    dr = 1.0 - shift;
+}
+
+template <typename T>
+__device__ inline bool sweep32(Lds32<T>& L, double (&m)[32], double& dr, int n, int r) {
+    bool ok = true;
+    for (int j = 0; j < n; ++j) {
+        if (r == j) {
+            const double pv = 1.0 / dr;
+#pragma unroll
+            for (int c = 0; c < 32; ++c) L.TR[c] = m[c] * pv;
+            L.TR[j] = 1.0 + pv;
+            L.PV = dr;
+        }
+        wsync();
+        const double p = L.PV;
+        ok = ok && (p > 0.0);
+        const double piv = 1.0 / p;
+        const double trr = L.TR[r];
+        const double f = (r < j ? -trr : trr) * p;       // M_rj = s_r M_jr
+        const bool me = (r == j);
+        const double al = me ? 0.0 : 1.0, be = me ? 1.0 : -f;
+#pragma unroll
+        for (int c = 0; c < 32; ++c) m[c] = fma(be, L.TR[c], al * m[c]);
+        dr = me ? piv : fma(-f, trr, dr);
+        wsync();
+    }
+    return ok;
+}
+
+// ---- Householder tridiagonalisation of the rows in h ---------------------
+template <typename T, int C0>
+__device__ inline void tri_step(Lds32<T>& L, cx<T> (&h)[32], int j, int n, int r) {
+    if (r == j) {
+#pragma unroll
+        for (int c = 0; c < 32; ++c) L.R[c] = h[c];
+    }
+    wsync();
+    const cx<T> alpha = cconj(L.R[j + 1]);
+    const cx<T> xr = (r > j + 1 && r < n) ? cconj(L.R[r]) : cx<T>{T(0), T(0)};
+    const T xn2 = gsum<32>(cabs2(xr));
+    const T ar = alpha.re, ai = alpha.im;
+    cx<T> tau = {T(0), T(0)}, scal = {T(0), T(0)};
+    T beta = ar;
+    if (xn2 != T(0) || ai != T(0)) {
+        const T nrm = sqrt(ar * ar + ai * ai + xn2);
+        beta = (ar >= T(0)) ? -nrm : nrm;
+        tau = cx<T>{(beta - ar) / beta, -ai / beta};
+        scal = crecip(cx<T>{ar - beta, ai});
+    }
+    const cx<T> vr = (r == j + 1) ? cx<T>{T(1), T(0)} : (r > j + 1 && r < n) ? cmul(xr, scal) : cx<T>{T(0), T(0)};
+    L.V[r] = vr;
+    if (r == 0) {
+        L.D[j] = L.R[j].re;
+        L.E[j] = beta;
+        L.TAU[j] = tau;
+        L.SCAL[j] = scal;
+    }
+    wsync();
+    cx<T> p = {T(0), T(0)};
+#pragma unroll
+    for (int c = C0; c < 32; ++c) cfma(p, h[c], L.V[c]);
+    p = cmul(tau, p);
+    const cx<T> s = gsum<32>(cmulc(vr, p));
+    const cx<T> a2 = cscale(cmul(tau, s), T(-0.5));
+    const cx<T> wr = (r > j && r < n) ? cadd(p, cmul(a2, vr)) : cx<T>{T(0), T(0)};
+    L.W[r] = wr;
+    wsync();
+#pragma unroll
+    for (int c = C0; c < 32; ++c) {
+        const cx<T> vc = L.V[c], wc = L.W[c];
+        h[c] = csub(h[c], cmulc(vr, wc));
+        h[c] = csub(h[c], cmulc(wr, vc));
+    }
+    wsync();
+}
+
+template <typename T>
+__device__ inline void tridiag32(Lds32<T>& L, cx<T> (&h)[32], int n, int r) {
+    for (int j = 0; j < n - 1; ++j) {
+        // columns below j+1 carry zero reflector entries; skip whole octets
+        if (j < 7) tri_step<T, 0>(L, h, j, n, r);
+        else if (j < 15) tri_step<T, 8>(L, h, j, n, r);
+        else if (j < 23) tri_step<T, 16>(L, h, j, n, r);
+        else tri_step<T, 24>(L, h, j, n, r);
+    }
+    // last diagonal entry
+    if (r == n - 1) {
+#pragma unroll
+        for (int c = 0; c < 32; ++c) L.R[c] = h[c];
+    }
+    wsync();
+    if (r == 0) L.D[n - 1] = L.R[n - 1].re;
+    wsync();
+}
+
+// ---- eigenvalue k (0-based ascending) of tridiag(D, E) --------------------
+template <typename T>
+__device__ inline T bisect32(Lds32<T>& L, int n, int k, int r, T& tnorm) {
+    T lo = T(3.0e38), hi = T(-3.0e38), em2 = T(0), tn = T(0);
+    if (r < n) {
+        const T a = (r > 0) ? fabs(L.E[r - 1]) : T(0);
+        const T b = (r < n - 1) ? fabs(L.E[r]) : T(0);
+        lo = L.D[r] - a - b;
+        hi = L.D[r] + a + b;
+        tn = fabs(L.D[r]) + a + b;
+        if (r < n - 1) {
+            L.E2[r] = L.E[r] * L.E[r];
+            em2 = L.E2[r];
+        }
+    }
+    lo = gmin<32>(lo);
+    hi = gmax<32>(hi);
+    em2 = gmax<32>(em2);
+    tn = gmax<32>(tn);
+    wsync();
+    tnorm = tn;
+    const T pivmin = Lim<T>::tiny() * max(T(1), em2);
+    const T pad = T(4) * Lim<T>::eps() * tn + T(2) * pivmin;
+    lo -= pad;
+    hi += pad;
+    for (int it = 0; it < Lim<T>::iters(32); ++it) {
+        const T w = hi - lo;
+        const T sig = lo + (w * T(r + 1)) / T(33);
+        T q = L.D[0] - sig;
+        if (fabs(q) < pivmin) q = -pivmin;
+        int cnt = q < T(0);
+        for (int i = 1; i < n; ++i) {
+            q = (L.D[i] - sig) - L.E2[i - 1] * fast_rcp(q);
+            if (fabs(q) < pivmin) q = -pivmin;
+            cnt += q < T(0);
+        }
+        const uint64_t b = group_ballot<32>(cnt > k);
+        const int f = b ? (int)__builtin_ctzll(b) : 32;
+        const T nhi = (f < 32) ? lo + (w * T(f + 1)) / T(33) : hi;
+        const T nlo = (f > 0) ? lo + (w * T(f)) / T(33) : lo;
+        lo = nlo;
+        hi = nhi;
+    }
+    return T(0.5) * (lo + hi);
+}
+
+// ---- eigenvector of tridiag(D, E) for lam: 2 rounds of inverse iteration --
+// Every lane of the half runs the same elimination chain on broadcast
+// operands; lane 0 records the eliminated rows for the back substitution,
+// and lane i keeps component i of the result.
+template <typename T>
+__device__ inline T tri_eigvec32(Lds32<T>& L, int n, int r, T lam, T tnorm) {
+    const T pert = max(Lim<T>::eps() * tnorm, Lim<T>::tiny());
+    L.B[r] = T(0.5) + T((r * 37 + 11) & 63) * T(1.0 / 64.0);
+    wsync();
+    T xme = T(0);
+    for (int round = 0; round < 2; ++round) {
+        T cd = L.D[0] - lam;
+        T cu = (n > 1) ? L.E[0] : T(0);
+        T cx_ = L.B[0];
+        for (int i = 0; i < n - 1; ++i) {
+            const T e = L.E[i];
+            const T dn = L.D[i + 1] - lam;
+            const T un = (i + 1 < n - 1) ? L.E[i + 1] : T(0);
+            const T bn = L.B[i + 1];
+            const bool keep = fabs(cd) >= fabs(e);
+            T pv, qv, sv, xv, ncd, ncu, ncx;
+            if (keep) {
+                const T cdp = (fabs(cd) < pert) ? (cd < T(0) ? -pert : pert) : cd;
+                const T f = e * fast_rcp(cdp);
+                pv = cdp; qv = cu; sv = T(0); xv = cx_;
+                ncd = dn - f * cu; ncu = un; ncx = bn - f * cx_;
+            } else {
+                const T f = cd * fast_rcp(e);
+                pv = e; qv = dn; sv = un; xv = bn;
+                ncd = cu - f * dn; ncu = -f * un; ncx = cx_ - f * bn;
+            }
+            if (r == 0) {
+                L.FP[i] = pv;
+                L.FQ[i] = qv;
+                L.FS[i] = sv;
+                L.FX[i] = xv;
+            }
+            cd = ncd;
+            cu = ncu;
+            cx_ = ncx;
+        }
+        if (fabs(cd) < pert) cd = (cd < T(0)) ? -pert : pert;
+        wsync();
+        T x1 = cx_ * fast_rcp(cd);   // x_{n-1}
+        T x2 = T(0);                 // x_{i+2}
+        if (r == n - 1) xme = x1;
+        for (int i = n - 2; i >= 0; --i) {
+            const T xi = (L.FX[i] - L.FQ[i] * x1 - L.FS[i] * x2) * fast_rcp(L.FP[i]);
+            if (r == i) xme = xi;
+            x2 = x1;
+            x1 = xi;
+        }
+        T mx = gmax<32>((r < n) ? fabs(xme) : T(0));
+        const T sc = (mx > T(0) && isfinite(mx)) ? T(1) / mx : T(1);
+        xme *= sc;
+        wsync();
+        L.B[r] = xme;
+        wsync();
+    }
+    return (r < n) ? xme : T(0);
+}
+
+// y <- Q y with Q = H_0 ... H_{n-2} from the rows kept in h.
+template <typename T>
+__device__ inline cx<T> back_transform32(Lds32<T>& L, const cx<T> (&h)[32], int n, int r, cx<T> y) {
+    for (int j = n - 2; j >= 0; --j) {
+        if (r == j) {
+#pragma unroll
+            for (int c = 0; c < 32; ++c) L.R[c] = h[c];
+        }
+        wsync();
+        const cx<T> v = (r == j + 1) ? cx<T>{T(1), T(0)}
+                        : (r > j + 1 && r < n) ? cmul(cconj(L.R[r]), L.SCAL[j]) : cx<T>{T(0), T(0)};
+        const cx<T> s = gsum<32>(cmulc(y, v));
+        y = csub(y, cmul(v, cmul(L.TAU[j], s)));
+        wsync();
+    }
+    return y;
+}
+
+template <typename T, typename Tg>
+__device__ inline bool load_rows(cx<T> (&h)[32], const cx<Tg>* grow, int n, int r, bool act, int band) {
+    bool fin = true;
+#pragma unroll
+    for (int c = 0; c < 32; ++c) {
+        cx<Tg> v = {Tg(0), Tg(0)};
+        if (r < n && c < n) {
+            if (act) v = grow[c];
+            else if (c == r) v = cx<Tg>{Tg(1), Tg(0)};
+            if (band >= 0 && (c - r > band || r - c > band)) v = cx<Tg>{Tg(0), Tg(0)};
+        }
+        fin = fin && isfinite(v.re) && isfinite(v.im);
+        h[c] = cx<T>{(T)v.re, (T)v.im};
+    }
+    return group_all<32>(fin);
+}
+
+template <typename T, typename Tg>
+__device__ inline bool hadamard_rows(cx<T> (&h)[32], const double (&m)[32], double dr,
+                                     const cx<Tg>* grow, int n, int r, bool act) {
+    bool fin = true;
+#pragma unroll
+    for (int c = 0; c < 32; ++c) {
+        cx<T> v = {T(0), T(0)};
+        if (r < n && c < n) {
+            const double mm = (c == r) ? dr : m[c];
+            cx<Tg> g = (c == r) ? cx<Tg>{Tg(1), Tg(0)} : cx<Tg>{Tg(0), Tg(0)};
+            if (act && c != r) g = grow[c];
+            const double hr = mm * (double)g.re, hi = mm * (double)g.im;
+            fin = fin && isfinite(hr) && isfinite(hi);
+            v = cx<T>{(T)hr, (T)hi};
+        }
+        h[c] = v;
+    }
+    return group_all<32>(fin);
+}
+
+template <typename T, typename Tg>
+__global__ __launch_bounds__(64) void solve32_kernel(SolveArgs a) {
+    __shared__ Lds32<T> sm[2];
+    const int grp = threadIdx.x >> 5, r = threadIdx.x & 31;
+    Lds32<T>& L = sm[grp];
+    const int slot = blockIdx.x * 2 + grp;
+    const int n = a.n;
+    const bool inslot = slot < a.slots;
+    const int np = inslot ? a.npix[slot] : -1;
+    const bool act = np >= 0;
+    const cx<Tg>* gam = (const cx<Tg>*)a.gamma + (size_t)(inslot ? slot : 0) * n * n;
+    const cx<Tg>* grow = gam + (size_t)min(r, n - 1) * n;
+    const long long pix = a.first + (inslot ? slot : 0);
+
+    float code = 0.0f;
+    bool failed = false;
+    cx<T> h[32];
+    cx<T> y = {T(0), T(0)};
+    T tn;
+
+    if (a.module == FRINGE_MODULE_EVD && a.method == FRINGE_METHOD_MLE) {
+        // evd.cpp:584-688
+        const bool fin = load_rows<T, Tg>(h, grow, n, r, act, -1);
+        tridiag32<T>(L, h, n, r);
+        const T lmin = bisect32<T>(L, n, 0, r, tn);
+        if (!fin) { code = -1.0f; failed = true; }
+        else if (!(lmin >= T(1.0e-6))) { code = -2.0f; failed = true; }
+        double m[32], dr;
+        abs_gamma_row<Tg>(m, dr, grow, n, r, act, 1.0e-6);
+        const bool pd_shift = sweep32<T>(L, m, dr, n, r);
+        if (!failed && !pd_shift) { code = -4.0f; failed = true; }
+        abs_gamma_row<Tg>(m, dr, grow, n, r, act, 0.0);
+        const bool pd = sweep32<T>(L, m, dr, n, r);
+        if (!failed && !pd) { code = -5.0f; failed = true; }
+        const bool hfin = hadamard_rows<T, Tg>(h, m, dr, grow, n, r, act);
+        tridiag32<T>(L, h, n, r);
+        const T lam = bisect32<T>(L, n, 0, r, tn);
+        const T yr = tri_eigvec32<T>(L, n, r, lam, tn);
+        y = back_transform32<T>(L, h, n, r, cx<T>{yr, T(0)});
+        if (!failed && !hfin) { code = -6.0f; failed = true; }
+        if (!failed && !(lam >= T(1.0e-6))) { code = -7.0f; failed = true; }
+    } else if (a.module == FRINGE_MODULE_EVD) {
+        // EVD / STBAS (evd.cpp:689-732)
+        const int band = (a.method == FRINGE_METHOD_STBAS) ? a.bw : -1;
+        const bool fin = load_rows<T, Tg>(h, grow, n, r, act, band);
+        tridiag32<T>(L, h, n, r);
+        const T lam = bisect32<T>(L, n, n - 1, r, tn);
+        const T yr = tri_eigvec32<T>(L, n, r, lam, tn);
+        y = back_transform32<T>(L, h, n, r, cx<T>{yr, T(0)});
+        if (!fin) { code = -6.0f; failed = true; }
+        else if (!(lam >= T(1.0e-6))) { code = -7.0f; failed = true; }
+    } else {
+        // phase_link EMI (phase_link.cpp:540-618)
+        double m[32], dr;
+        abs_gamma_row<Tg>(m, dr, grow, n, r, act, 0.0);
+        bool fin = true;
+#pragma unroll
+        for (int c = 0; c < 32; ++c) fin = fin && isfinite(m[c]);
+        fin = group_all<32>(fin);
+        const bool pd = sweep32<T>(L, m, dr, n, r);
+        const bool hfin = hadamard_rows<T, Tg>(h, m, dr, grow, n, r, act);
+        const bool run_evd = !pd || !hfin;
+        tridiag32<T>(L, h, n, r);
+        const T lam = bisect32<T>(L, n, 0, r, tn);
+        const T yr = tri_eigvec32<T>(L, n, r, lam, tn);
+        y = back_transform32<T>(L, h, n, r, cx<T>{yr, T(0)});
+        if (__any(act && fin && run_evd)) {
+            load_rows<T, Tg>(h, grow, n, r, act, -1);
+            tridiag32<T>(L, h, n, r);
+            const T le = bisect32<T>(L, n, n - 1, r, tn);
+            const T ye = tri_eigvec32<T>(L, n, r, le, tn);
+            const cx<T> yv = back_transform32<T>(L, h, n, r, cx<T>{ye, T(0)});
+            if (run_evd) y = yv;
+        }
+        if (!fin) { code = -1.0f; failed = true; }
+    }
+
+    // ---- referenced phases (evd.cpp:738-749) ----
+    if (r == a.ref) L.YREF = y;
+    wsync();
+    const cx<T> vref = L.YREF;
+    bool okr, oku, okv;
+    const cx<T> cref = unit_or(cconj(vref), cx<T>{T(1), T(0)}, okr);
+    const cx<T> u = unit_or(cmulc(y, vref), cx<T>{T(0), T(0)}, oku);
+    cx<T> o = oku ? u : (okr ? cref : unit_or(y, cx<T>{T(1), T(0)}, okv));
+    if (r == a.ref) o = cx<T>{T(1), T(0)};
+    const cx<float> of = {(float)o.re, (float)o.im};
+    L.OUT[r] = of;
+    wsync();
+
+    const bool write_ok = act && !failed;
+    // ---- compressed SLC (evd.cpp:755-762) ----
+    cx<double> cs = {0.0, 0.0};
+    if (r >= a.ref && r < n && write_ok) {
+        const float2 z = a.slc[(long long)r * a.npx + pix];
+        cs.re = (double)(z.x * of.re + z.y * of.im);
+        cs.im = (double)(z.y * of.re - z.x * of.im);
+    }
+    cs = gsum<32>(cs);
+    // ---- temporal coherence (evd.cpp:770-786) ----
+    const bool stbas = (a.method == FRINGE_METHOD_STBAS);
+    cx<T> tc = {T(0), T(0)};
+    if (r < n && write_ok) {
+        const int ulim = stbas ? min(n, r + a.bw + 1) : n;
+        bool okx;
+        const cx<T> ci = unit_or(cx<T>{(T)of.re, -(T)of.im}, cx<T>{T(1), T(0)}, okx);
+        for (int j = r + 1; j < ulim; ++j) {
+            const cx<Tg> gv = grow[j];
+            const cx<T> ph = unit_or(cx<T>{(T)gv.re, (T)gv.im}, cx<T>{T(1), T(0)}, okx);
+            const cx<float> oj = L.OUT[j];
+            const cx<T> cj = unit_or(cx<T>{(T)oj.re, (T)oj.im}, cx<T>{T(1), T(0)}, okx);
+            tc = cadd(tc, cmul(cmul(ph, ci), cj));
+        }
+    }
+    tc = gsum<32>(tc);
+
+    if (!inslot) return;
+    if (r < n) a.phase[(long long)r * a.npx + pix] = write_ok ? make_float2(of.re, of.im) : make_float2(0.0f, 0.0f);
+    if (r == 0) {
+        long long counter = 0;
+        if (stbas) {
+            for (int i = 0; i < n; ++i) counter += min(n, i + a.bw + 1) - i - 1;
+        } else {
+            counter = (long long)n * (n - 1) / 2;
+        }
+        float tco = 0.0f;
+        float2 co = make_float2(0.0f, 0.0f);
+        if (act && failed) {
+            tco = code;
+        } else if (write_ok) {
+            const double tre = (double)tc.re, tim = (double)tc.im;
+            tco = (float)(sqrt(tre * tre + tim * tim) / (double)counter);
+            const double inv = 1.0 / (double)(n - a.ref);
+            co = make_float2((float)(cs.re * inv), (float)(cs.im * inv));
+        }
+        a.tcorr[pix] = tco;
+        a.comp[pix] = co;
+    }
+}
+
+}  // namespace fringe

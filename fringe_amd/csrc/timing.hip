@@ -3,7 +3,7 @@
 #include <vector>
 
 #include "timing.hpp"
-#include "../../include/fringe_amd.h"
+#include "fringe_amd.h"
 
 namespace fringe {
 

@@ -168,6 +168,20 @@ void fringe_timing_reset(void);
 int fringe_timing_read(double* ms, long long* calls, int max_stages);
 const char* fringe_timing_stage_name(int stage);
 
+/* ---- raster utilities (ENVI and GDAL VRT subsets read by the modules) ----
+ * No counterpart in the reference, which reads through GDAL; the Python
+ * drivers use these to size stacks and inspect outputs without GDAL.
+ * band is 1-based (0 = dataset-level metadata, e.g. domain "ENVI"). */
+#define FRINGE_RASTER_C64 0
+#define FRINGE_RASTER_F32 1
+#define FRINGE_RASTER_U32 2
+#define FRINGE_RASTER_U8 3
+int fringe_raster_info(const char* path, int* xsize, int* ysize, int* nbands);
+int fringe_raster_metadata(const char* path, int band, const char* key, const char* domain,
+                           char* out, int cap);
+int fringe_raster_read(const char* path, int band, int x0, int y0, int w, int h, int type,
+                       void* out);
+
 /* ---- utilities ---------------------------------------------------------- */
 int fringe_mask_words(int Nx, int Ny);
 const char* fringe_version(void);

@@ -1,0 +1,152 @@
+"""GPU end-to-end tests of the module drop-in: on-disk stacks in the layout
+tops2vrt.py writes, run through the nmap / evd / phase_link / sequential
+drivers, checked against the CPU oracle."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+from helpers import PHASE_TOL, TCORR_TOL, wrapped_err
+
+pytestmark = pytest.mark.gpu
+if not gpu_available():
+    pytest.skip("needs a GPU", allow_module_level=True)
+
+import oracle  # noqa: E402
+from fringe_amd import raster, synthetic  # noqa: E402
+from fringe_amd.cli import evd as evd_cli  # noqa: E402
+from fringe_amd.cli import nmap as nmap_cli  # noqa: E402
+from fringe_amd.cli import phase_link as pl_cli  # noqa: E402
+
+
+def write_stack(root, slc, start=20220101, amp_const=None):
+    """SLC/<date>/<date>.slc.full + raw VRTs + slcs_base.vrt (tops2vrt.py:144-225)."""
+    import datetime
+    n, rows, cols = slc.shape
+    t0 = datetime.date(2022, 1, 1)
+    srcs, meta, dates = [], [], []
+    os.makedirs(os.path.join(root, "slcs"), exist_ok=True)
+    for t in range(n):
+        d = (t0 + datetime.timedelta(12 * t)).strftime("%Y%m%d")
+        dd = os.path.join(root, "SLC", d)
+        os.makedirs(dd, exist_ok=True)
+        f = os.path.join(dd, d + ".slc.full")
+        slc[t].tofile(f)
+        v = os.path.join(root, "slcs", d + ".vrt")
+        open(v, "w").write(raster.raw_vrt(f, cols, rows))
+        srcs.append((v, d, True))
+        meta.append({} if amp_const is None else {"amplitudeConstant": amp_const[t]})
+        dates.append(d)
+    base = os.path.join(root, "slcs_base.vrt")
+    open(base, "w").write(raster.stack_vrt(srcs, cols, rows, metadata=meta))
+    return base, dates
+
+
+def read_outputs(folder, dates, rows, cols):
+    ph = np.stack([raster.read(os.path.join(folder, d + ".slc")) for d in dates])
+    tc = raster.read(os.path.join(folder, "tcorr.bin"), dtype=np.float32)
+    co = raster.read(os.path.join(folder, "compslc.bin"))
+    return ph, tc, co
+
+
+def test_nmap_and_phase_link_modules(tmp_path):
+    n, rows, cols, Nx, Ny = 10, 28, 33, 3, 2
+    slc = synthetic.stack(n, rows, cols, seed=31, patches=6)
+    base, dates = write_stack(str(tmp_path), slc)
+    wf, cf = str(tmp_path / "KS2" / "nmap"), str(tmp_path / "KS2" / "count")
+    print("case: nmap module", flush=True)
+    assert nmap_cli.main(["-i", base, "-o", wf, "-c", cf, "-x", str(Nx), "-y", str(Ny)]) == 0
+    nul = oracle.mask_words(Nx, Ny)
+    wts = np.stack([raster.read(wf, b + 1, dtype=np.uint32) for b in range(nul)], axis=-1)
+    cnt = raster.read(cf, dtype=np.float32).astype(np.int32)
+    wr, cr = oracle.nmap(slc, Nx, Ny, 0.05)
+    assert np.array_equal(wts, wr) and np.array_equal(cnt, cr)
+    assert raster.metadata(wf, "HALFWINDOWX", "ENVI") == str(Nx)
+    print("case: phase_link module, one block and many blocks", flush=True)
+    out1, out2 = str(tmp_path / "pl1"), str(tmp_path / "pl2")
+    assert pl_cli.main(["-i", base, "-w", wf, "-o", out1, "-x", str(Nx), "-y", str(Ny)]) == 0
+    # tiny memsize/blocksize: many row blocks with Ny-row overlap
+    assert pl_cli.main(["-i", base, "-w", wf, "-o", out2, "-x", str(Nx), "-y", str(Ny), "-l", "8", "-r", "1"]) == 0
+    g1 = read_outputs(out1, dates, rows, cols)
+    g2 = read_outputs(out2, dates, rows, cols)
+    for a, b in zip(g1, g2):
+        assert np.array_equal(a, b)
+    ref = oracle.link(slc, wr, Nx, Ny, module="phase_link", method="MLE", min_neighbors=5)
+    ok = ref["tcorr"] > 0
+    assert np.array_equal(ok, g1[1] > 0)
+    assert wrapped_err(g1[0], ref["phase"])[:, ok].max() <= PHASE_TOL
+    assert np.abs(g1[1] - ref["tcorr"])[ok].max() <= TCORR_TOL
+    assert os.path.exists(os.path.join(out1, dates[0] + ".slc.vrt"))
+
+
+@pytest.mark.parametrize("method", ["EVD", "MLE"])
+def test_evd_module(tmp_path, method):
+    n, rows, cols, Nx, Ny = 8, 20, 21, 2, 2
+    slc = synthetic.stack(n, rows, cols, seed=41)
+    base, dates = write_stack(str(tmp_path), slc)
+    wts = synthetic.full_mask(rows, cols, Nx, Ny)
+    wf = str(tmp_path / "wts")
+    raster.write_envi(wf, wts, meta={"HALFWINDOWX": Nx, "HALFWINDOWY": Ny})
+    out = str(tmp_path / "evd")
+    print(f"case: evd module {method}", flush=True)
+    assert evd_cli.main(["-i", base, "-w", wf, "-o", out, "-x", str(Nx), "-y", str(Ny), "-m", method,
+                         "--precision", "fp64"]) == 0
+    ph, tc, co = read_outputs(out, dates, rows, cols)
+    ref = oracle.link(slc, wts, Nx, Ny, module="evd", method=method)
+    ok = ref["tcorr"] > 0
+    assert np.array_equal(np.where(ref["tcorr"] <= 0, ref["tcorr"], 1), np.where(tc <= 0, tc, 1))
+    assert wrapped_err(ph, ref["phase"])[:, ok].max() <= PHASE_TOL
+    assert np.abs(tc - ref["tcorr"])[ok].max() <= TCORR_TOL
+
+
+def test_sequential_against_oracle_replay(tmp_path):
+    from fringe_amd.sequential import run_sequential
+    n, rows, cols, Nx, Ny, L = 12, 16, 18, 3, 2, 5
+    slc = synthetic.stack(n, rows, cols, seed=51)
+    # SLC/<date>/<date>.slc as ENVI (Stack.py globs */*.slc)
+    import datetime
+    t0 = datetime.date(2022, 1, 1)
+    dates = []
+    for t in range(n):
+        d = (t0 + datetime.timedelta(12 * t)).strftime("%Y%m%d")
+        os.makedirs(tmp_path / "SLC" / d)
+        raster.write_envi(str(tmp_path / "SLC" / d / (d + ".slc")), slc[t])
+        dates.append(d)
+    wts = synthetic.full_mask(rows, cols, Nx, Ny)
+    wf = str(tmp_path / "wts")
+    raster.write_envi(wf, wts, meta={"HALFWINDOWX": Nx, "HALFWINDOWY": Ny})
+    print("case: sequential 12 dates, mini-stacks of 5", flush=True)
+    res = run_sequential(str(tmp_path / "SLC"), wf, str(tmp_path / "seq"), miniStackSize=L,
+                         halfWindowX=Nx, halfWindowY=Ny, minNeighbors=5, verbose=False)
+    assert all(rc == 0 for _, rc in res) and len(res) == 4
+    # oracle replay of the scheme (sequential.py:193-254, evd module MLE)
+    comps = []
+    k = 0
+    for s0 in range(0, n, L):
+        k += 1
+        s1 = min(n, s0 + L)
+        stack = np.concatenate([np.array(comps).reshape(-1, rows, cols), slc[s0:s1]]) if comps else slc[s0:s1]
+        ref = oracle.link(stack.astype(np.complex64), wts, Nx, Ny, module="evd", method="MLE",
+                          mini_stack_count=k)
+        folder = str(tmp_path / "seq" / "miniStacks" / f"{dates[s0]}_{dates[s1 - 1]}" / "EVD")
+        comp_dates = sorted(os.path.basename(os.path.dirname(p)) for p in
+                            glob.glob(str(tmp_path / "seq" / "compressedSlc" / "*" / "*.slc")))
+        band_dates = [d for d in comp_dates if d < dates[s0]] + dates[s0:s1]
+        ph = np.stack([raster.read(os.path.join(folder, d + ".slc")) for d in band_dates])
+        tc = raster.read(os.path.join(folder, "tcorr.bin"), dtype=np.float32)
+        comp = raster.read(str(tmp_path / "seq" / "compressedSlc" / dates[s1 - 1] / (dates[s1 - 1] + ".slc")))
+        ok = ref["tcorr"] > 0
+        assert ok.sum() > 0.8 * rows * cols
+        assert wrapped_err(ph, ref["phase"])[:, ok].max() <= PHASE_TOL
+        assert np.abs(tc - ref["tcorr"])[ok].max() <= TCORR_TOL
+        rel = np.abs(comp - ref["comp"])[ok] / np.maximum(np.abs(ref["comp"][ok]), 1e-6)
+        assert rel.max() < 1e-3
+        comps.append(ref["comp"])   # the next mini-stack sees the oracle's compressed SLC
+    # datum connection
+    dc = str(tmp_path / "seq" / "Datum_connection" / "EVD")
+    ref = oracle.link(np.array(comps).astype(np.complex64), wts, Nx, Ny, module="evd", method="MLE")
+    ph = np.stack([raster.read(os.path.join(dc, dates[min(n, s0 + L) - 1] + ".slc")) for s0 in range(0, n, L)])
+    ok = ref["tcorr"] > 0
+    assert wrapped_err(ph, ref["phase"])[:, ok].max() <= 5 * PHASE_TOL

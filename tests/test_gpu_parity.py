@@ -74,3 +74,63 @@ def test_nmap_bit_exact(n):
     cg = cg.cpu().numpy()
     assert np.array_equal(wg, wr)
     assert np.array_equal(cg, cr)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed_bf16"])
+def test_link_generic_solver_n40(precision):
+    # above 32 dates the generic LDS-resident solver runs
+    n, rows, cols, Nx, Ny = 40, 12, 14, 4, 3
+    print(f"case n=40 {precision}", flush=True)
+    slc = synthetic.stack(n, rows, cols, seed=103)
+    wts = synthetic.full_mask(rows, cols, Nx, Ny)
+    g, r = _run_both(slc, wts, Nx, Ny, "phase_link", "MLE", precision)
+    st = compare_link(g, r, r["gap"])
+    print(st)
+    assert st["untouched_equal"]
+    tol = PHASE_TOL if precision != "mixed_bf16" else 5 * PHASE_TOL
+    assert st["phase_max"] <= tol and st["tcorr_max"] <= TCORR_TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_link_stbas_and_reference_index(precision):
+    n, rows, cols, Nx, Ny = 12, 14, 16, 3, 2
+    print(f"case stbas {precision}", flush=True)
+    slc = synthetic.stack(n, rows, cols, seed=104)
+    wts = synthetic.full_mask(rows, cols, Nx, Ny)
+    g, r = _run_both(slc, wts, Nx, Ny, "evd", "STBAS", precision, bw=4, ref=2)
+    st = compare_link(g, r, r["gap"])
+    print(st)
+    assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL
+    assert np.all(g[0][2][r["tcorr"] > 0] == 1 + 0j)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_link_failure_codes_and_untouched(precision):
+    # 9 window samples for 10 dates: Gamma is rank deficient, evd MLE -> -2;
+    # a cleared centre bit leaves the pixel untouched
+    n, rows, cols, Nx, Ny = 10, 8, 9, 1, 1
+    print(f"case codes {precision}", flush=True)
+    slc = synthetic.stack(n, rows, cols, seed=105)
+    wts = synthetic.full_mask(rows, cols, Nx, Ny)
+    wts[3, 4, :] = 0
+    g, r = _run_both(slc, wts, Nx, Ny, "evd", "MLE", precision, want_gap=False)
+    assert np.array_equal(r["tcorr"] == 0, g[1] == 0)
+    assert np.array_equal(np.where(r["tcorr"] < 0, r["tcorr"], 0), np.where(g[1] < 0, g[1], 0))
+    assert not g[0][:, 3, 4].any() and g[2][3, 4] == 0
+    # phase_link honours minNeighbors
+    g2, r2 = _run_both(slc, wts, Nx, Ny, "phase_link", "MLE", precision, min_nb=10, want_gap=False)
+    assert np.all(g2[1] == 0) and np.all(r2["tcorr"] == 0)
+
+
+def test_mixed_precision_tcorr_and_phase_at_cfg1_scale():
+    # N=10, 64x64 crop of the 256^2 EVD workload (all-ones mask, 11x11 window)
+    n, rows, cols, Nx, Ny = 10, 64, 64, 5, 5
+    print("case cfg1 crop", flush=True)
+    slc = synthetic.stack(n, rows, cols, seed=101)
+    wts = synthetic.full_mask(rows, cols, Nx, Ny)
+    for precision in ("fp64", "mixed"):
+        g, r = _run_both(slc, wts, Nx, Ny, "evd", "EVD", precision)
+        st = compare_link(g, r, r["gap"])
+        print(precision, st)
+        assert st["n_well"] == rows * cols
+        assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL

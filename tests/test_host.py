@@ -1,0 +1,137 @@
+"""CPU tests of libfringe_amd.so's host side: exported C ABI, KS decision
+table, ENVI/VRT reading and writing, option objects and the module error
+codes that are decided before any GPU work."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from fringe_amd import _lib, raster, synthetic
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "fringe_amd.h")).read()
+    decl = set(re.findall(r"^\s*(?:[\w\s\*]+?)\b(fringe_\w+)\s*\(", hdr, re.M))
+    L = _lib.lib()
+    missing = [s for s in sorted(decl) if not hasattr(L, s)]
+    assert not missing, missing
+    assert set(_lib.EXPORTS) <= decl
+
+
+@pytest.mark.parametrize("n", [8, 10, 16, 20, 28, 30, 48, 64, 96])
+def test_ks_table_matches_oracle_decisions(n):
+    L = _lib.lib()
+    tab = np.zeros(n + 1, np.uint8)
+    margin = L.fringe_ks_table(n, 0.05, tab.ctypes.data)
+    for k in range(n + 1):
+        p = oracle.kolmogorov_prob(k / n * np.sqrt(n * n / (2.0 * n)))
+        assert tab[k] == (1 if p >= 0.05 else 0)
+    assert margin > 1e-6          # every decision is far from the threshold
+    kstar = int(np.argmin(tab)) if not tab.all() else n + 1
+    assert np.all(tab[:kstar] == 1) and np.all(tab[kstar:] == 0)
+
+
+def test_ks_table_thresholds_quoted_for_the_configs():
+    # similar iff k = D*n < k*(n, 0.05)
+    exp = {10: 7, 30: 11, 64: 16, 96: 19}
+    for n, ks in exp.items():
+        tab = np.zeros(n + 1, np.uint8)
+        _lib.lib().fringe_ks_table(n, 0.05, tab.ctypes.data)
+        assert int(np.argmin(tab)) == ks
+
+
+def test_ks_table_decides_like_the_oracle_walk():
+    # the decision of the integer walk equals the oracle's p >= alpha
+    rng = np.random.default_rng(1)
+    for n in (10, 30):
+        tab = np.zeros(n + 1, np.uint8)
+        _lib.lib().fringe_ks_table(n, 0.05, tab.ctypes.data)
+        for _ in range(500):
+            a = np.sort(rng.rayleigh(size=n).astype(np.float32))
+            b = np.sort((rng.rayleigh(size=n) * rng.uniform(0.6, 1.6)).astype(np.float32))
+            k = int(round(oracle.ks2_statistic(a, b) * n))
+            assert bool(tab[k]) == (oracle.ks2_pvalue(a, b) >= 0.05)
+
+
+def test_envi_and_vrt_roundtrip(tmp_path):
+    rng = np.random.default_rng(0)
+    rows, cols = 13, 17
+    z = (rng.standard_normal((rows, cols)) + 1j * rng.standard_normal((rows, cols))).astype(np.complex64)
+    f = str(tmp_path / "a.slc")
+    raster.write_envi(f, z)
+    assert raster.info(f) == (cols, rows, 1)
+    assert np.array_equal(raster.read(f), z)
+    assert np.array_equal(raster.read(f, window=(3, 4, 5, 6)), z[4:10, 3:8])
+    # raw VRT over the flat file (tops2vrt layout)
+    v = str(tmp_path / "a.vrt")
+    open(v, "w").write(raster.raw_vrt(f, cols, rows))
+    assert np.array_equal(raster.read(v), z)
+    # stack VRT with a bbox SrcRect and Date metadata
+    s = str(tmp_path / "stack.vrt")
+    open(s, "w").write(raster.stack_vrt([(v, "20220101", True), (f, "20220113", False)], cols, rows,
+                                        bbox=(2, 10, 1, 9)))
+    assert raster.info(s) == (8, 8, 2)
+    assert np.array_equal(raster.read(s, 1), z[2:10, 1:9])
+    assert np.array_equal(raster.read(s, 2), z[0:8, 0:8])
+    assert raster.metadata(s, "Date", "slc", band=2) == "20220113"
+
+
+def test_multiband_uint32_bip_and_envi_metadata(tmp_path):
+    rng = np.random.default_rng(2)
+    w = rng.integers(0, 2 ** 32, size=(5, 6, 4), dtype=np.uint64).astype(np.uint32)
+    f = str(tmp_path / "wts")
+    raster.write_envi(f, w, meta={"HALFWINDOWX": 5, "HALFWINDOWY": 5})
+    assert raster.info(f) == (6, 5, 4)
+    for b in range(4):
+        assert np.array_equal(raster.read(f, b + 1, dtype=np.uint32), w[:, :, b])
+    assert raster.metadata(f, "HALFWINDOWX", "ENVI") == "5"
+    assert raster.metadata(f, "halfwindowy", "ENVI") == "5"
+
+
+def test_module_errors_before_gpu_work(tmp_path):
+    from fringe_amd.evdlib import Evd
+    from fringe_amd.nmaplib import Nmap
+    from fringe_amd.phase_linklib import Phaselink
+    e = Evd()
+    e.verbose = False
+    e.inputDS = str(tmp_path / "missing.vrt")
+    assert e.run() == 102
+    # stack with dates, weights with the wrong window
+    rows, cols, n = 6, 7, 4
+    slc = synthetic.stack(n, rows, cols, seed=1)
+    srcs = []
+    for t in range(n):
+        d = tmp_path / f"2022010{t + 1}"
+        d.mkdir()
+        fn = str(d / f"2022010{t + 1}.slc")
+        raster.write_envi(fn, slc[t])
+        srcs.append((fn, f"2022010{t + 1}", False))
+    sv = str(tmp_path / "stack.vrt")
+    open(sv, "w").write(raster.stack_vrt(srcs, cols, rows))
+    wts = synthetic.full_mask(rows, cols, 2, 2)
+    wf = str(tmp_path / "wts")
+    raster.write_envi(wf, wts, meta={"HALFWINDOWX": 2, "HALFWINDOWY": 2})
+    p = Phaselink()
+    p.verbose = False
+    p.inputDS, p.weightsDS, p.outputFolder = sv, wf, str(tmp_path / "out")
+    p.halfWindowX, p.halfWindowY = 3, 2
+    assert p.run() in (108, 109)         # nulong and HALFWINDOWX mismatch
+    p.halfWindowX = 2
+    p.method, p.bandWidth = "STBAS", 0
+    assert p.run() == 101
+    p.method = "MLE"
+    (tmp_path / "exists").mkdir()
+    p.outputFolder = str(tmp_path / "exists")
+    assert p.run() == 113
+    p.outputFolder = str(tmp_path / "afile")
+    open(p.outputFolder, "w").close()
+    assert p.run() == 114
+    nm = Nmap()
+    nm.verbose = False
+    nm.inputDS, nm.method = sv, "XX"
+    assert nm.run() == 1

@@ -46,6 +46,55 @@ __device__ inline void wsync() {
 
 template <typename T> __device__ inline T fast_rcp(T x) { return T(1) / x; }
 template <> __device__ inline float fast_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+// 1/x in double: hardware estimate refined by two Newton steps (~1 ulp).
+template <> __device__ inline double fast_rcp<double>(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+
+// ---- reductions over the 32 lanes of a half-wave without LDS --------------
+// quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror and row_mirror
+// combine 16 lanes through DPP; v_permlane16_swap then exchanges the two
+// 16-lane rows of the half.  Every lane ends with the same, bitwise
+// identical value (each stage adds two equal-rank partials commutatively).
+template <int CTRL> __device__ inline uint32_t dpp32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+struct OpAdd { template <typename T> __device__ static T f(T a, T b) { return a + b; } };
+struct OpMax { template <typename T> __device__ static T f(T a, T b) { return max(a, b); } };
+struct OpMin { template <typename T> __device__ static T f(T a, T b) { return min(a, b); } };
+
+template <typename Op> __device__ inline float red32(float x) {
+    x = Op::f(x, __uint_as_float(dpp32<0xB1>(__float_as_uint(x))));
+    x = Op::f(x, __uint_as_float(dpp32<0x4E>(__float_as_uint(x))));
+    x = Op::f(x, __uint_as_float(dpp32<0x141>(__float_as_uint(x))));
+    x = Op::f(x, __uint_as_float(dpp32<0x140>(__float_as_uint(x))));
+    const auto s = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    return Op::f(__uint_as_float(s[0]), __uint_as_float(s[1]));
+}
+template <int CTRL> __device__ inline double dpp64(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = dpp32<CTRL>((uint32_t)u), hi = dpp32<CTRL>((uint32_t)(u >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+template <typename Op> __device__ inline double red32(double x) {
+    x = Op::f(x, dpp64<0xB1>(x));
+    x = Op::f(x, dpp64<0x4E>(x));
+    x = Op::f(x, dpp64<0x141>(x));
+    x = Op::f(x, dpp64<0x140>(x));
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)u, (uint32_t)u, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((uint32_t)(u >> 32), (uint32_t)(u >> 32), false, false);
+    const double a = __builtin_bit_cast(double, ((uint64_t)hi[0] << 32) | lo[0]);
+    const double b = __builtin_bit_cast(double, ((uint64_t)hi[1] << 32) | lo[1]);
+    return Op::f(a, b);
+}
+template <typename T> __device__ inline T sum32(T x) { return red32<OpAdd>(x); }
+template <typename T> __device__ inline T max32(T x) { return red32<OpMax>(x); }
+template <typename T> __device__ inline T min32(T x) { return red32<OpMin>(x); }
+template <typename T> __device__ inline cx<T> sum32(cx<T> x) { return cx<T>{sum32(x.re), sum32(x.im)}; }
 
 // ---- FP64 sweep inverse of |Gamma| - shift*I, rows in registers ----------
 // m[c] (c != r) holds M_rc, dr holds M_rr; m[r] is a free slot.
@@ -57,7 +106,7 @@ __device__ inline void abs_gamma_row(double (&m)[32], double& dr, const cx<Tg>* 
         double v = 0.0;
         if (act && c < n && r < n && c != r) {
             const cx<Tg> g = grow[c];
-            v = sqrt((double)g.re * (double)g.re + (double)g.im * (double)g.im);
+            v = (double)sqrt(g.re * g.re + g.im * g.im);
         }
         m[c] = v;
     }
@@ -69,7 +118,7 @@ __device__ inline bool sweep32(Lds32<T>& L, double (&m)[32], double& dr, int n, 
     bool ok = true;
     for (int j = 0; j < n; ++j) {
         if (r == j) {
-            const double pv = 1.0 / dr;
+            const double pv = fast_rcp(dr);
 #pragma unroll
             for (int c = 0; c < 32; ++c) L.TR[c] = m[c] * pv;
             L.TR[j] = 1.0 + pv;
@@ -78,13 +127,15 @@ __device__ inline bool sweep32(Lds32<T>& L, double (&m)[32], double& dr, int n, 
         wsync();
         const double p = L.PV;
         ok = ok && (p > 0.0);
-        const double piv = 1.0 / p;
+        const double piv = fast_rcp(p);
         const double trr = L.TR[r];
-        const double f = (r < j ? -trr : trr) * p;       // M_rj = s_r M_jr
+        // rows r != j: M_rc -= M_rj T_c with M_rj = s_r M_jr read from the
+        // published row; the pivot row becomes T = M_j. / p, which is
+        // M_jc - (p - 1) T_c, so one FMA form covers every lane.
         const bool me = (r == j);
-        const double al = me ? 0.0 : 1.0, be = me ? 1.0 : -f;
+        const double f = me ? p - 1.0 : (r < j ? -trr : trr) * p;
 #pragma unroll
-        for (int c = 0; c < 32; ++c) m[c] = fma(be, L.TR[c], al * m[c]);
+        for (int c = 0; c < 32; ++c) m[c] = fma(-f, L.TR[c], m[c]);
         dr = me ? piv : fma(-f, trr, dr);
         wsync();
     }
@@ -101,15 +152,18 @@ __device__ inline void tri_step(Lds32<T>& L, cx<T> (&h)[32], int j, int n, int r
     wsync();
     const cx<T> alpha = cconj(L.R[j + 1]);
     const cx<T> xr = (r > j + 1 && r < n) ? cconj(L.R[r]) : cx<T>{T(0), T(0)};
-    const T xn2 = gsum<32>(cabs2(xr));
+    const T xn2 = sum32(cabs2(xr));
     const T ar = alpha.re, ai = alpha.im;
     cx<T> tau = {T(0), T(0)}, scal = {T(0), T(0)};
     T beta = ar;
     if (xn2 != T(0) || ai != T(0)) {
         const T nrm = sqrt(ar * ar + ai * ai + xn2);
         beta = (ar >= T(0)) ? -nrm : nrm;
-        tau = cx<T>{(beta - ar) / beta, -ai / beta};
-        scal = crecip(cx<T>{ar - beta, ai});
+        const T ib = fast_rcp(beta);
+        tau = cx<T>{(beta - ar) * ib, -ai * ib};
+        const T dr_ = ar - beta;
+        const T id = fast_rcp(dr_ * dr_ + ai * ai);
+        scal = cx<T>{dr_ * id, -ai * id};
     }
     const cx<T> vr = (r == j + 1) ? cx<T>{T(1), T(0)} : (r > j + 1 && r < n) ? cmul(xr, scal) : cx<T>{T(0), T(0)};
     L.V[r] = vr;
@@ -124,7 +178,7 @@ __device__ inline void tri_step(Lds32<T>& L, cx<T> (&h)[32], int j, int n, int r
 #pragma unroll
     for (int c = C0; c < 32; ++c) cfma(p, h[c], L.V[c]);
     p = cmul(tau, p);
-    const cx<T> s = gsum<32>(cmulc(vr, p));
+    const cx<T> s = sum32(cmulc(vr, p));
     const cx<T> a2 = cscale(cmul(tau, s), T(-0.5));
     const cx<T> wr = (r > j && r < n) ? cadd(p, cmul(a2, vr)) : cx<T>{T(0), T(0)};
     L.W[r] = wr;
@@ -172,17 +226,21 @@ __device__ inline T bisect32(Lds32<T>& L, int n, int k, int r, T& tnorm) {
             em2 = L.E2[r];
         }
     }
-    lo = gmin<32>(lo);
-    hi = gmax<32>(hi);
-    em2 = gmax<32>(em2);
-    tn = gmax<32>(tn);
+    lo = min32(lo);
+    hi = max32(hi);
+    em2 = max32(em2);
+    tn = max32(tn);
     wsync();
     tnorm = tn;
     const T pivmin = Lim<T>::tiny() * max(T(1), em2);
     const T pad = T(4) * Lim<T>::eps() * tn + T(2) * pivmin;
     lo -= pad;
     hi += pad;
-    for (int it = 0; it < Lim<T>::iters(32); ++it) {
+    // 33-section shrinks the Gershgorin interval 33-fold per round: 5 rounds
+    // reach ~1e-7 |T| (enough for the inverse-iteration shift in FP32),
+    // 10 rounds reach FP64 resolution
+    const int rounds = sizeof(T) == 4 ? 5 : 10;
+    for (int it = 0; it < rounds; ++it) {
         const T w = hi - lo;
         const T sig = lo + (w * T(r + 1)) / T(33);
         T q = L.D[0] - sig;
@@ -255,7 +313,7 @@ __device__ inline T tri_eigvec32(Lds32<T>& L, int n, int r, T lam, T tnorm) {
             x2 = x1;
             x1 = xi;
         }
-        T mx = gmax<32>((r < n) ? fabs(xme) : T(0));
+        T mx = max32((r < n) ? fabs(xme) : T(0));
         const T sc = (mx > T(0) && isfinite(mx)) ? T(1) / mx : T(1);
         xme *= sc;
         wsync();
@@ -276,7 +334,7 @@ __device__ inline cx<T> back_transform32(Lds32<T>& L, const cx<T> (&h)[32], int 
         wsync();
         const cx<T> v = (r == j + 1) ? cx<T>{T(1), T(0)}
                         : (r > j + 1 && r < n) ? cmul(cconj(L.R[r]), L.SCAL[j]) : cx<T>{T(0), T(0)};
-        const cx<T> s = gsum<32>(cmulc(y, v));
+        const cx<T> s = sum32(cmulc(y, v));
         y = csub(y, cmul(v, cmul(L.TAU[j], s)));
         wsync();
     }
@@ -297,7 +355,7 @@ __device__ inline bool load_rows(cx<T> (&h)[32], const cx<Tg>* grow, int n, int 
         fin = fin && isfinite(v.re) && isfinite(v.im);
         h[c] = cx<T>{(T)v.re, (T)v.im};
     }
-    return group_all<32>(fin);
+    return min32(fin ? 1.0f : 0.0f) > 0.5f;
 }
 
 template <typename T, typename Tg>
@@ -317,10 +375,15 @@ __device__ inline bool hadamard_rows(cx<T> (&h)[32], const double (&m)[32], doub
         }
         h[c] = v;
     }
-    return group_all<32>(fin);
+    return min32(fin ? 1.0f : 0.0f) > 0.5f;
 }
 
-template <typename T, typename Tg>
+enum { MODE_EMI = 0, MODE_EVD = 1, MODE_MLE = 2 };
+
+// MODE_EMI: phase_link (EMI with EVD fallback, phase_link.cpp:540-618)
+// MODE_EVD: evd EVD / STBAS (evd.cpp:689-732)
+// MODE_MLE: evd MLE with its failure codes (evd.cpp:584-688)
+template <typename T, typename Tg, int MODE>
 __global__ __launch_bounds__(64) void solve32_kernel(SolveArgs a) {
     __shared__ Lds32<T> sm[2];
     const int grp = threadIdx.x >> 5, r = threadIdx.x & 31;
@@ -340,8 +403,7 @@ __global__ __launch_bounds__(64) void solve32_kernel(SolveArgs a) {
     cx<T> y = {T(0), T(0)};
     T tn;
 
-    if (a.module == FRINGE_MODULE_EVD && a.method == FRINGE_METHOD_MLE) {
-        // evd.cpp:584-688
+    if constexpr (MODE == MODE_MLE) {
         const bool fin = load_rows<T, Tg>(h, grow, n, r, act, -1);
         tridiag32<T>(L, h, n, r);
         const T lmin = bisect32<T>(L, n, 0, r, tn);
@@ -361,8 +423,7 @@ __global__ __launch_bounds__(64) void solve32_kernel(SolveArgs a) {
         y = back_transform32<T>(L, h, n, r, cx<T>{yr, T(0)});
         if (!failed && !hfin) { code = -6.0f; failed = true; }
         if (!failed && !(lam >= T(1.0e-6))) { code = -7.0f; failed = true; }
-    } else if (a.module == FRINGE_MODULE_EVD) {
-        // EVD / STBAS (evd.cpp:689-732)
+    } else if constexpr (MODE == MODE_EVD) {
         const int band = (a.method == FRINGE_METHOD_STBAS) ? a.bw : -1;
         const bool fin = load_rows<T, Tg>(h, grow, n, r, act, band);
         tridiag32<T>(L, h, n, r);
@@ -372,13 +433,12 @@ __global__ __launch_bounds__(64) void solve32_kernel(SolveArgs a) {
         if (!fin) { code = -6.0f; failed = true; }
         else if (!(lam >= T(1.0e-6))) { code = -7.0f; failed = true; }
     } else {
-        // phase_link EMI (phase_link.cpp:540-618)
         double m[32], dr;
         abs_gamma_row<Tg>(m, dr, grow, n, r, act, 0.0);
         bool fin = true;
 #pragma unroll
         for (int c = 0; c < 32; ++c) fin = fin && isfinite(m[c]);
-        fin = group_all<32>(fin);
+        fin = min32(fin ? 1.0f : 0.0f) > 0.5f;
         const bool pd = sweep32<T>(L, m, dr, n, r);
         const bool hfin = hadamard_rows<T, Tg>(h, m, dr, grow, n, r, act);
         const bool run_evd = !pd || !hfin;
@@ -418,23 +478,35 @@ __global__ __launch_bounds__(64) void solve32_kernel(SolveArgs a) {
         cs.re = (double)(z.x * of.re + z.y * of.im);
         cs.im = (double)(z.y * of.re - z.x * of.im);
     }
-    cs = gsum<32>(cs);
+    cs = sum32(cs);
     // ---- temporal coherence (evd.cpp:770-786) ----
+    // sum over pairs i < j (j <= i + bw for STBAS) of
+    //   exp(i(arg Gamma_ij - arg out_i + arg out_j)).
+    // Pairs are dealt out circulantly so every lane takes about (n-1)/2 of
+    // them: lane i forms t = g_ij conj(o_i) o_j with j = i + d (mod n) from
+    // its own row of Gamma, and the pair's term is t for i < j, conj(t) for
+    // i > j (Gamma and the phasor products are Hermitian).
     const bool stbas = (a.method == FRINGE_METHOD_STBAS);
     cx<T> tc = {T(0), T(0)};
     if (r < n && write_ok) {
-        const int ulim = stbas ? min(n, r + a.bw + 1) : n;
         bool okx;
         const cx<T> ci = unit_or(cx<T>{(T)of.re, -(T)of.im}, cx<T>{T(1), T(0)}, okx);
-        for (int j = r + 1; j < ulim; ++j) {
+        const int half = (n - 1) / 2;
+        const int dmax = ((n & 1) == 0 && r < n / 2) ? half + 1 : half;
+        for (int d = 1; d <= dmax; ++d) {
+            int j = r + d;
+            if (j >= n) j -= n;
+            const int lo = min(r, j), hi = max(r, j);
+            if (stbas && hi - lo > a.bw) continue;
             const cx<Tg> gv = grow[j];
             const cx<T> ph = unit_or(cx<T>{(T)gv.re, (T)gv.im}, cx<T>{T(1), T(0)}, okx);
             const cx<float> oj = L.OUT[j];
-            const cx<T> cj = unit_or(cx<T>{(T)oj.re, (T)oj.im}, cx<T>{T(1), T(0)}, okx);
-            tc = cadd(tc, cmul(cmul(ph, ci), cj));
+            cx<T> t = cmul(cmul(ph, ci), cx<T>{(T)oj.re, (T)oj.im});
+            if (r > j) t = cconj(t);
+            tc = cadd(tc, t);
         }
     }
-    tc = gsum<32>(tc);
+    tc = sum32(tc);
 
     if (!inslot) return;
     if (r < n) a.phase[(long long)r * a.npx + pix] = write_ok ? make_float2(of.re, of.im) : make_float2(0.0f, 0.0f);

@@ -23,6 +23,60 @@ import time
 
 import numpy as np
 
+PEAK_HBM = 8000.0          # GB/s, MI355X HBM3E
+PEAK_F16_MFMA = 2500.0     # TFLOP/s, dense fp16 MFMA
+PEAK_F32_VALU = 157.3      # TFLOP/s, fp32
+PROFILE_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+
+
+def gram_flops(n, keff):
+    """Useful complex-Gram flops: 8 per complex MAC, both triangles."""
+    return 8.0 * n * n * keff
+
+
+def solve_flops(n):
+    """Algorithmic flops of one EMI pixel: inverse of the real SPD |Gamma|
+    (N^3), Householder reduction of the Hermitian H to tridiagonal form
+    (16/3 N^3), back-transform of one eigenvector (8N^2), Hadamard product
+    (2N^2), temporal coherence over N(N-1)/2 pairs (~12 flops each)."""
+    return (19.0 / 3.0) * n ** 3 + 16.0 * n * n
+
+
+def pack_bytes(n, npx):
+    return 16.0 * n * npx
+
+
+def _roof(kernel, bound, achieved, peak, unit, ms, calls, note):
+    return {"kernel": kernel, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+            "frac": achieved / peak, "ms_per_run": ms, "launches_per_run": calls, "note": note}
+
+
+def _traffic(kernel, rows, cols, n, Nx, Ny, launches):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, each scaled by what the
+    counter reads for a device copy of known size); None when absent."""
+    path = os.path.join(PROFILE_DIR, "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    key = f"{kernel}:n{n}:{rows}x{cols}:{2 * Nx + 1}x{2 * Ny + 1}"
+    e = t.get("kernels", {}).get(key)
+    return None if e is None else e.get("bytes_per_launch")
+
+
+def copy_probe(mib=1024, reps=5):
+    """A plain device-to-device copy of known size, run under rocprofv3 --pmc
+    to calibrate FETCH_SIZE / WRITE_SIZE against true bytes."""
+    import torch
+    x = torch.ones(mib << 18, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": x.numel() * 4, "reps": reps}))
+
 
 def parse():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
@@ -38,6 +92,7 @@ def parse():
     ap.add_argument("--cpu-rows", type=int, default=24, help="output rows of the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=102)
+    ap.add_argument("--copy-probe", action="store_true", help="run only the copy kernel used to calibrate PMC bytes")
     return ap.parse_args()
 
 
@@ -46,6 +101,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
     import torch
     from fringe_amd import _lib, synthetic, tile
 
@@ -56,6 +114,12 @@ def main():
     slc = synthetic.stack_torch(n, rows, cols, args.seed, device=dev, patches=32)
     wts, cnt = tile.nmap(slc, Nx, Ny, 0.05)
     torch.cuda.synchronize()
+    L.fringe_timing_reset()
+    L.fringe_timing_enable(1)
+    wts, cnt = tile.nmap(slc, Nx, Ny, 0.05)
+    torch.cuda.synchronize()
+    L.fringe_timing_enable(0)
+    nm_per = {k: v for k, v in _lib.stage_times().items() if v[1]}
     min_nb = 5
     W = (2 * Nx + 1) * (2 * Ny + 1)
     cnt_f = cnt.to(torch.float64)
@@ -101,33 +165,32 @@ def main():
         "tcorr_err_max": float(tcerr.max()), "reference": "FP64 GPU path (itself checked against the FP64 oracle in tests/)",
     }
 
-    # ---- roofline of the dominant stage --------------------------------
+    # ---- roofline of every stage, the dominant one on the line -----------
     steps_timed = args.steps
     per = {k: (v[0] / max(steps_timed, 1), v[1] / max(steps_timed, 1)) for k, v in stages.items() if v[1]}
     keff = float(cnt_f[(tc != 0)].sum())
-    gram_flops = 8.0 * n * n * keff                        # useful complex-Gram flops per run
-    out_bytes = linked * (8 * n + 4 + 8)
-    pk_bytes = rows * cols * (8 * n)
     rl = {}
     if "gram" in per:
-        g_ms = per["gram"][0]
-        rl["gram"] = {"bound": "mfma", "achieved": gram_flops / (g_ms / 1e3) / 1e12, "peak": 2500.0,
-                      "unit": "TFLOP/s", "ms_per_run": g_ms, "launches_per_run": per["gram"][1]}
-        rl["gram"]["frac"] = rl["gram"]["achieved"] / rl["gram"]["peak"]
+        g_ms, g_calls = per["gram"]
+        rl["gram"] = _roof("gram_mfma", "mfma", gram_flops(n, keff) / (g_ms / 1e3) / 1e12, PEAK_F16_MFMA,
+                           "TFLOP/s", g_ms, g_calls, "8*K_eff*N^2 flops per linked pixel (hi/lo passes not counted)")
     if "solve" in per:
-        s_ms = per["solve"][0]
-        rl["solve"] = {"bound": "hbm", "achieved": (linked * (n * n * 8 * 2 + 8 * n + 12)) / (s_ms / 1e3) / 1e9,
-                       "peak": 8000.0, "unit": "GB/s", "ms_per_run": s_ms, "launches_per_run": per["solve"][1],
-                       "note": "per-pixel eigensolver; VALU-bound, HBM figure shown for scale"}
-        rl["solve"]["frac"] = rl["solve"]["achieved"] / rl["solve"]["peak"]
+        s_ms, s_calls = per["solve"]
+        rl["solve"] = _roof("solve32_kernel", "valu", solve_flops(n) * linked / (s_ms / 1e3) / 1e12, PEAK_F32_VALU,
+                            "TFLOP/s", s_ms, s_calls,
+                            "(19/3)N^3 + 16N^2 flops per linked pixel: |Gamma| inverse, Hermitian tridiagonal "
+                            "reduction, back-transform, Hadamard product, temporal coherence")
     if "pack" in per:
-        p_ms = per["pack"][0]
-        rl["pack"] = {"bound": "hbm", "achieved": (pk_bytes * 2 + 2 * pk_bytes) / (p_ms / 1e3) / 1e9,
-                      "peak": 8000.0, "unit": "GB/s", "ms_per_run": p_ms}
-        rl["pack"]["frac"] = rl["pack"]["achieved"] / rl["pack"]["peak"]
-    dom = max(per, key=lambda k: per[k][0]) if per else None
-    roof = dict(rl.get(dom, {}))
-    roof.update({"kernel": dom, "traffic": None})
+        p_ms, p_calls = per["pack"]
+        rl["pack"] = _roof("pack_hilo", "hbm", pack_bytes(n, rows * cols) / (p_ms / 1e3) / 1e9, PEAK_HBM,
+                           "GB/s", p_ms, p_calls, "16N bytes per pixel of the tile (stack read twice, records written)")
+    if "nmap_prep" in nm_per:
+        m_ms, m_calls = nm_per["nmap_prep"]
+        rl["nmap_prep"] = _roof("nmap_prep", "hbm", rows * cols * 12 * n / (m_ms / 1e3) / 1e9, PEAK_HBM, "GB/s",
+                                m_ms, m_calls, "12N bytes per pixel (complex64 read, sorted amplitudes written)")
+    dom = max(("gram", "solve", "pack"), key=lambda k: per.get(k, (0, 0))[0])
+    roof = {k: v for k, v in rl[dom].items() if k in ("kernel", "bound", "achieved", "peak", "unit", "frac", "note")}
+    roof["traffic"] = _traffic(roof["kernel"], rows, cols, n, Nx, Ny, per[dom][1])
 
     # ---- CPU baseline: the oracle on a bounded row band -------------------
     cpu = None
@@ -165,6 +228,7 @@ def main():
         "p50_tile_ms": float(np.percentile(lat, 50)), "p99_tile_ms": float(np.percentile(lat, 99)),
         "accuracy_vs_fp64": acc,
         "stages_ms_per_step": {k: v[0] for k, v in per.items()},
+        "nmap_ms": {k: v[0] for k, v in nm_per.items()},
         "stages_fp64_ms_per_step": {k: v[0] / max(args.fp64_steps, 1) for k, v in stages64.items() if v[1]},
         "roofline": roof, "roofline_stages": rl,
         "cpu_baseline": cpu,

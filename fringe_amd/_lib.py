@@ -24,7 +24,11 @@ EXPORTS = [
     "fringe_mask_words", "fringe_version", "fringe_last_error",
     "fringe_timing_enable", "fringe_timing_reset", "fringe_timing_read", "fringe_timing_stage_name",
     "fringe_raster_info", "fringe_raster_metadata", "fringe_raster_read",
+    "fringe_server_request_bytes", "fringe_server_create", "fringe_server_submit", "fringe_server_poll",
+    "fringe_server_wait", "fringe_server_result", "fringe_server_times", "fringe_server_release",
+    "fringe_server_destroy",
 ]
+FRINGE_REQ_QUEUED, FRINGE_REQ_RUNNING, FRINGE_REQ_DONE = 0, 1, 2
 
 
 class LinkTile(ctypes.Structure):
@@ -55,6 +59,11 @@ class NmapOptions(ctypes.Structure):
                 ("method", ctypes.c_char_p), ("noGPU", ctypes.c_int), ("blocksize", ctypes.c_int),
                 ("memsize", ctypes.c_int), ("Nx", ctypes.c_int), ("Ny", ctypes.c_int),
                 ("prob", ctypes.c_double), ("verbose", ctypes.c_int)]
+
+
+class ServerRequest(ctypes.Structure):
+    _fields_ = [("tile", LinkTile), ("d_slc", ctypes.c_void_p), ("d_wts", ctypes.c_void_p),
+                ("d_mask", ctypes.c_void_p), ("prob", ctypes.c_double)]
 
 
 _LIB = None
@@ -88,6 +97,18 @@ def lib():
     L.fringe_raster_metadata.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p,
                                          ctypes.c_char_p, ctypes.c_int]
     L.fringe_raster_read.argtypes = [ctypes.c_char_p] + [ctypes.c_int] * 6 + [vp]
+    rq = ctypes.POINTER(ServerRequest)
+    ll = ctypes.c_longlong
+    L.fringe_server_request_bytes.argtypes = [rq]
+    L.fringe_server_request_bytes.restype = sz
+    L.fringe_server_create.argtypes = [ctypes.c_int, sz, ctypes.c_int, ctypes.POINTER(vp)]
+    L.fringe_server_submit.argtypes = [vp, rq, ctypes.POINTER(ll)]
+    L.fringe_server_poll.argtypes = [vp, ll, ctypes.POINTER(ctypes.c_int)]
+    L.fringe_server_wait.argtypes = [vp, ll]
+    L.fringe_server_result.argtypes = [vp, ll] + [ctypes.POINTER(vp)] * 5
+    L.fringe_server_times.argtypes = [vp, ll, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    L.fringe_server_release.argtypes = [vp, ll]
+    L.fringe_server_destroy.argtypes = [vp]
     for name, st in (("fringe_evd_options_init", EvdOptions), ("fringe_evd_process", EvdOptions),
                      ("fringe_phase_link_process", EvdOptions),
                      ("fringe_nmap_options_init", NmapOptions), ("fringe_nmap_process", NmapOptions)):

@@ -41,6 +41,11 @@ template <typename T> __device__ inline void cfma_c(cx<T>& acc, cx<T> a, cx<T> b
     acc.re = fma(a.re, b.re, fma(a.im, b.im, acc.re));
     acc.im = fma(a.im, b.re, fma(-a.re, b.im, acc.im));
 }
+// acc -= a * conj(b), as four FMAs
+template <typename T> __device__ inline void cfms_c(cx<T>& acc, cx<T> a, cx<T> b) {
+    acc.re = fma(-a.re, b.re, fma(-a.im, b.im, acc.re));
+    acc.im = fma(-a.im, b.re, fma(a.re, b.im, acc.im));
+}
 // acc += a * b
 template <typename T> __device__ inline void cfma(cx<T>& acc, cx<T> a, cx<T> b) {
     acc.re = fma(a.re, b.re, fma(-a.im, b.im, acc.re));

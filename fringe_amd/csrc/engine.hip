@@ -58,7 +58,7 @@ static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 // ---------------------------------------------------------------------------
 // phase linking
 // ---------------------------------------------------------------------------
-static int link_check(const fringe_link_tile* t) {
+int link_check(const fringe_link_tile* t) {
     if (!t) return FRINGE_ERR_ARGUMENT;
     if (t->rows <= 0 || t->cols <= 0 || t->nbands <= 0 || t->Nx < 0 || t->Ny < 0)
         return FRINGE_ERR_ARGUMENT;

@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "fringe_amd.h"
 
 namespace fringe {
@@ -39,6 +41,10 @@ struct LinkPlan {
     int* d_npix;        // [chunk]
     uint32_t* d_pack;   // mixed precision: packed (hi|lo) records, [rows*cols+1][2n]
 };
+
+// 0, or the code fringe_link_run would return for these options.
+int link_check(const fringe_link_tile* t);
+void set_last_error(const std::string& what);
 
 int link_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, float2* phase,
                 float* tcorr, float2* comp, hipStream_t s);

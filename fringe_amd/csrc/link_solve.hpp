@@ -270,9 +270,8 @@ __device__ void tridiagonalize(SolveLds<T>& L, int S, int n) {
             const cx<T> vr = L.V[r], wr = L.W[r];
             for (int c = j + 1; c < n; ++c) {
                 cx<T> a = A[r * S + c];
-                const cx<T> vc = L.V[c], wc = L.W[c];
-                a = csub(a, cmulc(vr, wc));
-                a = csub(a, cmulc(wr, vc));
+                cfms_c(a, vr, L.W[c]);
+                cfms_c(a, wr, L.V[c]);
                 A[r * S + c] = a;
             }
         }

@@ -185,9 +185,8 @@ __device__ inline void tri_step(Lds32<T>& L, cx<T> (&h)[32], int j, int n, int r
     wsync();
 #pragma unroll
     for (int c = C0; c < 32; ++c) {
-        const cx<T> vc = L.V[c], wc = L.W[c];
-        h[c] = csub(h[c], cmulc(vr, wc));
-        h[c] = csub(h[c], cmulc(wr, vc));
+        cfms_c(h[c], vr, L.W[c]);
+        cfms_c(h[c], wr, L.V[c]);
     }
     wsync();
 }

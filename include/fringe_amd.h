@@ -59,7 +59,7 @@ extern "C" {
 #define FRINGE_METHOD_STBAS 2
 #define FRINGE_PREC_FP64 0          /* FP64 Gram, FP64 eigensolver ("plain" path) */
 #define FRINGE_PREC_MIXED 1         /* fp16 hi/lo MFMA Gram (per-date power-of-two scaled),
-                                       FP32 eigensolver, FP64 |Gamma| inverse and refinement */
+                                       FP32 eigensolver, FP64 |Gamma| inverse */
 #define FRINGE_PREC_MIXED_BF16 2    /* as MIXED with bf16 hi/lo samples (no scaling needed) */
 
 /* ---- module level: replaces evd_process (evd.cpp:16) and
@@ -157,6 +157,56 @@ int fringe_nmap_run(const fringe_nmap_tile* t, const void* d_slc, const uint8_t*
  * nmap.cpp:462).  Returns the smallest distance |p_k - prob| over k, so a
  * caller can see how far every decision is from the threshold. */
 double fringe_ks_table(int n, double prob, uint8_t* h_table);
+
+/* ---- tile server: submit / poll / complete ------------------------------
+ * No counterpart in the reference, which links one file-backed stack per
+ * process call (evd.cpp:16).  A server keeps the tiles of many requests
+ * resident in HBM and runs them concurrently: every request names a
+ * device-resident stack (band-planar complex64, kept alive by the caller
+ * until the request completes), an optional device-resident SHP mask (NULL:
+ * the server computes it with the KS2 nmap at prob) and the link options.
+ * The server reserves the request's workspace and outputs in one arena of
+ * budget_bytes allocated at creation, admits queued requests oldest first as
+ * streams and arena space free up (a request passed over too often blocks
+ * later ones until it fits), and launches each admitted request on its own
+ * stream, so requests overlap on the device.  Outputs stay in the arena until
+ * fringe_server_release.  Codes are per request: the return value of
+ * fringe_server_poll / fringe_server_wait is the request's result code
+ * (0, 201-204), per-pixel codes go into its tcorr as for fringe_link_run. */
+typedef struct fringe_server fringe_server;
+
+typedef struct fringe_server_request {
+    fringe_link_tile tile;
+    const void* d_slc;          /* [nbands][rows][cols] complex64 on the server's device */
+    const uint32_t* d_wts;      /* [rows][cols][nulong], or NULL: computed by the server */
+    const uint8_t* d_mask;      /* optional validity mask for the server's nmap */
+    double prob;                /* KS2 significance when d_wts is NULL (nmap.hpp default 0.05) */
+} fringe_server_request;
+
+#define FRINGE_REQ_QUEUED 0
+#define FRINGE_REQ_RUNNING 1
+#define FRINGE_REQ_DONE 2
+
+/* Arena bytes a request reserves while it runs (workspace + outputs). */
+size_t fringe_server_request_bytes(const fringe_server_request* r);
+int fringe_server_create(int device, size_t budget_bytes, int streams, fringe_server** out);
+/* Queue a request (never blocks).  201 for bad arguments, 204 if the request
+ * can never fit the arena. */
+int fringe_server_submit(fringe_server* s, const fringe_server_request* r, long long* id);
+/* *state = FRINGE_REQ_*; returns the request's code (0 while not done). */
+int fringe_server_poll(fringe_server* s, long long id, int* state);
+/* Block until the request is done; returns its code. */
+int fringe_server_wait(fringe_server* s, long long id);
+/* Device pointers of a done request's outputs (valid until release); d_wts /
+ * d_count are the server-computed SHP map, NULL if the request brought one. */
+int fringe_server_result(fringe_server* s, long long id, void** d_phase, float** d_tcorr,
+                         void** d_comp, uint32_t** d_wts, int32_t** d_count);
+/* Milliseconds from submit to admission and from submit to completion. */
+int fringe_server_times(fringe_server* s, long long id, double* queued_ms, double* total_ms);
+/* Free a done request's outputs. */
+int fringe_server_release(fringe_server* s, long long id);
+/* Run every submitted request to completion, then free the arena and streams. */
+int fringe_server_destroy(fringe_server* s);
 
 /* ---- stage timing (no counterpart in the reference; for profiling) ------
  * When enabled, every engine stage (0 pack, 1 gram, 2 solve, 3 nmap_prep,

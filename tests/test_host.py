@@ -135,3 +135,33 @@ def test_module_errors_before_gpu_work(tmp_path):
     nm.verbose = False
     nm.inputDS, nm.method = sv, "XX"
     assert nm.run() == 1
+
+
+def test_server_request_bytes_and_create_without_device():
+    L = _lib.lib()
+    t = _lib.LinkTile(64, 80, 30, 7, 5, 1, 0, 5, 0, -1, 1, 0, 64)
+    r = _lib.ServerRequest(t, 1, None, None, 0.05)
+    with_nmap = L.fringe_server_request_bytes(ctypes.byref(r))
+    r.d_wts = 1
+    given = L.fringe_server_request_bytes(ctypes.byref(r))
+    words = L.fringe_mask_words(7, 5)
+    npx = 64 * 80
+    # the server-computed mask and count are kept with the outputs, and its
+    # nmap workspace holds N float amplitudes and a validity byte per pixel
+    extra = (npx * words * 4 + 255) // 256 * 256 + (npx * 4 + 255) // 256 * 256
+    extra += ((npx * 30 * 4 + 255) // 256 * 256 + (npx + 255) // 256 * 256 + 255) // 256 * 256
+    assert with_nmap - given == extra
+    outputs = sum((b + 255) // 256 * 256 for b in (npx * 30 * 8, npx * 4, npx * 8))
+    assert given >= outputs + L.fringe_link_workspace_bytes(ctypes.byref(t))
+    r.prob = 0.0                      # no significance level for the server's nmap
+    r.d_wts = None
+    assert L.fringe_server_request_bytes(ctypes.byref(r)) == 0
+    t.method, t.bandwidth = 2, 0       # STBAS without a bandwidth
+    assert L.fringe_server_request_bytes(ctypes.byref(_lib.ServerRequest(t, 1, 1, None, 0.05))) == 0
+    h = ctypes.c_void_p()
+    rc = L.fringe_server_create(0, 1 << 20, 2, ctypes.byref(h))
+    if rc == 0:                        # a GPU is present
+        assert L.fringe_server_destroy(h) == 0
+    else:
+        assert rc == 203 and not h.value
+    assert L.fringe_server_create(0, 0, 2, ctypes.byref(h)) == 201

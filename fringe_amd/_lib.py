@@ -20,7 +20,7 @@ EXPORTS = [
     "fringe_evd_options_init", "fringe_evd_process", "fringe_phase_link_process",
     "fringe_nmap_options_init", "fringe_nmap_process",
     "fringe_link_workspace_bytes", "fringe_link_run",
-    "fringe_nmap_workspace_bytes", "fringe_nmap_run", "fringe_ks_table",
+    "fringe_nmap_workspace_bytes", "fringe_nmap_run", "fringe_ks_table", "fringe_ad2_threshold",
     "fringe_mask_words", "fringe_version", "fringe_last_error",
     "fringe_timing_enable", "fringe_timing_reset", "fringe_timing_read", "fringe_timing_stage_name",
     "fringe_raster_info", "fringe_raster_metadata", "fringe_raster_read",
@@ -86,6 +86,8 @@ def lib():
     L.fringe_nmap_run.argtypes = [ctypes.POINTER(NmapTile), vp, vp, vp, vp, vp, vp, sz, vp]
     L.fringe_ks_table.argtypes = [ctypes.c_int, ctypes.c_double, vp]
     L.fringe_ks_table.restype = ctypes.c_double
+    L.fringe_ad2_threshold.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+    L.fringe_ad2_threshold.restype = ctypes.c_double
     L.fringe_mask_words.argtypes = [ctypes.c_int, ctypes.c_int]
     L.fringe_version.restype = ctypes.c_char_p
     L.fringe_last_error.restype = ctypes.c_char_p

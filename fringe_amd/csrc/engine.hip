@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "engine_internal.hpp"
 #include "link_solve.hpp"
@@ -54,6 +55,97 @@ static double kolmogorov_prob(double z) {
 }
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// ---------------------------------------------------------------------------
+// Two-sample Anderson-Darling decision (AD2unique.hpp): sigma_N of Scholz &
+// Stephens' A2_kN for k = 2 (AD2unique.hpp:162-203) and the p-value by
+// log-odds interpolation in the k = 2 column of their quantile table
+// (AD2unique.hpp:8-52, :125-160).
+// ---------------------------------------------------------------------------
+static const double kAdQuantile[35] = {
+    -1.1954, -1.1786, -1.166, -1.1407, -1.1253, -1.0777, -1.0489, -0.9978, -0.9417, -0.8981,
+    -0.8598, -0.7258, -0.5966, -0.4572, -0.2966, -0.1009, 0.1571, 0.5357, 1.2255, 1.5262,
+    1.9633, 2.7314, 3.7825, 4.1241, 4.6044, 5.409, 6.4954, 6.8279, 7.2755, 8.1885,
+    9.3061, 9.6132, 10.0989, 10.8825, 11.8537};
+static const double kAdProb[35] = {
+    .00001, .00005, .0001, .0005, .001, .005, .01, .025, .05, .075, .1, .2, .3, .4, .5, .6, .7, .8, .9,
+    .925, .95, .975, .99, .9925, .995, .9975, .999, .99925, .9995, .99975, .9999, .999925,
+    .99995, .999975, .99999};
+
+static double ad2_sigma(int n1, int n2) {
+    const int N = n1 + n2;
+    const double H = 1.0 / (1.0 * n1) + 1.0 / (1.0 * n2);
+    double h = 0.0, g = 0.0;
+    if (N < 2000) {
+        std::vector<double> inv(N, 0.0);
+        for (int i = 1; i < N; ++i) {
+            inv[i] = 1.0 / i;
+            h += inv[i];
+        }
+        for (int i = 1; i < N - 1; ++i)
+            for (int j = i + 1; j < N; ++j) g += inv[N - i] * inv[j];
+    } else {
+        h = std::log(double(N - 1)) + 0.5772156649015328606065120900824024;
+        g = M_PI * M_PI / 6.0;
+    }
+    const double k = 2.0, k2 = std::pow(k, 2);
+    const double a = (4 * g - 6) * (k - 1) + (10 - 6 * g) * H;
+    const double b = (2 * g - 4) * k2 + 8 * h * k + (2 * g - 14 * h - 4) * H - 8 * h + 4 * g - 6;
+    const double c = (6 * h + 2 * g - 2) * k2 + (4 * h - 4 * g + 6) * k + (2 * h - 6) * H + 4 * h;
+    const double d = (2 * h + 6) * k2 - 4 * h * k;
+    double v = 0.0;
+    v += a * std::pow(double(N), 3) + b * std::pow(double(N), 2) + c * N + d;
+    v /= (double(N - 1) * double(N - 2) * double(N - 3));
+    return std::sqrt(v);
+}
+
+static double ad2_pvalue(double t) {
+    int first = 35;
+    for (int i = 0; i < 35; ++i)
+        if (t <= kAdQuantile[i]) {
+            first = i;
+            break;
+        }
+    int lo = (first == 35) ? 33 : first - 1;
+    if (lo < 0) lo = 0;
+    const int hi = lo + 1;
+    const double l1 = std::log((1.0 - kAdProb[lo]) / kAdProb[lo]);
+    const double l2 = std::log((1.0 - kAdProb[hi]) / kAdProb[hi]);
+    const double l0 = (l1 - l2) * (t - kAdQuantile[hi]) / (kAdQuantile[lo] - kAdQuantile[hi]) + l2;
+    return std::exp(l0) / (1. + std::exp(l0));
+}
+
+// Doubles mapped to unsigned keys in the same order (finite values).
+static uint64_t dkey(double x) {
+    uint64_t b;
+    std::memcpy(&b, &x, 8);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+static double dval(uint64_t k) {
+    const uint64_t b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    double x;
+    std::memcpy(&x, &b, 8);
+    return x;
+}
+
+// Largest standardised statistic t with p(t) >= prob (p decreases with t);
+// -inf when no statistic is similar, +inf when every one is.  The search
+// starts at -8, where p is still 1: the interpolated log-odds overflow exp
+// (and p becomes NaN) only below about -8.5, and no statistic goes below
+// -1/sigma_N >= -2.2 (A2_kN >= 0).
+static double ad2_threshold(double prob) {
+    const double lo0 = -8.0, hi0 = 1.0e6;
+    if (!(ad2_pvalue(lo0) >= prob)) return -INFINITY;
+    if (ad2_pvalue(hi0) >= prob) return INFINITY;
+    uint64_t lo = dkey(lo0), hi = dkey(hi0);   // p(lo) >= prob > p(hi)
+    while (hi - lo > 1) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (ad2_pvalue(dval(mid)) >= prob) lo = mid;
+        else hi = mid;
+    }
+    return dval(lo);
+}
+
 
 // ---------------------------------------------------------------------------
 // phase linking
@@ -181,7 +273,15 @@ double fringe_ks_table(int n, double prob, uint8_t* h_table) {
 size_t fringe_nmap_workspace_bytes(const fringe_nmap_tile* t) {
     if (!t || t->rows <= 0 || t->cols <= 0 || t->nbands <= 0) return 0;
     const size_t npx = (size_t)t->rows * t->cols;
-    return align256(npx * t->nbands * sizeof(float)) + align256(npx);
+    size_t b = align256(npx * t->nbands * sizeof(float)) + align256(npx);
+    if (t->method == 1) b += align256((size_t)(2 * t->nbands - 1) * (t->nbands + 1) * sizeof(double));
+    return b;
+}
+
+double fringe_ad2_threshold(int n, double prob, double* h_sigma) {
+    if (n < 2) return NAN;
+    if (h_sigma) *h_sigma = ad2_sigma(n, n);
+    return ad2_threshold(prob);
 }
 
 int fringe_nmap_run(const fringe_nmap_tile* t, const void* d_slc, const uint8_t* d_mask,
@@ -193,7 +293,8 @@ int fringe_nmap_run(const fringe_nmap_tile* t, const void* d_slc, const uint8_t*
     if (t->row_begin < 0 || t->row_end > t->rows || t->row_begin > t->row_end) return FRINGE_ERR_ARGUMENT;
     if (t->nbands > kMaxBandsNmap) return FRINGE_ERR_UNSUPPORTED;
     if (shp_words(t->Nx, t->Ny) > kMaxMaskWords) return FRINGE_ERR_UNSUPPORTED;
-    if (t->method != 0) return FRINGE_ERR_UNSUPPORTED;   // AD2 is not built on the GPU yet
+    if (t->method != 0 && t->method != 1) return FRINGE_ERR_ARGUMENT;
+    if (t->method == 1 && t->nbands < 4) return FRINGE_ERR_UNSUPPORTED;   // sigma_N needs 2n > 3
     if ((long long)t->rows * t->cols >= (1ll << 31) - 1) return FRINGE_ERR_UNSUPPORTED;
     if (work_bytes < fringe_nmap_workspace_bytes(t)) return FRINGE_ERR_WORKSPACE;
     NmapPlan P{};
@@ -222,6 +323,12 @@ int fringe_nmap_run(const fringe_nmap_tile* t, const void* d_slc, const uint8_t*
     const size_t npx = (size_t)t->rows * t->cols;
     P.d_amp = (float*)d_work;
     P.d_valid = (uint8_t*)d_work + align256(npx * t->nbands * sizeof(float));
+    P.method = t->method;
+    if (t->method == 1) {
+        P.ad2_sigma = ad2_sigma(t->nbands, t->nbands);
+        P.ad2_max = ad2_threshold(t->prob);
+        P.d_ad2 = (double*)((char*)d_work + align256(npx * t->nbands * sizeof(float)) + align256(npx));
+    }
     if (nmap_launch(P, (const float2*)d_slc, d_mask, d_wts, d_count, (hipStream_t)stream) != 0) {
         set_last_error("fringe_nmap_run", hipGetLastError());
         return FRINGE_ERR_DEVICE;

@@ -26,6 +26,10 @@ struct NmapPlan {
     KsTable table;
     float* d_amp;      // [rows*cols][n] sorted amplitudes
     uint8_t* d_valid;  // [rows*cols]
+    int method;        // 0 = KS2, 1 = AD2
+    double ad2_sigma;  // AD2: standard deviation of A2_kN (k = 2, n + n samples)
+    double ad2_max;    // AD2: similar iff the standardised statistic <= ad2_max
+    double* d_ad2;     // AD2: [2n-1][n+1] per-position terms of the inner sum
 };
 
 int nmap_launch(const NmapPlan& P, const float2* slc, const uint8_t* mask, uint32_t* wts,

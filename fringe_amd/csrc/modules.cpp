@@ -374,10 +374,6 @@ int fringe_nmap_process(const fringe_nmap_options* o) {
         std::printf("Could not create weights dataset {%s}\n", nz(o->wtsDS));
         return 105;
     }
-    if (meth == 1) {
-        std::printf("AD2 is not available on the GPU in this build\n");
-        return FRINGE_ERR_UNSUPPORTED;
-    }
     fringe_nmap_tile t{};
     t.rows = blockysize;
     t.cols = cols;

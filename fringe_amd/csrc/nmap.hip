@@ -7,10 +7,13 @@
 //
 // Stage 1 (nmap_prep): per pixel, amplitude |z| / alpha_b for every band,
 // validity, and an ascending sort held in registers (bitonic network).
-// Stage 2 (nmap_ks): per pixel, the two-sample KS statistic against every
-// valid in-image neighbour as an integer count k = D*n, decided through a
-// host-built table (k -> similar) instead of evaluating the Kolmogorov series
-// on the device.
+// Stage 2 (nmap_sweep): per pixel, the two-sample test against every valid
+// in-image neighbour.  KS2: the statistic as an integer count k = D*n,
+// decided through a host-built table (k -> similar) instead of evaluating the
+// Kolmogorov series on the device.  AD2: the standardised Anderson-Darling
+// statistic in double, bit-identical to the host's, compared with the largest
+// statistic whose interpolated p-value still reaches the threshold (found on
+// the host, so no exp/log runs on the device).
 #include "device_common.hpp"
 #include "engine_internal.hpp"
 #include "timing.hpp"
@@ -109,9 +112,49 @@ __device__ inline int ks_count(const float* __restrict__ a, const float* __restr
     return max(dmax, abs(d));
 }
 
+// Anderson-Darling inner sum of AD2unique::test (AD2unique.hpp:206-351) for
+// ascending a (first sample) and b: merge with ties taken from b first and
+// add, for every merged position j < 2n-1, the term of (j, m_j) where m_j
+// counts the a values among the first j+1.  The terms are tabulated once per
+// call (ad2_terms), so the double sum adds the same values in the same order.
+__device__ inline double ad2_inner(const float* __restrict__ a, const float* __restrict__ b, int n,
+                                   const double* __restrict__ T) {
+    int ia = 0, ib = 0, m = 0;
+    double S = 0.0;
+    float x = a[0], y = b[0];
+    for (int j = 0; j < 2 * n - 1; ++j) {
+        const bool take_a = (ia < n && ib < n) ? (x < y) : (ia < n);
+        if (take_a) {
+            ++ia;
+            ++m;
+            x = ia < n ? a[ia] : x;
+        } else {
+            ++ib;
+            y = ib < n ? b[ib] : y;
+        }
+        S = S + T[j * (n + 1) + m];
+    }
+    return S;
+}
+
+// term(j, m) = (2n m - n (j+1))^2 / ((j+1)(2n - j - 1)); every intermediate
+// is an exact integer in double, so the one rounding is the division's.
+__global__ void ad2_terms(double* __restrict__ T, int n) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (2 * n - 1) * (n + 1)) return;
+    const int j = idx / (n + 1), m = idx % (n + 1);
+    const double nsum = 2.0 * n, bj = j + 1;
+    const double t = nsum * m - (double)n * bj;
+    T[idx] = t * t / (bj * (nsum - bj));
+}
+
 // One thread per output pixel (rows [row_begin, row_end) of the tile).
-__global__ __launch_bounds__(256) void nmap_ks(const NmapPlan P, uint32_t* __restrict__ wts,
-                                               int32_t* __restrict__ count) {
+// The reference sweeps half the window and tests (p, q) with p first for
+// q = p + (dy, dx), dy > 0 or dy = 0 < dx (nmap.cpp:402-457); the other half
+// of a pixel's window is therefore tested with the neighbour first.
+template <bool AD2>
+__global__ __launch_bounds__(256) void nmap_sweep(const NmapPlan P, uint32_t* __restrict__ wts,
+                                                  int32_t* __restrict__ count) {
     const float* __restrict__ amp_sorted = P.d_amp;
     const uint8_t* __restrict__ valid = P.d_valid;
     const int n = P.n, rows = P.rows, cols = P.cols, Nx = P.Nx, Ny = P.Ny, nul = P.nul;
@@ -147,9 +190,20 @@ __global__ __launch_bounds__(256) void nmap_ks(const NmapPlan P, uint32_t* __res
             if (!valid[q]) continue;
             bool similar = true;
             if (q != p) {
-                const int k = ks_count(a, amp_sorted + q * n, n);
-                const uint32_t tw = k < 32 ? t0 : k < 64 ? t1 : k < 96 ? t2 : k < 128 ? t3 : t4;
-                similar = ((tw >> (k & 31)) & 1u) != 0u;
+                const float* b = amp_sorted + q * n;
+                if constexpr (AD2) {
+                    const bool fwd = dy > 0 || (dy == 0 && dx > 0);
+                    const double S = ad2_inner(fwd ? a : b, fwd ? b : a, n, P.d_ad2);
+                    double A = 0.0;
+                    A = A + S / (n * 1.0);
+                    A = A + S / (n * 1.0);
+                    A = A / (2.0 * n);
+                    similar = (A - 1) / P.ad2_sigma <= P.ad2_max;
+                } else {
+                    const int k = ks_count(a, b, n);
+                    const uint32_t tw = k < 32 ? t0 : k < 64 ? t1 : k < 96 ? t2 : k < 128 ? t3 : t4;
+                    similar = ((tw >> (k & 31)) & 1u) != 0u;
+                }
             }
             if (similar) {
                 cur |= (1u << (flat & 31));
@@ -185,7 +239,13 @@ int nmap_launch(const NmapPlan& P, const float2* slc, const uint8_t* mask, uint3
         const int threads = 256;
         const long long blocks = (nout + threads - 1) / threads;
         StageScope sc(STAGE_NMAP_KS, s);
-        hipLaunchKernelGGL(nmap_ks, dim3((unsigned)blocks), dim3(threads), 0, s, P, wts, count);
+        if (P.method == 1) {
+            const int nt = (2 * P.n - 1) * (P.n + 1);
+            hipLaunchKernelGGL(ad2_terms, dim3((nt + 255) / 256), dim3(256), 0, s, P.d_ad2, P.n);
+            hipLaunchKernelGGL(nmap_sweep<true>, dim3((unsigned)blocks), dim3(threads), 0, s, P, wts, count);
+        } else {
+            hipLaunchKernelGGL(nmap_sweep<false>, dim3((unsigned)blocks), dim3(threads), 0, s, P, wts, count);
+        }
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

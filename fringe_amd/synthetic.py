@@ -103,3 +103,25 @@ def full_mask(rows, cols, Nx, Ny):
     for f in range(W):
         words[f >> 5] |= np.uint32(1 << (f & 31))
     return np.broadcast_to(words, (rows, cols, nul)).copy()
+
+
+def random_mask_torch(rows, cols, Nx, Ny, fill, seed, device="cuda"):
+    """SHP bitmask int32 [rows, cols, nulong] on the device: the centre bit
+    always set, every other window bit set with probability `fill`."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    W = (2 * Nx + 1) * (2 * Ny + 1)
+    nul = (W + 31) // 32
+    centre = Ny * (2 * Nx + 1) + Nx
+    out = torch.empty((rows, cols, nul), dtype=torch.int32, device=device)
+    shifts = torch.arange(32, device=device, dtype=torch.int64)
+    step = max(1, (1 << 22) // max(cols * W, 1) + 1)
+    for r0 in range(0, rows, step):
+        r1 = min(rows, r0 + step)
+        bits = torch.rand((r1 - r0, cols, nul * 32), generator=g, device=device) < fill
+        bits[..., W:] = False
+        bits[..., centre] = True
+        words = (bits.view(r1 - r0, cols, nul, 32).to(torch.int64) << shifts).sum(-1)
+        out[r0:r1] = (words - ((words >> 31) << 32)).to(torch.int32)   # two's-complement bit pattern
+    return out

@@ -93,13 +93,13 @@ def link(slc, wts, Nx, Ny, **kw):
 
 
 def nmap(slc, Nx, Ny, prob=0.05, mask=None, amp_const=None, row_begin=0, row_end=None,
-         stream=None):
-    """KS2 SHP map of a device tile.  Returns (wts int32 [rows, cols, nulong],
-    count int32 [rows, cols])."""
+         stream=None, method="KS2"):
+    """SHP map of a device tile (method "KS2" or "AD2").  Returns
+    (wts int32 [rows, cols, nulong], count int32 [rows, cols])."""
     n, rows, cols = slc.shape
     t = _lib.NmapTile()
     t.rows, t.cols, t.nbands, t.Nx, t.Ny = rows, cols, n, Nx, Ny
-    t.prob, t.method = prob, 0
+    t.prob, t.method = prob, {"KS2": 0, "AD2": 1}[method]
     t.row_begin, t.row_end = row_begin, rows if row_end is None else row_end
     L = _lib.lib()
     wb = L.fringe_nmap_workspace_bytes(ctypes.byref(t))

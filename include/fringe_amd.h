@@ -143,7 +143,7 @@ typedef struct fringe_nmap_tile {
     int rows, cols, nbands;
     int Nx, Ny;
     double prob;
-    int method;          /* 0 = KS2, 1 = AD2 */
+    int method;          /* 0 = KS2, 1 = AD2 (nbands >= 4) */
     int row_begin, row_end;
 } fringe_nmap_tile;
 
@@ -157,6 +157,12 @@ int fringe_nmap_run(const fringe_nmap_tile* t, const void* d_slc, const uint8_t*
  * nmap.cpp:462).  Returns the smallest distance |p_k - prob| over k, so a
  * caller can see how far every decision is from the threshold. */
 double fringe_ks_table(int n, double prob, uint8_t* h_table);
+
+/* AD2 decision used by the nmap kernel (AD2unique.hpp:125-351): a pair is
+ * similar when its standardised statistic (A2_kN - 1) / sigma_N is at most
+ * the returned value, the largest statistic whose interpolated p-value is
+ * still >= prob.  *h_sigma (optional) receives sigma_N for n + n samples. */
+double fringe_ad2_threshold(int n, double prob, double* h_sigma);
 
 /* ---- tile server: submit / poll / complete ------------------------------
  * No counterpart in the reference, which links one file-backed stack per

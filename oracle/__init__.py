@@ -61,7 +61,15 @@ def lib():
         L.oracle_ks2_pvalue.restype = ctypes.c_double
         L.oracle_nmap.argtypes = [_f32p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
-                                  ctypes.c_double, _u32p, _i32p, ctypes.c_int]
+                                  ctypes.c_double, ctypes.c_int, _u32p, _i32p, ctypes.c_int]
+        L.oracle_ad2_statistic.argtypes = [_f32p, _f32p, ctypes.c_int]
+        L.oracle_ad2_statistic.restype = ctypes.c_double
+        L.oracle_ad2_pvalue.argtypes = [_f32p, _f32p, ctypes.c_int]
+        L.oracle_ad2_pvalue.restype = ctypes.c_double
+        L.oracle_ad2_pvalue_of.argtypes = [ctypes.c_double]
+        L.oracle_ad2_pvalue_of.restype = ctypes.c_double
+        L.oracle_ad2_sigma.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.oracle_ad2_sigma.restype = ctypes.c_double
         L.oracle_link.argtypes = [_f32p, _u32p] + [ctypes.c_int] * 12 + [
             _f32p, _f32p, _f32p, ctypes.c_void_p, ctypes.c_int]
         L.oracle_eigen_index.argtypes = [_f64p, ctypes.c_int, ctypes.c_int,
@@ -108,6 +116,22 @@ def ks2_statistic(a, b):
     return lib().oracle_ks2_statistic(a, b, a.size)
 
 
+def ad2_statistic(a, b):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    return lib().oracle_ad2_statistic(a, b, a.size)
+
+
+def ad2_pvalue(a, b):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    return lib().oracle_ad2_pvalue(a, b, a.size)
+
+
+def ad2_pvalue_of(t):
+    return lib().oracle_ad2_pvalue_of(float(t))
+
+
 def ks2_pvalue(a, b):
     a = np.ascontiguousarray(a, np.float32)
     b = np.ascontiguousarray(b, np.float32)
@@ -119,8 +143,8 @@ def _slc_f32(slc):
     return slc.view(np.float32).reshape(-1)
 
 
-def nmap(slc, Nx, Ny, prob=0.05, mask=None, amp_const=None, threads=0):
-    """Race-free KS2 SHP map.  slc: complex64 [nbands, rows, cols].
+def nmap(slc, Nx, Ny, prob=0.05, mask=None, amp_const=None, threads=0, method="KS2"):
+    """Race-free SHP map (method "KS2" or "AD2").  slc: complex64 [nbands, rows, cols].
 
     Returns (wts uint32 [rows, cols, nulong], count int32 [rows, cols])."""
     nb, rows, cols = slc.shape
@@ -131,7 +155,7 @@ def nmap(slc, Nx, Ny, prob=0.05, mask=None, amp_const=None, threads=0):
     ac = None if amp_const is None else np.ascontiguousarray(amp_const, np.float64)
     lib().oracle_nmap(_slc_f32(slc), None if m is None else m.ctypes.data, rows, cols, nb,
                       None if ac is None else ac.ctypes.data, Nx, Ny, float(prob),
-                      wts.reshape(-1), cnt.reshape(-1), int(threads))
+                      {"KS2": 0, "AD2": 1}[method], wts.reshape(-1), cnt.reshape(-1), int(threads))
     return wts, cnt
 
 

@@ -185,8 +185,108 @@ extern "C" double oracle_ks2_pvalue(const float* a, const float* b, int n) {
 }
 
 // --------------------------------------------------------------------------
-// nmap: race-free full-window KS sweep (nmap.cpp:204-233, :345-473;
-// nmap_cuda.cu:259-334 semantics).
+// Two-sample Anderson-Darling test as AD2unique evaluates it
+// (AD2unique.hpp:125-351): Scholz & Stephens' k-sample statistic A2_kN for
+// k = 2 equal samples, standardised by its variance sigma_N
+// (Scholz & Stephens 1987, eq. 4 with the harmonic sums h and g), and a
+// p-value by linear interpolation of the log-odds in the k = 2 column of
+// their quantile table.
+// --------------------------------------------------------------------------
+// quantiles t(p) of the standardised statistic for k = 2 (AD2unique.hpp:8-52,
+// every 8th entry) and the upper-tail probabilities they belong to
+static const double kAdT[35] = {
+    -1.1954, -1.1786, -1.166, -1.1407, -1.1253, -1.0777, -1.0489, -0.9978, -0.9417, -0.8981,
+    -0.8598, -0.7258, -0.5966, -0.4572, -0.2966, -0.1009, 0.1571, 0.5357, 1.2255, 1.5262,
+    1.9633, 2.7314, 3.7825, 4.1241, 4.6044, 5.409, 6.4954, 6.8279, 7.2755, 8.1885,
+    9.3061, 9.6132, 10.0989, 10.8825, 11.8537};
+static const double kAdP[35] = {
+    .00001, .00005, .0001, .0005, .001, .005, .01, .025, .05, .075, .1, .2, .3, .4, .5, .6, .7, .8, .9,
+    .925, .95, .975, .99, .9925, .995, .9975, .999, .99925, .9995, .99975, .9999, .999925,
+    .99995, .999975, .99999};
+
+extern "C" double oracle_ad2_sigma(int n1, int n2) {
+    const int N = n1 + n2;
+    const double H = 1.0 / (1.0 * n1) + 1.0 / (1.0 * n2);
+    double h = 0.0, g = 0.0;
+    std::vector<double> inv(N > 0 ? N : 1, 0.0);
+    if (N < 2000) {
+        for (int i = 1; i < N; ++i) {
+            inv[i] = 1.0 / i;
+            h += inv[i];
+        }
+        for (int i = 1; i < N - 1; ++i)
+            for (int j = i + 1; j < N; ++j) g += inv[N - i] * inv[j];
+    } else {
+        h = std::log(double(N - 1)) + 0.5772156649015328606065120900824024;
+        g = M_PI * M_PI / 6.0;
+    }
+    const double k = 2.0, k2 = std::pow(k, 2);
+    const double a = (4 * g - 6) * (k - 1) + (10 - 6 * g) * H;
+    const double b = (2 * g - 4) * k2 + 8 * h * k + (2 * g - 14 * h - 4) * H - 8 * h + 4 * g - 6;
+    const double c = (6 * h + 2 * g - 2) * k2 + (4 * h - 4 * g + 6) * k + (2 * h - 6) * H + 4 * h;
+    const double d = (2 * h + 6) * k2 - 4 * h * k;
+    double s = 0.0;
+    s += a * std::pow(double(N), 3) + b * std::pow(double(N), 2) + c * N + d;
+    s /= (double(N - 1) * double(N - 2) * double(N - 3));
+    return std::sqrt(s);
+}
+
+extern "C" double oracle_ad2_pvalue_of(double t) {
+    int lo = 0;
+    int first = 35;   // first table entry >= t
+    for (int i = 0; i < 35; ++i)
+        if (t <= kAdT[i]) {
+            first = i;
+            break;
+        }
+    lo = first - 1;
+    if (first == 35) lo = 33;   // beyond the table: last segment
+    if (lo < 0) lo = 0;         // before the table: first segment
+    const int hi = lo + 1;
+    const double l1 = std::log((1.0 - kAdP[lo]) / kAdP[lo]), l2 = std::log((1.0 - kAdP[hi]) / kAdP[hi]);
+    const double l0 = (l1 - l2) * (t - kAdT[hi]) / (kAdT[lo] - kAdT[hi]) + l2;
+    return std::exp(l0) / (1. + std::exp(l0));
+}
+
+// a, b ascending, n each; ties are merged b first.  Only the merge pattern
+// enters: with m_j = number of a among the first j+1 merged values, the
+// inner sums of both samples equal S = sum_{j < 2n-1} (2n m_j - n (j+1))^2 /
+// ((j+1)(2n - j - 1)), accumulated in ascending j.
+extern "C" double oracle_ad2_statistic(const float* a, const float* b, int n) {
+    const int l = 2 * n;
+    const double nsum = l;
+    int ia = 0, ib = 0, m = 0;
+    double S = 0.0;
+    for (int j = 0; j < l - 1; ++j) {
+        const bool take_a = (ia < n && ib < n) ? (a[ia] < b[ib]) : (ia < n);
+        if (take_a) {
+            ++ia;
+            ++m;
+        } else {
+            ++ib;
+        }
+        const double bj = j + 1;
+        const double t = nsum * m - (double)n * bj;
+        S = S + t * t / (bj * (nsum - bj));
+    }
+    double A = 0.0;
+    A = A + S / (n * 1.0);
+    A = A + S / (n * 1.0);
+    A = A / nsum;
+    return (A - 1) / oracle_ad2_sigma(n, n);
+}
+
+extern "C" double oracle_ad2_pvalue(const float* a, const float* b, int n) {
+    return oracle_ad2_pvalue_of(oracle_ad2_statistic(a, b, n));
+}
+
+// --------------------------------------------------------------------------
+// nmap: race-free full-window sweep (nmap.cpp:204-233, :345-473;
+// nmap_cuda.cu:259-334 semantics).  method 0 = KS2, 1 = AD2.  The reference
+// tests a pair (p, q) with p first when q lies in the half-plane it sweeps
+// (row offset > 0, or = 0 with column offset > 0), so the other half of a
+// pixel's window is tested with the neighbour first (AD2 merges ties
+// asymmetrically; KS2 is symmetric).
 //   slc   : [nbands][rows][cols] complex64
 //   mask  : [rows][cols] uint8 or NULL
 //   aconst: [nbands] amplitudeConstant per band, <= 0 meaning "missing"
@@ -194,7 +294,7 @@ extern "C" double oracle_ks2_pvalue(const float* a, const float* b, int n) {
 // --------------------------------------------------------------------------
 extern "C" int oracle_nmap(const float* slc, const uint8_t* mask, int rows, int cols,
                            int nbands, const double* aconst, int Nx, int Ny, double prob,
-                           uint32_t* wts, int32_t* count, int nthreads) {
+                           int method, uint32_t* wts, int32_t* count, int nthreads) {
     const int nulong = oracle_mask_words(Nx, Ny);
     const size_t npx = (size_t)rows * cols;
     std::vector<double> alpha(nbands, 1.0);
@@ -241,7 +341,13 @@ extern "C" int oracle_nmap(const float* slc, const uint8_t* mask, int rows, int 
                     if (q == p) {
                         set_bit(w, 0, 0, Nx, Ny);
                         ++cnt;
-                    } else if (oracle_ks2_pvalue(&amp[p * nbands], &amp[q * nbands], nbands) >= prob) {
+                    } else {
+                        const bool fwd = ii > 0 || (ii == 0 && jj > 0);
+                        const float* x = &amp[(fwd ? p : q) * nbands];
+                        const float* y = &amp[(fwd ? q : p) * nbands];
+                        const double pv = method == 1 ? oracle_ad2_pvalue(x, y, nbands)
+                                                      : oracle_ks2_pvalue(x, y, nbands);
+                        if (!(pv >= prob)) continue;
                         set_bit(w, ii, jj, Nx, Ny);
                         ++cnt;
                     }

@@ -4,6 +4,8 @@ header-only kernels, compiled by oracle/Makefile into oracle/_ref:
   * KS2sample::test p-values (src/nmap/KS2sample.hpp) on seeded sample pairs,
     including tied values;
   * KS2sample::KolmogorovProb on a grid of z;
+  * AD2unique::test p-values (src/nmap/AD2unique.hpp) on seeded pairs,
+    including tied values;
   * Ulongmask bit positions (include/fringe/ulongmask.hpp) for the two window
     sizes tests/bitmask/test_bits.cpp exercises (5x5 and 15x15);
   * EVWorker (include/fringe/EigenLapack.hpp) on the 3x3 known-answer matrix
@@ -83,6 +85,24 @@ def main():
             out[f"herm_{n}_val_{idx}"] = val
             out[f"herm_{n}_vec_{idx}"] = vec
     np.savez(os.path.join(OUT, "eigen.npz"), **out)
+    # AD2unique::test p-values on seeded pairs (own generator, so the files
+    # above do not change when this part does)
+    rng2 = np.random.Generator(np.random.PCG64(2025))
+    ns, A, B, P = [], [], [], []
+    for n in (8, 10, 16, 30, 48, 64, 96):
+        for t in range(40):
+            a = rng2.gamma(2.0, size=n) * rng2.uniform(0.5, 2.0)
+            b = rng2.gamma(2.0, size=n) * rng2.uniform(0.5, 2.0)
+            if t % 4 == 0:   # ties inside and across the samples
+                a, b = np.round(a * 3) / 3, np.round(b * 3) / 3
+            a = np.sort(a.astype(np.float32))
+            b = np.sort(b.astype(np.float32))
+            pa = np.zeros(96, np.float32); pb = np.zeros(96, np.float32)
+            pa[:n] = a; pb[:n] = b
+            ns.append(n); A.append(pa); B.append(pb)
+            P.append(R.ref_ad2_test(a, b, n))
+    np.savez_compressed(os.path.join(OUT, "ad2.npz"), n=np.array(ns, np.int32), a=np.array(A), b=np.array(B),
+                        p=np.array(P))
     print("golden vectors written to", OUT)
 
 

@@ -60,15 +60,16 @@ def test_link_n30_nmap_mask(precision):
     assert st["tcorr_max"] <= TCORR_TOL
 
 
+@pytest.mark.parametrize("method", ["KS2", "AD2"])
 @pytest.mark.parametrize("n", [8, 10, 30, 64])
-def test_nmap_bit_exact(n):
+def test_nmap_bit_exact(n, method):
     rows, cols, Nx, Ny = 20, 28, 5, 3
-    print(f"case nmap n={n}", flush=True)
+    print(f"case nmap n={n} {method}", flush=True)
     slc = synthetic.stack(n, rows, cols, seed=200 + n, patches=6)
-    if n == 10:   # quantised amplitudes exercise the tie branch of the KS walk
+    if n == 10:   # quantised amplitudes exercise the tie branches of the walks
         slc = (np.round(np.abs(slc) * 4) / 4 * np.exp(1j * np.angle(slc))).astype(np.complex64)
-    wr, cr = oracle.nmap(slc, Nx, Ny, 0.05)
-    wg, cg = tile.nmap(to_dev(slc, torch), Nx, Ny, 0.05)
+    wr, cr = oracle.nmap(slc, Nx, Ny, 0.05, method=method)
+    wg, cg = tile.nmap(to_dev(slc, torch), Nx, Ny, 0.05, method=method)
     torch.cuda.synchronize()
     wg = wg.cpu().numpy().view(np.uint32)
     cg = cg.cpu().numpy()

@@ -165,3 +165,27 @@ def test_server_request_bytes_and_create_without_device():
     else:
         assert rc == 203 and not h.value
     assert L.fringe_server_create(0, 0, 2, ctypes.byref(h)) == 201
+
+
+@pytest.mark.parametrize("prob", [0.01, 0.05, 0.1, 0.3])
+def test_ad2_threshold_decides_like_the_oracle_pvalue(prob):
+    L = _lib.lib()
+    sig = ctypes.c_double()
+    rng = np.random.default_rng(int(prob * 1000))
+    for n in (8, 10, 30, 64, 96):
+        thr = L.fringe_ad2_threshold(n, prob, ctypes.byref(sig))
+        assert sig.value == oracle.lib().oracle_ad2_sigma(n, n)
+        # every statistic within 2000 ulps of the threshold is decided as the
+        # p-value decides it (the p-value is monotone there)
+        for k in range(-2000, 2001, 7):
+            t = float(np.nextafter(thr, np.inf if k > 0 else -np.inf)) if k else thr
+            for _ in range(abs(k) - 1 if k else 0):
+                t = float(np.nextafter(t, np.inf if k > 0 else -np.inf))
+            assert (oracle.ad2_pvalue_of(t) >= prob) == (t <= thr)
+        for t in range(200):
+            a = rng.gamma(2.0, size=n)
+            b = rng.gamma(2.0, size=n) * rng.uniform(0.7, 1.4)
+            if t % 3 == 0:
+                a, b = np.round(a * 3) / 3, np.round(b * 3) / 3
+            a, b = np.sort(a.astype(np.float32)), np.sort(b.astype(np.float32))
+            assert (oracle.ad2_statistic(a, b) <= thr) == (oracle.ad2_pvalue(a, b) >= prob)

@@ -128,9 +128,18 @@ def test_eigen_random_hermitian_matches_reference_golden():
 
 
 # --------------------------------------------------------- nmap (oracle) --
+def _amp(slc):
+    """|z| as glibc's hypotf (and std::abs of complex<float>) rounds it: the
+    double square root of the exact double sum of squares, rounded to float
+    (numpy's complex64 abs can differ in the last bit)."""
+    z = slc.astype(np.complex64)
+    re, im = z.real.astype(np.float64), z.imag.astype(np.float64)
+    return np.sqrt(re * re + im * im).astype(np.float32)
+
+
 def _nmap_brute(slc, Nx, Ny, prob):
     nb, rows, cols = slc.shape
-    amp = np.abs(slc.astype(np.complex64))
+    amp = _amp(slc)
     valid = np.all((amp != 0) & ~np.isnan(amp), axis=0)
     srt = np.sort(amp, axis=0)
     nul = oracle.mask_words(Nx, Ny)
@@ -175,6 +184,81 @@ def test_nmap_amplitude_constants_and_mask():
     w2, c2 = oracle.nmap(scaled, 2, 2, 0.05, amp_const=np.array([1.0, 1, 1, 2, 1, 1, 1, 1]))
     w3, c3 = oracle.nmap(slc, 2, 2, 0.05)
     assert np.array_equal(w2, w3) and np.array_equal(c2, c3)
+
+
+def test_ad2_pvalues_match_reference_golden():
+    g = _load("ad2.npz")
+    for n, a, b, p in zip(g["n"], g["a"], g["b"], g["p"]):
+        assert oracle.ad2_pvalue(a[:n], b[:n]) == p
+
+
+def test_ad2_statistic_matches_scipy_anderson_ksamp():
+    # without ties the standardised statistic is scipy's k-sample A2 (k = 2,
+    # midrank=False), computed there from the textbook formula
+    import warnings
+    from scipy.stats import anderson_ksamp
+    rng = np.random.default_rng(3)
+    for n in (8, 10, 30, 64):
+        for t in range(40):
+            a = np.sort(rng.gamma(2.0, size=n)).astype(np.float32)
+            b = np.sort(rng.gamma(2.0, size=n) * rng.uniform(0.6, 1.6)).astype(np.float32)
+            with warnings.catch_warnings():
+                warnings.simplefilter("ignore")
+                s = anderson_ksamp([a.astype(np.float64), b.astype(np.float64)], midrank=False).statistic
+            assert abs(oracle.ad2_statistic(a, b) - s) < 1e-9
+
+
+def test_ad2_pvalue_at_table_points():
+    # the interpolated log-odds pass through every tabulated quantile, where
+    # the p-value is the table's upper-tail probability
+    t = [-1.1954, -0.2966, 1.9633, 3.7825, 11.8537]
+    p = [1 - .00001, 1 - .5, 1 - .95, 1 - .99, 1 - .99999]
+    for tt, pp in zip(t, p):
+        assert abs(oracle.ad2_pvalue_of(tt) - pp) < 1e-12
+    xs = np.linspace(-3, 15, 2001)
+    ps = np.array([oracle.ad2_pvalue_of(x) for x in xs])
+    assert np.all(np.diff(ps) <= 0)       # decreasing in the statistic
+
+
+def test_ad2_nmap_follows_the_reference_half_plane_sweep():
+    # the reference tests (p, q) with p first for q = p + (ii, jj) in its
+    # half plane and sets both bits (nmap.cpp:402-473); ties make the order
+    # matter, so quantised amplitudes are used
+    R = oracle.ref()
+    pv = (lambda a, b: R.ref_ad2_test(a, b, a.size)) if R is not None else oracle.ad2_pvalue
+    slc = synthetic.stack(10, 8, 9, seed=13, patches=3)
+    slc = (np.round(np.abs(slc) * 3) / 3 * np.exp(1j * np.angle(slc))).astype(np.complex64)
+    slc[2, 3, 3] = 0
+    Nx, Ny, prob = 2, 2, 0.05
+    nb, rows, cols = slc.shape
+    amp = _amp(slc)
+    valid = np.all(amp != 0, axis=0)
+    srt = np.sort(amp, axis=0)
+    nul = oracle.mask_words(Nx, Ny)
+    w = np.zeros((rows, cols, nul), np.uint32)
+    cnt = np.zeros((rows, cols), np.int32)
+
+    def setb(r, c, ii, jj):
+        f = (ii + Ny) * (2 * Nx + 1) + jj + Nx
+        w[r, c, f >> 5] |= np.uint32(1 << (f & 31))
+        cnt[r, c] += 1
+
+    for ii in range(0, Ny + 1):
+        for jj in range(0 if ii == 0 else -Nx, Nx + 1):
+            for r in range(rows - ii):
+                for c in range(max(-jj, 0), min(cols - jj, cols)):
+                    rr, cc = r + ii, c + jj
+                    if not (valid[r, c] and valid[rr, cc]):
+                        continue
+                    if (ii, jj) == (0, 0):
+                        setb(r, c, 0, 0)
+                    elif pv(np.ascontiguousarray(srt[:, r, c]), np.ascontiguousarray(srt[:, rr, cc])) >= prob:
+                        setb(r, c, ii, jj)
+                        setb(rr, cc, -ii, -jj)
+    w1, c1 = oracle.nmap(slc, Nx, Ny, prob, method="AD2")
+    assert np.array_equal(w1, w) and np.array_equal(c1, cnt)
+    w2, _ = oracle.nmap(slc, Nx, Ny, prob, method="KS2")
+    assert not np.array_equal(w1, w2)     # the two tests decide differently here
 
 
 # ------------------------------------------------------ linking (oracle) --

@@ -88,8 +88,10 @@ def test_link_generic_solver_n40(precision):
     st = compare_link(g, r, r["gap"])
     print(st)
     assert st["untouched_equal"]
-    tol = PHASE_TOL if precision != "mixed_bf16" else 5 * PHASE_TOL
-    assert st["phase_max"] <= tol and st["tcorr_max"] <= TCORR_TOL
+    # the bf16 hi/lo pair carries ~16 significant bits (fp16 with per-date
+    # scaling ~22): its Gamma error is ~64x larger, so it gets 5x the bounds
+    ptol, ttol = (PHASE_TOL, TCORR_TOL) if precision != "mixed_bf16" else (5 * PHASE_TOL, 5 * TCORR_TOL)
+    assert st["phase_max"] <= ptol and st["tcorr_max"] <= ttol
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

@@ -168,7 +168,7 @@ def parse():
     ap.add_argument("--nx", type=int, default=7)
     ap.add_argument("--ny", type=int, default=5)
     ap.add_argument("--fp64-steps", type=int, default=2)
-    ap.add_argument("--cpu-rows", type=int, default=24, help="output rows of the CPU oracle sample")
+    ap.add_argument("--cpu-rows", type=int, default=512, help="output rows of the CPU oracle sample (about 10 s on 16 cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=102)
     ap.add_argument("--copy-probe", action="store_true", help="run only the copy kernel used to calibrate PMC bytes")

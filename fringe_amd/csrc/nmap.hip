@@ -7,8 +7,8 @@
 //
 // Stage 1 (nmap_prep): per pixel, amplitude |z| / alpha_b for every band,
 // validity, and an ascending sort held in registers (bitonic network).
-// Stage 2 (nmap_sweep): per pixel, the two-sample test against every valid
-// in-image neighbour.  KS2: the statistic as an integer count k = D*n,
+// Stage 2 (nmap_net for n <= 64, nmap_sweep above): per pixel, the
+// two-sample test against every valid in-image neighbour.  KS2: the statistic as an integer count k = D*n,
 // decided through a host-built table (k -> similar) instead of evaluating the
 // Kolmogorov series on the device.  AD2: the standardised Anderson-Darling
 // statistic in double, bit-identical to the host's, compared with the largest
@@ -53,7 +53,8 @@ template <int NP> __device__ inline void bitonic_sort(float (&a)[NP]) {
     }
 }
 
-// One thread per pixel of the whole tile.  amp_sorted is [pixel][n].
+// One thread per pixel of the whole tile.  amp_sorted is date-planar,
+// [rank][pixel], so neighbouring threads read and write neighbouring words.
 template <int NP>
 __global__ __launch_bounds__(256) void nmap_prep(const NmapPlan P, const float2* __restrict__ slc,
                                                  const uint8_t* __restrict__ mask) {
@@ -78,18 +79,18 @@ __global__ __launch_bounds__(256) void nmap_prep(const NmapPlan P, const float2*
     valid[p] = ok ? 1 : 0;
     if (!ok) return;
     bitonic_sort<NP>(a);
-    float* out = amp_sorted + p * n;
 #pragma unroll
     for (int b = 0; b < NP; ++b)
-        if (b < n) out[b] = a[b];
+        if (b < n) amp_sorted[(long long)b * npx + p] = a[b];
 }
 
-// Integer KS walk over two ascending arrays of length n, stepping exactly as
-// KS2sample::test does (KS2sample.hpp:105-138): one element on strict
-// order, every copy of a tied value on both sides on a tie, the running
-// difference sampled after each step while both sides have elements left.
-// Returns k = D * n.
-__device__ inline int ks_count(const float* __restrict__ a, const float* __restrict__ b, int n) {
+// Integer KS walk over two ascending arrays of length n (element i at
+// x[i * st]), stepping exactly as KS2sample::test does
+// (KS2sample.hpp:105-138): one element on strict order, every copy of a tied
+// value on both sides on a tie, the running difference sampled after each
+// step while both sides have elements left.  Returns k = D * n.
+__device__ inline int ks_count(const float* __restrict__ a, const float* __restrict__ b, int n,
+                               long long st) {
     int ia = 0, ib = 0, d = 0, dmax = 0;
     float x = a[0], y = b[0];
     for (int it = 0; it < 2 * n; ++it) {
@@ -101,12 +102,12 @@ __device__ inline int ks_count(const float* __restrict__ a, const float* __restr
             ++ib;
         } else {
             const float v = x;
-            while (ia < n && a[ia] == v) { --d; ++ia; }
-            while (ib < n && b[ib] == v) { ++d; ++ib; }
+            while (ia < n && a[ia * st] == v) { --d; ++ia; }
+            while (ib < n && b[ib * st] == v) { ++d; ++ib; }
         }
         if (ia >= n || ib >= n) break;
-        x = a[ia];
-        y = b[ib];
+        x = a[ia * st];
+        y = b[ib * st];
         dmax = max(dmax, abs(d));
     }
     return max(dmax, abs(d));
@@ -118,7 +119,7 @@ __device__ inline int ks_count(const float* __restrict__ a, const float* __restr
 // counts the a values among the first j+1.  The terms are tabulated once per
 // call (ad2_terms), so the double sum adds the same values in the same order.
 __device__ inline double ad2_inner(const float* __restrict__ a, const float* __restrict__ b, int n,
-                                   const double* __restrict__ T) {
+                                   long long st, const double* __restrict__ T) {
     int ia = 0, ib = 0, m = 0;
     double S = 0.0;
     float x = a[0], y = b[0];
@@ -127,10 +128,10 @@ __device__ inline double ad2_inner(const float* __restrict__ a, const float* __r
         if (take_a) {
             ++ia;
             ++m;
-            x = ia < n ? a[ia] : x;
+            x = ia < n ? a[ia * st] : x;
         } else {
             ++ib;
-            y = ib < n ? b[ib] : y;
+            y = ib < n ? b[ib * st] : y;
         }
         S = S + T[j * (n + 1) + m];
     }
@@ -148,7 +149,124 @@ __global__ void ad2_terms(double* __restrict__ T, int n) {
     T[idx] = t * t / (bj * (nsum - bj));
 }
 
-// One thread per output pixel (rows [row_begin, row_end) of the tile).
+__device__ inline double ad2_standardise(double S, int n, double sigma) {
+    double A = 0.0;
+    A = A + S / (n * 1.0);
+    A = A + S / (n * 1.0);
+    A = A / (2.0 * n);
+    return (A - 1) / sigma;
+}
+
+__device__ inline bool ks_similar(int k, const NmapPlan& P) {
+    const uint32_t tw = k < 32 ? P.table.bits[0] : k < 64 ? P.table.bits[1] : k < 96 ? P.table.bits[2]
+                        : k < 128 ? P.table.bits[3] : P.table.bits[4];
+    return ((tw >> (k & 31)) & 1u) != 0u;
+}
+
+// ---------------------------------------------------------------------------
+// n <= 64: merge network in registers.  Positive floats order like their bit
+// patterns, so key = bits << 1 | tag orders by value and, among equal values,
+// by tag.  The pixel's own keys stay in registers; a neighbour's keys are read
+// coalesced (date-planar layout) and the two sorted runs are merged by a
+// bitonic network (own keys ascending, neighbour's descending).
+//   KS2: tag 1 = neighbour.  d after a prefix = #neighbour - #own; the walk of
+//        KS2sample::test samples d exactly at the ends of runs of equal
+//        values (inside a run of one sample d is monotone, so the samples
+//        the walk takes there add nothing), so k = max |d| over positions
+//        whose successor has a different value.
+//   AD2: tag 1 = the test's first sample, so ties put the second sample
+//        first as AD2unique::test merges them; m_j = number of tag-1 keys in
+//        the first j+1 positions.
+// Pads (beyond n) get keys above every finite or infinite amplitude.
+// ---------------------------------------------------------------------------
+template <int NP, bool AD2>
+__global__ __launch_bounds__(256) void nmap_net(const NmapPlan P, uint32_t* __restrict__ wts,
+                                                int32_t* __restrict__ count) {
+    const uint32_t* __restrict__ keys = reinterpret_cast<const uint32_t*>(P.d_amp);
+    const uint8_t* __restrict__ valid = P.d_valid;
+    const int n = P.n, rows = P.rows, cols = P.cols, Nx = P.Nx, Ny = P.Ny, nul = P.nul;
+    const long long npx = (long long)rows * cols;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nout = (long long)(P.row_end - P.row_begin) * cols;
+    if (t >= nout) return;
+    const int r = P.row_begin + (int)(t / cols);
+    const int c = (int)(t % cols);
+    const long long p = (long long)r * cols + c;
+    uint32_t* out = wts + p * nul;
+    const bool pv = valid[p] != 0;
+    uint32_t A[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) A[i] = (pv && i < n) ? (keys[i * npx + p] << 1) : 0xFFFFFFFEu;
+    int cnt = 0, word = 0, flat = 0;
+    uint32_t cur = 0u;
+    for (int dy = -Ny; dy <= Ny; ++dy) {
+        const int rr = r + dy;
+        for (int dx = -Nx; dx <= Nx; ++dx, ++flat) {
+            if ((flat >> 5) != word) {
+                out[word] = cur;
+                cur = 0u;
+                word = flat >> 5;
+            }
+            const int cc = c + dx;
+            if (!pv || rr < 0 || rr >= rows || cc < 0 || cc >= cols) continue;
+            const long long q = (long long)rr * cols + cc;
+            if (!valid[q]) continue;
+            bool similar = true;
+            if (q != p) {
+                // own tag: KS2 0; AD2 1 when the pixel is the test's first sample
+                const bool fwd = dy > 0 || (dy == 0 && dx > 0);
+                const uint32_t tp = (AD2 && fwd) ? 1u : 0u, tq = tp ^ 1u;
+                uint32_t S[2 * NP];
+#pragma unroll
+                for (int i = 0; i < NP; ++i) {
+                    S[i] = A[i] | tp;
+                    S[2 * NP - 1 - i] = (i < n ? (keys[i * npx + q] << 1) : 0xFFFFFFFEu) | tq;
+                }
+#pragma unroll
+                for (int s = NP; s >= 1; s >>= 1)
+#pragma unroll
+                    for (int i = 0; i < 2 * NP; ++i)
+                        if ((i & s) == 0) {
+                            const uint32_t x = S[i], y = S[i + s];
+                            S[i] = min(x, y);
+                            S[i + s] = max(x, y);
+                        }
+                if constexpr (AD2) {
+                    // first sample = tag 1
+                    int m = 0;
+                    double Ssum = 0.0;
+#pragma unroll
+                    for (int j = 0; j < 2 * NP - 1; ++j) {
+                        m += (int)(S[j] & 1u);
+                        if (j < 2 * n - 1) Ssum = Ssum + P.d_ad2[j * (n + 1) + m];
+                    }
+                    similar = ad2_standardise(Ssum, n, P.ad2_sigma) <= P.ad2_max;
+                } else {
+                    // d_j = 2 (#tag-1 in the first j+1) - (j+1)
+                    int nb = 0, hi = 0, lo = 0;
+#pragma unroll
+                    for (int j = 0; j < 2 * NP - 1; ++j) {
+                        nb += (int)(S[j] & 1u);
+                        const int d = 2 * nb - (j + 1);
+                        const bool end = ((S[j] ^ S[j + 1]) > 1u) && (j < 2 * n);
+                        hi = end ? max(hi, d) : hi;
+                        lo = end ? min(lo, d) : lo;
+                    }
+                    similar = ks_similar(max(hi, -lo), P);
+                }
+            }
+            if (similar) {
+                cur |= (1u << (flat & 31));
+                ++cnt;
+            }
+        }
+    }
+    out[word] = cur;
+    for (int w = word + 1; w < nul; ++w) out[w] = 0u;
+    count[p] = cnt;
+}
+
+// n > 64: one thread per output pixel walks the sorted arrays in memory.
 // The reference sweeps half the window and tests (p, q) with p first for
 // q = p + (dy, dx), dy > 0 or dy = 0 < dx (nmap.cpp:402-457); the other half
 // of a pixel's window is therefore tested with the neighbour first.
@@ -158,19 +276,16 @@ __global__ __launch_bounds__(256) void nmap_sweep(const NmapPlan P, uint32_t* __
     const float* __restrict__ amp_sorted = P.d_amp;
     const uint8_t* __restrict__ valid = P.d_valid;
     const int n = P.n, rows = P.rows, cols = P.cols, Nx = P.Nx, Ny = P.Ny, nul = P.nul;
-    const int row_begin = P.row_begin, row_end = P.row_end;
-    // decision bits k -> similar, held in scalars and picked by a select chain
-    const uint32_t t0 = P.table.bits[0], t1 = P.table.bits[1], t2 = P.table.bits[2],
-                   t3 = P.table.bits[3], t4 = P.table.bits[4];
+    const long long npx = (long long)rows * cols;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long nout = (long long)(row_end - row_begin) * cols;
+    const long long nout = (long long)(P.row_end - P.row_begin) * cols;
     if (t >= nout) return;
-    const int r = row_begin + (int)(t / cols);
+    const int r = P.row_begin + (int)(t / cols);
     const int c = (int)(t % cols);
     const long long p = (long long)r * cols + c;
     uint32_t* out = wts + p * nul;
     const bool pv = valid[p] != 0;
-    const float* a = amp_sorted + p * n;
+    const float* a = amp_sorted + p;
     // Window positions are visited in bit order (flat = 0, 1, 2, ...), so
     // the word being filled only ever advances by one; each finished word is
     // stored once.
@@ -190,19 +305,13 @@ __global__ __launch_bounds__(256) void nmap_sweep(const NmapPlan P, uint32_t* __
             if (!valid[q]) continue;
             bool similar = true;
             if (q != p) {
-                const float* b = amp_sorted + q * n;
+                const float* b = amp_sorted + q;
                 if constexpr (AD2) {
                     const bool fwd = dy > 0 || (dy == 0 && dx > 0);
-                    const double S = ad2_inner(fwd ? a : b, fwd ? b : a, n, P.d_ad2);
-                    double A = 0.0;
-                    A = A + S / (n * 1.0);
-                    A = A + S / (n * 1.0);
-                    A = A / (2.0 * n);
-                    similar = (A - 1) / P.ad2_sigma <= P.ad2_max;
+                    const double S = ad2_inner(fwd ? a : b, fwd ? b : a, n, npx, P.d_ad2);
+                    similar = ad2_standardise(S, n, P.ad2_sigma) <= P.ad2_max;
                 } else {
-                    const int k = ks_count(a, b, n);
-                    const uint32_t tw = k < 32 ? t0 : k < 64 ? t1 : k < 96 ? t2 : k < 128 ? t3 : t4;
-                    similar = ((tw >> (k & 31)) & 1u) != 0u;
+                    similar = ks_similar(ks_count(a, b, n, npx), P);
                 }
             }
             if (similar) {
@@ -224,6 +333,13 @@ static void launch_prep(const NmapPlan& P, const float2* slc, const uint8_t* mas
     hipLaunchKernelGGL(nmap_prep<NP>, dim3((unsigned)blocks), dim3(threads), 0, s, P, slc, mask);
 }
 
+template <int NP>
+static void launch_net(bool ad2, dim3 g, dim3 blk, const NmapPlan& P, uint32_t* wts, int32_t* count,
+                       hipStream_t s) {
+    if (ad2) hipLaunchKernelGGL((nmap_net<NP, true>), g, blk, 0, s, P, wts, count);
+    else hipLaunchKernelGGL((nmap_net<NP, false>), g, blk, 0, s, P, wts, count);
+}
+
 int nmap_launch(const NmapPlan& P, const float2* slc, const uint8_t* mask, uint32_t* wts,
                 int32_t* count, hipStream_t s) {
     {
@@ -237,15 +353,20 @@ int nmap_launch(const NmapPlan& P, const float2* slc, const uint8_t* mask, uint3
     const long long nout = (long long)(P.row_end - P.row_begin) * P.cols;
     if (nout > 0) {
         const int threads = 256;
-        const long long blocks = (nout + threads - 1) / threads;
+        const unsigned blocks = (unsigned)((nout + threads - 1) / threads);
         StageScope sc(STAGE_NMAP_KS, s);
         if (P.method == 1) {
             const int nt = (2 * P.n - 1) * (P.n + 1);
             hipLaunchKernelGGL(ad2_terms, dim3((nt + 255) / 256), dim3(256), 0, s, P.d_ad2, P.n);
-            hipLaunchKernelGGL(nmap_sweep<true>, dim3((unsigned)blocks), dim3(threads), 0, s, P, wts, count);
-        } else {
-            hipLaunchKernelGGL(nmap_sweep<false>, dim3((unsigned)blocks), dim3(threads), 0, s, P, wts, count);
         }
+        const bool ad2 = P.method == 1;
+        const dim3 g(blocks), blk(threads);
+        if (P.n <= 8) launch_net<8>(ad2, g, blk, P, wts, count, s);
+        else if (P.n <= 16) launch_net<16>(ad2, g, blk, P, wts, count, s);
+        else if (P.n <= 32) launch_net<32>(ad2, g, blk, P, wts, count, s);
+        else if (P.n <= 64) launch_net<64>(ad2, g, blk, P, wts, count, s);
+        else if (ad2) hipLaunchKernelGGL(nmap_sweep<true>, g, blk, 0, s, P, wts, count);
+        else hipLaunchKernelGGL(nmap_sweep<false>, g, blk, 0, s, P, wts, count);
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -12,8 +12,9 @@
 //     from the published pivot row instead of from a runtime register index;
 //   * H = |Gamma|^-1 o Gamma (or Gamma itself for EVD) into registers;
 //   * Householder tridiagonalisation: the owner of row j publishes it, the
-//     reflector v_j = conj(row j) * scal_j is applied, and row j is never
-//     touched again, so it still holds v_j for the back-transformation;
+//     reflector v_j = conj(row j) * scal_j is applied, and every lane parks
+//     its own entry of v_j in the register of the column just annihilated,
+//     where the back-transformation finds it;
 //   * eigenvalue by 33-section of Sturm counts, eigenvector of T by inverse
 //     iteration (every lane runs the same pivoted elimination chain on
 //     broadcast operands), back-transformation by the stored reflectors;
@@ -28,7 +29,6 @@ template <typename T> struct Lds32 {
     cx<T> V[32];
     cx<T> W[32];
     cx<T> TAU[32];
-    cx<T> SCAL[32];
     cx<T> YREF;
     T D[32], E[32], E2[32];
     T B[32];          // inverse-iteration right-hand side
@@ -171,7 +171,6 @@ __device__ inline void tri_step(Lds32<T>& L, cx<T> (&h)[32], int j, int n, int r
         L.D[j] = L.R[j].re;
         L.E[j] = beta;
         L.TAU[j] = tau;
-        L.SCAL[j] = scal;
     }
     wsync();
     cx<T> p = {T(0), T(0)};
@@ -188,18 +187,25 @@ __device__ inline void tri_step(Lds32<T>& L, cx<T> (&h)[32], int j, int n, int r
         cfms_c(h[c], vr, L.W[c]);
         cfms_c(h[c], wr, L.V[c]);
     }
+    // column j of the rows below j+1 has just been annihilated: park this
+    // lane's entry of v_j there for the back-transformation (no later step
+    // reads it: V and W vanish at column j from here on)
+#pragma unroll
+    for (int c = (C0 > 0 ? C0 - 1 : 0); c < (C0 + 8 < 32 ? C0 + 8 : 32); ++c)
+        if (c == j && r > j + 1) h[c] = vr;
     wsync();
 }
 
 template <typename T>
 __device__ inline void tridiag32(Lds32<T>& L, cx<T> (&h)[32], int n, int r) {
-    for (int j = 0; j < n - 1; ++j) {
-        // columns below j+1 carry zero reflector entries; skip whole octets
-        if (j < 7) tri_step<T, 0>(L, h, j, n, r);
-        else if (j < 15) tri_step<T, 8>(L, h, j, n, r);
-        else if (j < 23) tri_step<T, 16>(L, h, j, n, r);
-        else tri_step<T, 24>(L, h, j, n, r);
-    }
+    // columns below j+1 carry zero reflector entries: one loop per octet of
+    // columns left to update (separate loops keep the register allocation of
+    // each variant to itself)
+    int j = 0;
+    for (; j < min(7, n - 1); ++j) tri_step<T, 0>(L, h, j, n, r);
+    for (; j < min(15, n - 1); ++j) tri_step<T, 8>(L, h, j, n, r);
+    for (; j < min(23, n - 1); ++j) tri_step<T, 16>(L, h, j, n, r);
+    for (; j < n - 1; ++j) tri_step<T, 24>(L, h, j, n, r);
     // last diagonal entry
     if (r == n - 1) {
 #pragma unroll
@@ -322,20 +328,16 @@ __device__ inline T tri_eigvec32(Lds32<T>& L, int n, int r, T lam, T tnorm) {
     return (r < n) ? xme : T(0);
 }
 
-// y <- Q y with Q = H_0 ... H_{n-2} from the rows kept in h.
+// y <- Q y with Q = H_0 ... H_{n-2}; entry r > j+1 of v_j is h[j] of lane r
+// (unrolled over j so the register index is static).
 template <typename T>
 __device__ inline cx<T> back_transform32(Lds32<T>& L, const cx<T> (&h)[32], int n, int r, cx<T> y) {
-    for (int j = n - 2; j >= 0; --j) {
-        if (r == j) {
 #pragma unroll
-            for (int c = 0; c < 32; ++c) L.R[c] = h[c];
-        }
-        wsync();
-        const cx<T> v = (r == j + 1) ? cx<T>{T(1), T(0)}
-                        : (r > j + 1 && r < n) ? cmul(cconj(L.R[r]), L.SCAL[j]) : cx<T>{T(0), T(0)};
+    for (int j = 30; j >= 0; --j) {
+        if (j > n - 2) continue;   // uniform
+        const cx<T> v = (r == j + 1) ? cx<T>{T(1), T(0)} : (r > j + 1 && r < n) ? h[j] : cx<T>{T(0), T(0)};
         const cx<T> s = sum32(cmulc(y, v));
         y = csub(y, cmul(v, cmul(L.TAU[j], s)));
-        wsync();
     }
     return y;
 }
@@ -361,18 +363,24 @@ template <typename T, typename Tg>
 __device__ inline bool hadamard_rows(cx<T> (&h)[32], const double (&m)[32], double dr,
                                      const cx<Tg>* grow, int n, int r, bool act) {
     bool fin = true;
+    // eight columns at a time, so that the row of Gamma is not loaded whole
+    // while the FP64 row of the inverse is still live
 #pragma unroll
-    for (int c = 0; c < 32; ++c) {
-        cx<T> v = {T(0), T(0)};
-        if (r < n && c < n) {
-            const double mm = (c == r) ? dr : m[c];
-            cx<Tg> g = (c == r) ? cx<Tg>{Tg(1), Tg(0)} : cx<Tg>{Tg(0), Tg(0)};
-            if (act && c != r) g = grow[c];
-            const double hr = mm * (double)g.re, hi = mm * (double)g.im;
-            fin = fin && isfinite(hr) && isfinite(hi);
-            v = cx<T>{(T)hr, (T)hi};
+    for (int c0 = 0; c0 < 32; c0 += 8) {
+#pragma unroll
+        for (int c = c0; c < c0 + 8; ++c) {
+            cx<T> v = {T(0), T(0)};
+            if (r < n && c < n) {
+                const double mm = (c == r) ? dr : m[c];
+                cx<Tg> g = (c == r) ? cx<Tg>{Tg(1), Tg(0)} : cx<Tg>{Tg(0), Tg(0)};
+                if (act && c != r) g = grow[c];
+                const double hr = mm * (double)g.re, hi = mm * (double)g.im;
+                fin = fin && isfinite(hr) && isfinite(hi);
+                v = cx<T>{(T)hr, (T)hi};
+            }
+            h[c] = v;
         }
-        h[c] = v;
+        __builtin_amdgcn_sched_barrier(0);
     }
     return min32(fin ? 1.0f : 0.0f) > 0.5f;
 }

@@ -19,9 +19,9 @@
 //
 // Data flow:
 //   pack_hilo : the band-planar stack slc[t][row][col] is rewritten once as
-//               position-major packed words pk[pixel][re_0..re_{n-1},
-//               im_0..im_{n-1}] with (hi | lo << 16); one extra all-zero
-//               record at index rows*cols pads neighbour lists.
+//               position-major packed words pk[pixel][re_0, im_0, re_1,
+//               im_1, ...] with (hi | lo << 16); one extra all-zero record at
+//               index rows*cols pads neighbour lists.
 //   gram_mfma : one wave per pixel at a time.  The wave compacts the mask
 //               into a list of pixel indices (padded to a multiple of 16) in
 //               LDS; per 16 positions lane l gathers component (l % 32) of 8
@@ -106,11 +106,11 @@ __global__ __launch_bounds__(256) void pack_hilo(const float2* __restrict__ slc,
         if (p < npx) z = slc[(long long)t * npx + p];
         if (F16) {
             const int k = date_scale_exp(dmax[t]);
-            st[i * CS + t] = split_hi_lo_f16(ldexpf(z.x, k));
-            st[i * CS + n + t] = split_hi_lo_f16(ldexpf(z.y, k));
+            st[i * CS + 2 * t] = split_hi_lo_f16(ldexpf(z.x, k));
+            st[i * CS + 2 * t + 1] = split_hi_lo_f16(ldexpf(z.y, k));
         } else {
-            st[i * CS + t] = split_hi_lo_bf16(z.x);
-            st[i * CS + n + t] = split_hi_lo_bf16(z.y);
+            st[i * CS + 2 * t] = split_hi_lo_bf16(z.x);
+            st[i * CS + 2 * t + 1] = split_hi_lo_bf16(z.y);
         }
     }
     __syncthreads();
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(256) void pack_hilo(const float2* __restrict__ slc,
 }
 
 struct MfmaArgs {
-    const uint32_t* pk;   // packed records, [rows*cols + 1][2n]
+    const uint32_t* pk;   // packed records, [rows*cols + 1][n][re, im]
     const uint32_t* wts;
     int rows, cols, n, Nx, Ny, nul, min_nb;
     int row0;             // first output row of this chunk
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(64 * WAVES) void gram_mfma(MfmaArgs a) {
     for (int b = 0; b < NB; ++b) {
         const int date = 32 * (b % NBH) + m;
         cval[b] = date < n;
-        coff[b] = cval[b] ? (uint32_t)((b < NBH) ? date : n + date) : 0u;
+        coff[b] = cval[b] ? (uint32_t)(2 * date + (b < NBH ? 0 : 1)) : 0u;
     }
 
     for (int pid = wave; pid < a.TH * a.TW; pid += WAVES) {
@@ -247,7 +247,9 @@ __global__ __launch_bounds__(64 * WAVES) void gram_mfma(MfmaArgs a) {
             for (int b = 0; b < NB; ++b) {
                 uint32_t wv[8];
 #pragma unroll
-                for (int e = 0; e < 8; ++e) wv[e] = cval[b] ? a.pk[base[e] + coff[b]] : 0u;
+                for (int e = 0; e < 8; ++e) wv[e] = a.pk[base[e] + coff[b]];   // valid address for every lane
+#pragma unroll
+                for (int e = 0; e < 8; ++e) wv[e] = cval[b] ? wv[e] : 0u;
                 u32x4 hv, lv;
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
@@ -317,6 +319,191 @@ __global__ __launch_bounds__(64 * WAVES) void gram_mfma(MfmaArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// n <= 32: the records of the union of a workgroup's windows are staged in
+// LDS once, and the waves take their pixels' neighbour lists from there.
+// A TH x TW pixel tile needs (TH + 2Ny) x (TW + 2Nx) positions: for 4 x 8
+// pixels and a 15 x 11 window that is 308 records for 32 pixels, against
+// 165 per pixel gathered one by one.  All four real 32 x 32 tiles
+// P(re,re), P(im,im), P(re,im), P(im,re) are accumulated, so the fold needs
+// no transpose: Re G = RR + II, Im G(i,j) = P(im_i,re_j) - P(re_i,im_j).
+// Workgroups are numbered so that each XCD (which owns an L2) takes a
+// contiguous run of tiles and the halos neighbouring tiles share stay in it.
+// ---------------------------------------------------------------------------
+template <bool F16>
+__global__ __launch_bounds__(256) void gram_mfma_lds(MfmaArgs a) {
+    typedef typename Frag<F16>::type frag_t;
+    extern __shared__ __align__(16) char smem[];
+    const int n = a.n, Nx = a.Nx, Ny = a.Ny;
+    const int TH = a.TH, TW = a.TW;
+    const int WR = TH + 2 * Ny, WC = TW + 2 * Nx, NPOS = WR * WC;
+    uint2* rec = (uint2*)smem;                             // [NPOS + 1][32] (re, im) words
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int* list = (int*)(rec + (size_t)(NPOS + 1) * 32) + wave * (a.list_cap + 32);
+    float* dg = (float*)(list + a.list_cap);                // diagonal of Re G, 32
+
+    // XCD-aware tile order (blocks are dealt to the 8 XCDs round-robin)
+    const unsigned nb = gridDim.x * gridDim.y, bid = blockIdx.y * gridDim.x + blockIdx.x;
+    const unsigned xcd = bid % 8u, q = nb / 8u, rem = nb % 8u;
+    const unsigned logical = xcd * q + min(xcd, rem) + bid / 8u;
+    const int tx = (int)(logical % gridDim.x), ty = (int)(logical / gridDim.x);
+    const int R0 = a.row0 + ty * TH, C0 = tx * TW;
+    const int row_end = a.row0 + a.chunk_rows;
+    const int rbase = R0 - Ny, cbase = C0 - Nx;
+
+    // ---- stage the records, padded to 32 dates (zeros beyond n and outside
+    //      the image), four loads in flight ----
+    {
+        const int total = (NPOS + 1) * 32;
+        const uint2* src = (const uint2*)a.pk;
+        for (int e0 = threadIdx.x; e0 < total; e0 += 4 * 256) {
+            uint2 v[4];
+            bool in[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = e0 + u * 256;
+                const int pos = e >> 5, d = e & 31;
+                const int rr = rbase + pos / WC, cc = cbase + pos % WC;
+                in[u] = e < total && d < n && pos < NPOS && rr >= 0 && rr < a.rows && cc >= 0 && cc < a.cols;
+                // slots without data read record 0 and are zeroed below
+                const size_t at = in[u] ? ((size_t)rr * a.cols + cc) * n + d : 0;
+                v[u] = src[at];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = e0 + u * 256;
+                if (e < total) rec[e] = in[u] ? v[u] : make_uint2(0u, 0u);
+            }
+        }
+    }
+    __syncthreads();
+
+    const int Wx = 2 * Nx + 1, Wn = (2 * Ny + 1) * Wx;
+    const int m = lane & 31, hf = lane >> 5;
+    // window offsets of this lane in every 64-wide pass (pixel-independent)
+    constexpr int kPass = (16 * 32 + 63) / 64;
+    int offy[kPass], offx[kPass];
+#pragma unroll
+    for (int it = 0; it < kPass; ++it) {
+        const int f = it * 64 + lane;
+        offy[it] = f / Wx;
+        offx[it] = f - offy[it] * Wx;
+    }
+
+    for (int pid = wave; pid < TH * TW; pid += 4) {
+        const int r = R0 + pid / TW, c = C0 + pid % TW;
+        if (r >= row_end || r >= a.rows || c >= a.cols) continue;   // wave-uniform
+        const long long pix = (long long)r * a.cols + c;
+        const int slot = (r - a.row0) * a.cols + c;
+        // mask words in lanes 0..nul-1, one load for the pixel
+        const uint32_t wword = a.wts[pix * a.nul + (lane < a.nul ? lane : 0)];
+        const int fc = shp_flat(0, 0, Nx, Ny);
+        const bool cen = (__shfl(wword, fc >> 5) >> (fc & 31)) & 1u;
+        // ---- neighbour list (staged record indices) ----
+        const int lr = r - R0, lc = c - C0;                  // window origin in the stage
+        int cnt = 0;
+#pragma unroll
+        for (int it = 0; it < kPass; ++it) {
+            if (it * 64 >= Wn) break;   // uniform
+            const int f = it * 64 + lane;
+            const uint32_t wf = __shfl(wword, (f < Wn ? f : 0) >> 5);
+            bool take = false;
+            int qpos = 0;
+            if (cen && f < Wn) {
+                const int fy = offy[it], fx = offx[it];
+                const int rr = r + fy - Ny, cc = c + fx - Nx;
+                if (rr >= 0 && rr < a.rows && cc >= 0 && cc < a.cols && ((wf >> (f & 31)) & 1u)) {
+                    take = true;
+                    qpos = (lr + fy) * WC + (lc + fx);
+                }
+            }
+            const uint64_t bal = __ballot(take);
+            if (take) {
+                const uint64_t below = (lane == 0) ? 0ull : (bal & ((~0ull) >> (64 - lane)));
+                list[cnt + __popcll(below)] = qpos;
+            }
+            cnt += __popcll(bal);
+        }
+        if (!(cen && cnt >= a.min_nb)) {
+            if (lane == 0) a.npix[slot] = -1;
+            continue;
+        }
+        const int steps = (cnt + 15) >> 4;
+        for (int e = cnt + lane; e < steps * 16; e += 64) list[e] = NPOS;
+        wave_sync_lds();
+
+        f32x16 RR, II, RI, IR;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) RR[v] = II[v] = RI[v] = IR[v] = 0.0f;
+        // operands of step st + 1 are loaded while step st's MFMAs run; one
+        // 8-byte read gives this lane's (re, im) words of a position
+        uint2 x[8];
+        auto load_step = [&](int st, uint2 (&y)[8]) {
+            const int* lp = list + 16 * st + 8 * hf;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) y[e] = rec[(uint32_t)lp[e] * 32u + (uint32_t)m];
+        };
+        load_step(0, x);
+        for (int st = 0; st < steps; ++st) {
+            uint2 nx[8];
+            load_step(st + 1 < steps ? st + 1 : st, nx);
+            u32x4 hr, lr4, hi4, li4;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                hr[e] = __builtin_amdgcn_perm(x[2 * e + 1].x, x[2 * e].x, 0x05040100u);
+                lr4[e] = __builtin_amdgcn_perm(x[2 * e + 1].x, x[2 * e].x, 0x07060302u);
+                hi4[e] = __builtin_amdgcn_perm(x[2 * e + 1].y, x[2 * e].y, 0x05040100u);
+                li4[e] = __builtin_amdgcn_perm(x[2 * e + 1].y, x[2 * e].y, 0x07060302u);
+            }
+            const frag_t fhr = __builtin_bit_cast(frag_t, hr), flr = __builtin_bit_cast(frag_t, lr4);
+            const frag_t fhi = __builtin_bit_cast(frag_t, hi4), fli = __builtin_bit_cast(frag_t, li4);
+            // interleaved so that consecutive MFMAs feed different accumulators
+            RR = Frag<F16>::mfma(fhr, fhr, RR);
+            II = Frag<F16>::mfma(fhi, fhi, II);
+            RI = Frag<F16>::mfma(fhr, fhi, RI);
+            IR = Frag<F16>::mfma(fhi, fhr, IR);
+            RR = Frag<F16>::mfma(fhr, flr, RR);
+            II = Frag<F16>::mfma(fhi, fli, II);
+            RI = Frag<F16>::mfma(fhr, fli, RI);
+            IR = Frag<F16>::mfma(fhi, flr, IR);
+            RR = Frag<F16>::mfma(flr, fhr, RR);
+            II = Frag<F16>::mfma(fli, fhi, II);
+            RI = Frag<F16>::mfma(flr, fhi, RI);
+            IR = Frag<F16>::mfma(fli, fhr, IR);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) x[e] = nx[e];
+        }
+
+        // ---- fold P into Gamma ----
+#pragma unroll
+        for (int v = 0; v < 16; ++v)
+            if (acc_row(v, lane) == m) dg[m] = RR[v] + II[v];
+        wave_sync_lds();
+        float* gout = a.gamma + (size_t)slot * n * n * 2;
+        const float dj = dg[m];
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int i = acc_row(v, lane);
+            if (i >= n || m >= n) continue;
+            float gr = 1.0f, gi = 0.0f;
+            if (i != m) {
+                const float sc = rsqrtf(dg[i] * dj);
+                gr = (RR[v] + II[v]) * sc;
+                gi = (IR[v] - RI[v]) * sc;   // P(im_i, re_j) - P(re_i, im_j)
+            }
+            reinterpret_cast<float2*>(gout)[(size_t)i * n + m] = make_float2(gr, gi);
+        }
+        if (lane == 0) a.npix[slot] = cnt;
+        wave_sync_lds();
+    }
+}
+
+static size_t gram_lds_bytes(int n, int Nx, int Ny, int TH, int TW, int list_cap) {
+    (void)n;   // records are staged padded to 32 dates
+    const size_t stage = (size_t)((TH + 2 * Ny) * (TW + 2 * Nx) + 1) * 32 * 8;
+    return stage + (size_t)4 * (list_cap + 32) * 4;
+}
+
 // packed records followed by the per-date maxima (256-byte aligned)
 size_t mfma_pack_bytes(int rows, int cols, int n) {
     const size_t rec = ((size_t)rows * cols + 1) * 2 * n * sizeof(uint32_t);
@@ -372,8 +559,23 @@ int gram_mfma_launch(const LinkPlan& P, const uint32_t* pk, const uint32_t* wts,
     a.list_cap = ((Wn + 15) / 16) * 16 + 16;
     const bool f16 = (P.precision != FRINGE_PREC_MIXED_BF16);
     if (P.n <= 32) {
-        if (f16) launch_mfma<1, 4, true>(a, s);
-        else launch_mfma<1, 4, false>(a, s);
+        // largest tile whose stage keeps two workgroups on a CU
+        const size_t budget = 78 * 1024;
+        int TH = 4, TW = 8;
+        while (TH > 1 && gram_lds_bytes(P.n, P.Nx, P.Ny, TH, TW, a.list_cap) > budget) --TH;
+        while (TW > 1 && gram_lds_bytes(P.n, P.Nx, P.Ny, TH, TW, a.list_cap) > budget) TW >>= 1;
+        const size_t bytes = gram_lds_bytes(P.n, P.Nx, P.Ny, TH, TW, a.list_cap);
+        if (bytes <= 160 * 1024) {
+            a.TH = TH;
+            a.TW = TW;
+            const dim3 grid((unsigned)((a.cols + TW - 1) / TW), (unsigned)((a.chunk_rows + TH - 1) / TH));
+            if (f16) hipLaunchKernelGGL(gram_mfma_lds<true>, grid, dim3(256), bytes, s, a);
+            else hipLaunchKernelGGL(gram_mfma_lds<false>, grid, dim3(256), bytes, s, a);
+        } else if (f16) {
+            launch_mfma<1, 4, true>(a, s);
+        } else {
+            launch_mfma<1, 4, false>(a, s);
+        }
     } else {
         if (f16) launch_mfma<2, 4, true>(a, s);
         else launch_mfma<2, 4, false>(a, s);

@@ -117,26 +117,27 @@ template <typename T>
 __device__ inline bool sweep32(Lds32<T>& L, double (&m)[32], double& dr, int n, int r) {
     bool ok = true;
     for (int j = 0; j < n; ++j) {
+        // the pivot row is published unscaled; every lane folds 1/p into its
+        // own multiplier (T = M_j. / p is never formed)
         if (r == j) {
-            const double pv = fast_rcp(dr);
 #pragma unroll
-            for (int c = 0; c < 32; ++c) L.TR[c] = m[c] * pv;
-            L.TR[j] = 1.0 + pv;
+            for (int c = 0; c < 32; ++c) L.TR[c] = m[c];
+            L.TR[j] = dr + 1.0;   // (p + 1) / p = 1 + 1/p after the scaling
             L.PV = dr;
         }
         wsync();
         const double p = L.PV;
         ok = ok && (p > 0.0);
         const double piv = fast_rcp(p);
-        const double trr = L.TR[r];
+        const double mjr = L.TR[r];
         // rows r != j: M_rc -= M_rj T_c with M_rj = s_r M_jr read from the
-        // published row; the pivot row becomes T = M_j. / p, which is
-        // M_jc - (p - 1) T_c, so one FMA form covers every lane.
+        // published row; the pivot row becomes M_j. / p = M_jc - ((p - 1) / p) M_jc,
+        // so one FMA form covers every lane
         const bool me = (r == j);
-        const double f = me ? p - 1.0 : (r < j ? -trr : trr) * p;
+        const double g = (me ? p - 1.0 : (r < j ? -mjr : mjr)) * piv;
 #pragma unroll
-        for (int c = 0; c < 32; ++c) m[c] = fma(-f, L.TR[c], m[c]);
-        dr = me ? piv : fma(-f, trr, dr);
+        for (int c = 0; c < 32; ++c) m[c] = fma(-g, L.TR[c], m[c]);
+        dr = me ? piv : fma(-g, mjr, dr);
         wsync();
     }
     return ok;
@@ -145,11 +146,7 @@ __device__ inline bool sweep32(Lds32<T>& L, double (&m)[32], double& dr, int n, 
 // ---- Householder tridiagonalisation of the rows in h ---------------------
 template <typename T, int C0>
 __device__ inline void tri_step(Lds32<T>& L, cx<T> (&h)[32], int j, int n, int r) {
-    if (r == j) {
-#pragma unroll
-        for (int c = 0; c < 32; ++c) L.R[c] = h[c];
-    }
-    wsync();
+    // row j was published at the end of the previous step (or before the first)
     const cx<T> alpha = cconj(L.R[j + 1]);
     const cx<T> xr = (r > j + 1 && r < n) ? cconj(L.R[r]) : cx<T>{T(0), T(0)};
     const T xn2 = sum32(cabs2(xr));
@@ -193,11 +190,21 @@ __device__ inline void tri_step(Lds32<T>& L, cx<T> (&h)[32], int j, int n, int r
 #pragma unroll
     for (int c = (C0 > 0 ? C0 - 1 : 0); c < (C0 + 8 < 32 ? C0 + 8 : 32); ++c)
         if (c == j && r > j + 1) h[c] = vr;
+    // publish row j + 1 for the next step (everyone is past reading row j)
+    if (r == j + 1) {
+#pragma unroll
+        for (int c = 0; c < 32; ++c) L.R[c] = h[c];
+    }
     wsync();
 }
 
 template <typename T>
 __device__ inline void tridiag32(Lds32<T>& L, cx<T> (&h)[32], int n, int r) {
+    if (r == 0) {
+#pragma unroll
+        for (int c = 0; c < 32; ++c) L.R[c] = h[c];
+    }
+    wsync();
     // columns below j+1 carry zero reflector entries: one loop per octet of
     // columns left to update (separate loops keep the register allocation of
     // each variant to itself)
@@ -206,12 +213,7 @@ __device__ inline void tridiag32(Lds32<T>& L, cx<T> (&h)[32], int n, int r) {
     for (; j < min(15, n - 1); ++j) tri_step<T, 8>(L, h, j, n, r);
     for (; j < min(23, n - 1); ++j) tri_step<T, 16>(L, h, j, n, r);
     for (; j < n - 1; ++j) tri_step<T, 24>(L, h, j, n, r);
-    // last diagonal entry
-    if (r == n - 1) {
-#pragma unroll
-        for (int c = 0; c < 32; ++c) L.R[c] = h[c];
-    }
-    wsync();
+    // last diagonal entry: row n - 1 is the one published last
     if (r == 0) L.D[n - 1] = L.R[n - 1].re;
     wsync();
 }

@@ -123,9 +123,10 @@ def serve(args):
                 time.sleep(at - now)
             ids.append(srv.submit(stacks[(n, side)], params(n, side, Nx, "mixed"), wts=wts))
         lat, linked = [], 0
-        for rid in ids:
+        for i, rid in enumerate(ids):
             assert srv.wait(rid) == 0
             lat.append(srv.times(rid)[1])
+            print(f"request {i} done: N={reqs[i][0]} side={reqs[i][1]} {lat[-1]:.1f} ms", file=sys.stderr, flush=True)
         t_end = time.perf_counter() - t0
         for rid in ids:
             tc = srv.fetch(rid)[1]
@@ -134,12 +135,15 @@ def serve(args):
     # plain FP64, serial, same requests
     t1 = time.perf_counter()
     linked64 = 0
-    for (n, side, Nx, fill, wts) in reqs:
+    for i, (n, side, Nx, fill, wts) in enumerate(reqs):
+        t2 = time.perf_counter()
         r = tile.LinkRunner(params(n, side, Nx, "fp64"), device=dev)
         r.run(stacks[(n, side)], wts)
         torch.cuda.synchronize()
         linked64 += int((r.tcorr != 0).sum())
         del r
+        print(f"fp64 request {i}: N={n} side={side} {1e3 * (time.perf_counter() - t2):.1f} ms", file=sys.stderr,
+              flush=True)
     t64 = time.perf_counter() - t1
     lat = np.array(lat)
     out = {

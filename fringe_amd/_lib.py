@@ -26,7 +26,8 @@ EXPORTS = [
     "fringe_raster_info", "fringe_raster_metadata", "fringe_raster_read",
     "fringe_server_request_bytes", "fringe_server_create", "fringe_server_submit", "fringe_server_poll",
     "fringe_server_wait", "fringe_server_result", "fringe_server_times", "fringe_server_release",
-    "fringe_server_destroy",
+    "fringe_server_destroy", "fringe_adjust_wrapped", "fringe_adjust_unwrapped",
+    "fringe_despeck_options_init", "fringe_despeck_process",
 ]
 FRINGE_REQ_QUEUED, FRINGE_REQ_RUNNING, FRINGE_REQ_DONE = 0, 1, 2
 
@@ -59,6 +60,13 @@ class NmapOptions(ctypes.Structure):
                 ("method", ctypes.c_char_p), ("noGPU", ctypes.c_int), ("blocksize", ctypes.c_int),
                 ("memsize", ctypes.c_int), ("Nx", ctypes.c_int), ("Ny", ctypes.c_int),
                 ("prob", ctypes.c_double), ("verbose", ctypes.c_int)]
+
+
+class DespeckOptions(ctypes.Structure):
+    _fields_ = [("inputDS", ctypes.c_char_p), ("wtsDS", ctypes.c_char_p), ("outputDS", ctypes.c_char_p),
+                ("blocksize", ctypes.c_int), ("memsize", ctypes.c_int), ("band1", ctypes.c_int),
+                ("band2", ctypes.c_int), ("Nx", ctypes.c_int), ("Ny", ctypes.c_int),
+                ("computeCoherence", ctypes.c_int), ("verbose", ctypes.c_int)]
 
 
 class ServerRequest(ctypes.Structure):
@@ -99,6 +107,8 @@ def lib():
     L.fringe_raster_metadata.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p,
                                          ctypes.c_char_p, ctypes.c_int]
     L.fringe_raster_read.argtypes = [ctypes.c_char_p] + [ctypes.c_int] * 6 + [vp]
+    L.fringe_adjust_wrapped.argtypes = [vp, vp, vp, ctypes.c_longlong, vp]
+    L.fringe_adjust_unwrapped.argtypes = [vp, vp, vp, ctypes.c_longlong, vp]
     rq = ctypes.POINTER(ServerRequest)
     ll = ctypes.c_longlong
     L.fringe_server_request_bytes.argtypes = [rq]
@@ -113,7 +123,9 @@ def lib():
     L.fringe_server_destroy.argtypes = [vp]
     for name, st in (("fringe_evd_options_init", EvdOptions), ("fringe_evd_process", EvdOptions),
                      ("fringe_phase_link_process", EvdOptions),
-                     ("fringe_nmap_options_init", NmapOptions), ("fringe_nmap_process", NmapOptions)):
+                     ("fringe_nmap_options_init", NmapOptions), ("fringe_nmap_process", NmapOptions),
+                     ("fringe_despeck_options_init", DespeckOptions),
+                     ("fringe_despeck_process", DespeckOptions)):
         if hasattr(L, name):
             getattr(L, name).argtypes = [ctypes.POINTER(st)]
     _LIB = L

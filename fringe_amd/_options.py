@@ -78,20 +78,25 @@ class _LinkOptions:
         return self.status
 
 
-class _NmapOptions:
-    # attribute name -> field of fringe_nmap_options (nmaplib.pyx:27-135)
-    _map = {"inputDS": "inputDS", "countDS": "ncountDS", "weightsDS": "wtsDS", "maskDS": "maskDS",
-            "method": "method", "minimumProbability": "prob", "blocksize": "blocksize",
-            "memsize": "memsize", "halfWindowX": "Nx", "halfWindowY": "Ny", "noGPU": "noGPU"}
+class _PlainOptions:
+    """Options of a module whose C struct holds strings, ints and doubles;
+    subclasses name the struct, its init and process entry points and the
+    attribute -> field map of the reference binding."""
+    _struct_cls = None
+    _init = None
+    _entry = None
+    _map = {}
+    _bool_attrs = ()
 
     def __init__(self):
-        o = _lib.NmapOptions()
-        _lib.lib().fringe_nmap_options_init(ctypes.byref(o))
+        o = self._struct_cls()
+        getattr(_lib.lib(), self._init)(ctypes.byref(o))
         vals = {}
         for attr, field in self._map.items():
             v = getattr(o, field)
             vals[attr] = v.decode() if isinstance(v, bytes) else v
-        vals["noGPU"] = bool(vals["noGPU"])
+        for attr in self._bool_attrs:
+            vals[attr] = bool(vals[attr])
         object.__setattr__(self, "_vals", vals)
         object.__setattr__(self, "verbose", True)
         object.__setattr__(self, "status", None)
@@ -113,8 +118,8 @@ class _NmapOptions:
             print(f"{attr} = {self._vals[attr]}")
 
     def run(self):
-        o = _lib.NmapOptions()
-        _lib.lib().fringe_nmap_options_init(ctypes.byref(o))
+        o = self._struct_cls()
+        getattr(_lib.lib(), self._init)(ctypes.byref(o))
         keep = []
         for attr, field in self._map.items():
             v = self._vals[attr]
@@ -127,5 +132,27 @@ class _NmapOptions:
             else:
                 setattr(o, field, int(v))
         o.verbose = 1 if self.verbose else 0
-        self.status = _lib.lib().fringe_nmap_process(ctypes.byref(o))
+        self.status = getattr(_lib.lib(), self._entry)(ctypes.byref(o))
         return self.status
+
+
+class _NmapOptions(_PlainOptions):
+    # attribute name -> field of fringe_nmap_options (nmaplib.pyx:27-135)
+    _struct_cls = _lib.NmapOptions
+    _init = "fringe_nmap_options_init"
+    _entry = "fringe_nmap_process"
+    _map = {"inputDS": "inputDS", "countDS": "ncountDS", "weightsDS": "wtsDS", "maskDS": "maskDS",
+            "method": "method", "minimumProbability": "prob", "blocksize": "blocksize",
+            "memsize": "memsize", "halfWindowX": "Nx", "halfWindowY": "Ny", "noGPU": "noGPU"}
+    _bool_attrs = ("noGPU",)
+
+
+class _DespeckOptions(_PlainOptions):
+    # attribute name -> field of fringe_despeck_options (despecklib.pyx:26-125)
+    _struct_cls = _lib.DespeckOptions
+    _init = "fringe_despeck_options_init"
+    _entry = "fringe_despeck_process"
+    _map = {"inputDS": "inputDS", "weightsDS": "wtsDS", "outputDS": "outputDS", "blocksize": "blocksize",
+            "memsize": "memsize", "halfWindowX": "Nx", "halfWindowY": "Ny", "band1": "band1",
+            "band2": "band2", "coherenceFlag": "computeCoherence"}
+    _bool_attrs = ("coherenceFlag",)

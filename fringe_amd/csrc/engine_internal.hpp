@@ -46,6 +46,12 @@ struct LinkPlan {
     uint32_t* d_pack;   // mixed precision: packed (hi|lo) records, [rows*cols+1][2n]
 };
 
+// SHP-masked neighbourhood mean (despeck.cpp:300-433); mode 0 amplitude,
+// 1 interferogram, 2 interferogram with coherence.  c1, c2: rows*cols
+// scratch planes; out rows [row_begin, row_end) written.
+int despeck_launch(const float2* b1, const float2* b2, const uint32_t* wts, int rows, int cols, int Nx, int Ny,
+                   int mode, int row_begin, int row_end, float2* c1, float2* c2, float2* out, hipStream_t s);
+
 // 0, or the code fringe_link_run would return for these options.
 int link_check(const fringe_link_tile* t);
 void set_last_error(const std::string& what);

@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "engine_internal.hpp"
 #include "fringe_amd.h"
 #include "raster.hpp"
 
@@ -270,6 +271,119 @@ int link_module(const fringe_evd_options* o, int module) {
     return status;
 }
 
+// despeck_process (despeck.cpp:18-497): same checks, codes, block schedule
+// and output type; the masked mean of every block runs on the GPU.
+int despeck_module(const fringe_despeck_options* o) {
+    const bool verbose = o->verbose != 0;
+    const int Nx = o->Nx, Ny = o->Ny;
+    const int nulong = ceil_words(Nx, Ny);
+    std::string err;
+    auto in = fringe::io::open_raster(nz(o->inputDS), &err);
+    if (!in) {
+        std::printf("Cannot open stack file { %s } (%s)\nExiting with error code .... (102)\n", nz(o->inputDS),
+                    err.c_str());
+        return 102;
+    }
+    const int cols = in->xsize, rows = in->ysize, nbands = (int)in->bands.size();
+    if (o->band1 <= 0 || o->band1 > nbands) {
+        std::printf("Master band %d outside the range of permissible bands\nExiting with error code .... (102)\n",
+                    o->band1);
+        return 102;
+    }
+    if (o->band2 > 0 && o->band2 > nbands) {
+        std::printf("Slave band %d outside the range of permissible bands\nExiting with error code .... (102)\n",
+                    o->band2);
+        return 102;
+    }
+    auto wts = fringe::io::open_raster(nz(o->wtsDS), &err);
+    if (!wts) {
+        std::printf("Could not open weights dataset {%s}\nExiting with non-zero error code ... 105\n", nz(o->wtsDS));
+        return 105;
+    }
+    {
+        int code = 0;
+        if (wts->xsize != cols) code = 106;
+        if (wts->ysize != rows) code = 107;
+        if ((int)wts->bands.size() != nulong) code = 108;
+        const int inNx = std::atoi(wts->metadata_item("HALFWINDOWX", "ENVI").c_str());
+        const int inNy = std::atoi(wts->metadata_item("HALFWINDOWY", "ENVI").c_str());
+        if (inNx != Nx || inNx == 0) code = 109;
+        if (inNy != Ny || inNy == 0) code = 110;
+        if (code != 0) {
+            std::printf("Weights do not match the input or the window\nExiting with error code ....(%d)\n", code);
+            return code;
+        }
+    }
+    // boxes per block = int(memsize 1e6 / cols) / (blocksize 4 (6 + nulong)) (despeck.cpp:153-162)
+    int blockysize = block_rows(o->memsize, o->blocksize, cols, rows, 4 * (6 + nulong));
+    if (blockysize < rows && blockysize <= 2 * Ny) blockysize = std::min(rows, 2 * Ny + 1);
+    if (verbose) std::printf("Block size = %d lines\n", blockysize);
+    const bool ifg = o->band2 > 0;
+    const int mode = ifg ? (o->computeCoherence ? 2 : 1) : 0;
+    EnviWriter out;
+    if (!out.create(nz(o->outputDS), cols, rows, 1, ifg ? DType::CFloat32 : DType::Float32)) {
+        std::printf("Could not create despecked dataset {%s}\nExiting with non-zero error code ... 104\n",
+                    nz(o->outputDS));
+        return 104;
+    }
+    const size_t px = (size_t)blockysize * cols;
+    DevBuf d_b1, d_b2, d_wts, d_c1, d_c2, d_out;
+    if (!d_b1.alloc(px * 8) || !d_b2.alloc(ifg ? px * 8 : 8) || !d_wts.alloc(px * nulong * 4) ||
+        !d_c1.alloc(px * 8) || !d_c2.alloc(px * 8) || !d_out.alloc(px * 8))
+        return FRINGE_ERR_DEVICE;
+    hipStream_t stream;
+    if (hipStreamCreate(&stream) != hipSuccess) return FRINGE_ERR_DEVICE;
+    std::vector<std::complex<float>> h_b1(px), h_b2(ifg ? px : 1), h_out(px);
+    std::vector<uint32_t> h_wts(px * nulong), band_buf(px);
+    std::vector<float> h_re(px);
+    int yoff = 0, count = 0, status = 0;
+    BlockPlan bp;
+    while (status == 0 && next_block(yoff, count, blockysize, rows, Ny, bp)) {
+        const size_t bpx = (size_t)bp.inysize * cols;
+        if (!fringe::io::read_window(*in, o->band1, 0, bp.yoff, cols, bp.inysize, h_b1.data()) ||
+            (ifg && !fringe::io::read_window(*in, o->band2, 0, bp.yoff, cols, bp.inysize, h_b2.data()))) {
+            std::printf("Error reading data at line %d\nExiting with error code .... (108)\n", bp.yoff);
+            status = 108;
+            break;
+        }
+        for (int w = 0; w < nulong && status == 0; ++w) {
+            if (!fringe::io::read_window(*wts, w + 1, 0, bp.yoff, cols, bp.inysize, band_buf.data())) {
+                status = 108;
+                break;
+            }
+            for (size_t p = 0; p < bpx; ++p) h_wts[p * nulong + w] = band_buf[p];
+        }
+        if (status) break;
+        if (hipMemcpyAsync(d_b1.p, h_b1.data(), bpx * 8, hipMemcpyHostToDevice, stream) != hipSuccess ||
+            (ifg && hipMemcpyAsync(d_b2.p, h_b2.data(), bpx * 8, hipMemcpyHostToDevice, stream) != hipSuccess) ||
+            hipMemcpyAsync(d_wts.p, h_wts.data(), bpx * nulong * 4, hipMemcpyHostToDevice, stream) != hipSuccess ||
+            fringe::despeck_launch((const float2*)d_b1.p, (const float2*)d_b2.p, (const uint32_t*)d_wts.p,
+                                   bp.inysize, cols, Nx, Ny, mode, bp.first, bp.first + bp.lines,
+                                   (float2*)d_c1.p, (float2*)d_c2.p, (float2*)d_out.p, stream) != 0 ||
+            hipMemcpyAsync(h_out.data(), d_out.p, bpx * 8, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+            hipStreamSynchronize(stream) != hipSuccess) {
+            status = FRINGE_ERR_DEVICE;
+            break;
+        }
+        const size_t off = (size_t)bp.first * cols;
+        bool ok;
+        if (ifg) {
+            ok = out.write_rows(bp.yoff + bp.first, bp.lines, h_out.data() + off);
+        } else {   // the Float32 band takes the real part (despeck.cpp:450-455)
+            for (size_t k = 0; k < (size_t)bp.lines * cols; ++k) h_re[k] = h_out[off + k].real();
+            ok = out.write_rows(bp.yoff + bp.first, bp.lines, h_re.data());
+        }
+        if (!ok) {
+            std::printf("Error writing despeck data at line %d\nExiting with error code .... (110)\n", bp.yoff);
+            status = 110;
+            break;
+        }
+        advance(yoff, blockysize, rows, bp);
+    }
+    hipStreamDestroy(stream);
+    return status;
+}
+
 }  // namespace
 
 extern "C" {
@@ -438,6 +552,25 @@ int fringe_nmap_process(const fringe_nmap_options* o) {
     if (verbose) std::printf("\n");
     hipStreamDestroy(stream);
     return status;
+}
+
+void fringe_despeck_options_init(fringe_despeck_options* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->inputDS = "";
+    o->wtsDS = "";
+    o->outputDS = "";
+    o->blocksize = 64;
+    o->memsize = 512;
+    o->band1 = 1;
+    o->band2 = -1;
+    o->Nx = 5;
+    o->Ny = 5;
+    o->computeCoherence = 0;
+    o->verbose = 1;
+}
+
+int fringe_despeck_process(const fringe_despeck_options* o) {
+    return o ? despeck_module(o) : FRINGE_ERR_ARGUMENT;
 }
 
 }  // extern "C"

@@ -164,6 +164,38 @@ double fringe_ks_table(int n, double prob, uint8_t* h_table);
  * still >= prob.  *h_sigma (optional) receives sigma_N for n + n samples. */
 double fringe_ad2_threshold(int n, double prob, double* h_sigma);
 
+/* ---- module level: replaces despeck_process (despeck.cpp:18); fields
+ *      follow despeckOptions (despeck.hpp:17-40).  band2 <= 0: despeckle the
+ *      amplitude of band1 (Float32 output); otherwise the interferogram
+ *      band1 x conj(band2) (CFloat32), or its coherence when
+ *      computeCoherence is set. */
+typedef struct fringe_despeck_options {
+    const char* inputDS;   /* stack VRT */
+    const char* wtsDS;     /* SHP bitmask (required, as in the reference) */
+    const char* outputDS;  /* ENVI output */
+    int blocksize;         /* 64 */
+    int memsize;           /* 512 */
+    int band1;             /* 1 */
+    int band2;             /* -1 */
+    int Nx;                /* 5 */
+    int Ny;                /* 5 */
+    int computeCoherence;  /* 0 */
+    int verbose;
+} fringe_despeck_options;
+
+void fringe_despeck_options_init(fringe_despeck_options* o);
+int fringe_despeck_process(const fringe_despeck_options* o);   /* despeck.cpp:18 */
+
+/* ---- datum adjustment of sequential mini-stacks -------------------------
+ * Replaces the VRT pixel functions of adjustMiniStacks.py:157-228: wrapped
+ * phases are multiplied by the mini-stack's datum phase ("mul", CFloat32),
+ * unwrapped phases added ("sum", Float32), both evaluated in double and
+ * rounded to the band type as GDAL does.  npx complex64 / float32 values,
+ * device pointers, asynchronous on the stream. */
+int fringe_adjust_wrapped(const void* d_mini, const void* d_datum, void* d_out, long long npx, void* stream);
+int fringe_adjust_unwrapped(const float* d_mini, const float* d_datum, float* d_out, long long npx,
+                            void* stream);
+
 /* ---- tile server: submit / poll / complete ------------------------------
  * No counterpart in the reference, which links one file-backed stack per
  * process call (evd.cpp:16).  A server keeps the tiles of many requests

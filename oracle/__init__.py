@@ -16,6 +16,8 @@ Parity status of each oracle function (what pins it, see tests/):
   nmap                          -> brute-force numpy window sweep + _ref KS
   eigen_index, pd_inverse       -> reference EigenLapack.hpp via _ref, the
                                    3x3 known-answer case of tests/eigen/test_eig.cpp
+  despeck                       -> closed forms (constant fields, single-bit
+                                   masks, coherence of a band with itself)
   link (Gram/coherence)         -> numpy FP64 brute force on tiny windows
   link (EVD/MLE/EMI)            -> closed forms (Perron-Frobenius, EMI
                                    lambda=1 / v=exp(i theta)), numpy eigh
@@ -62,6 +64,8 @@ def lib():
         L.oracle_nmap.argtypes = [_f32p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_double, ctypes.c_int, _u32p, _i32p, ctypes.c_int]
+        L.oracle_despeck.argtypes = [_f32p, ctypes.c_void_p, _u32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, _f32p]
         L.oracle_ad2_statistic.argtypes = [_f32p, _f32p, ctypes.c_int]
         L.oracle_ad2_statistic.restype = ctypes.c_double
         L.oracle_ad2_pvalue.argtypes = [_f32p, _f32p, ctypes.c_int]
@@ -157,6 +161,21 @@ def nmap(slc, Nx, Ny, prob=0.05, mask=None, amp_const=None, threads=0, method="K
                       None if ac is None else ac.ctypes.data, Nx, Ny, float(prob),
                       {"KS2": 0, "AD2": 1}[method], wts.reshape(-1), cnt.reshape(-1), int(threads))
     return wts, cnt
+
+
+def despeck(b1, wts, Nx, Ny, b2=None, coherence=False):
+    """SHP-masked mean (despeck.cpp:300-433): amplitude of b1, or the
+    interferogram b1 conj(b2), or its coherence.  Returns complex64
+    [rows, cols] (the amplitude is the real part)."""
+    rows, cols = b1.shape
+    mode = 0 if b2 is None else (2 if coherence else 1)
+    out = np.zeros((rows, cols), np.complex64)
+    a = np.ascontiguousarray(b1, np.complex64)
+    b = None if b2 is None else np.ascontiguousarray(b2, np.complex64)
+    lib().oracle_despeck(a.view(np.float32).reshape(-1), None if b is None else b.ctypes.data,
+                         np.ascontiguousarray(wts, np.uint32).reshape(-1), rows, cols, Nx, Ny, mode,
+                         out.view(np.float32).reshape(-1))
+    return out
 
 
 def link(slc, wts, Nx, Ny, module="phase_link", method="MLE", min_neighbors=2,

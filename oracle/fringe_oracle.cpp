@@ -360,6 +360,61 @@ extern "C" int oracle_nmap(const float* slc, const uint8_t* mask, int rows, int 
 }
 
 // --------------------------------------------------------------------------
+// despeck: SHP-masked neighbourhood mean (despeck.cpp:300-433) over a whole
+// image (the reference's blocks only bound memory: windows are truncated at
+// the image edges, as here).  mode 0: |b1|; 1: b1 conj(b2); 2: b1 conj(b2)
+// normalised to coherence.  out: [rows][cols] complex64 (zeros where the
+// centre bit is clear); the amplitude output is its real part.
+// --------------------------------------------------------------------------
+extern "C" int oracle_despeck(const float* b1, const float* b2, const uint32_t* wts, int rows, int cols,
+                              int Nx, int Ny, int mode, float* out) {
+    const int nulong = oracle_mask_words(Nx, Ny);
+    const size_t npx = (size_t)rows * cols;
+    std::vector<zf> c1(npx), c2(npx);
+    for (size_t p = 0; p < npx; ++p) {
+        const zf a(b1[2 * p], b1[2 * p + 1]);
+        if (mode == 0) {
+            c1[p] = zf(std::abs(a), 0.0f);
+            c2[p] = zf(1.0f, 0.0f);
+            continue;
+        }
+        const zf b(b2[2 * p], b2[2 * p + 1]);
+        const float m1 = std::abs(a), m2 = std::abs(b);
+        c1[p] = a * std::conj(b);
+        c2[p] = (mode == 2) ? zf(m1 * m1, m2 * m2) : zf(1.0f, 0.0f);
+    }
+    for (int r = 0; r < rows; ++r)
+        for (int c = 0; c < cols; ++c) {
+            const size_t p = (size_t)r * cols + c;
+            const uint32_t* w = wts + p * nulong;
+            zf res(0.0f, 0.0f);
+            auto bit = [&](int ii, int jj) {
+                const int f = (ii + Ny) * (2 * Nx + 1) + jj + Nx;
+                return (w[f >> 5] >> (f & 31)) & 1u;
+            };
+            if (bit(0, 0)) {
+                zf val(0.0f, 0.0f), sumw(0.0f, 0.0f);
+                for (int ii = std::max(r - Ny, 0); ii <= std::min(rows - 1, r + Ny); ++ii)
+                    for (int jj = std::max(c - Nx, 0); jj <= std::min(cols - 1, c + Nx); ++jj)
+                        if (bit(ii - r, jj - c)) {
+                            val += c1[(size_t)ii * cols + jj];
+                            sumw += c2[(size_t)ii * cols + jj];
+                        }
+                if (sumw.real() > 0) {
+                    if (mode == 2) {
+                        if (sumw.imag() > 0) res = val / (std::sqrt(sumw.real()) * std::sqrt(sumw.imag()));
+                    } else {
+                        res = val / sumw.real();
+                    }
+                }
+            }
+            out[2 * p] = res.real();
+            out[2 * p + 1] = res.imag();
+        }
+    return 0;
+}
+
+// --------------------------------------------------------------------------
 // Phase linking.  module: 0 = evd (evd.cpp), 1 = phase_link (phase_link.cpp).
 // method: 0 = MLE, 1 = EVD, 2 = STBAS (phase_link always runs EMI; STBAS
 // there only limits the temporal-coherence sum, phase_link.cpp:654).

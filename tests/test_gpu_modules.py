@@ -150,3 +150,97 @@ def test_sequential_against_oracle_replay(tmp_path):
     ph = np.stack([raster.read(os.path.join(dc, dates[min(n, s0 + L) - 1] + ".slc")) for s0 in range(0, n, L)])
     ok = ref["tcorr"] > 0
     assert wrapped_err(ph, ref["phase"])[:, ok].max() <= 5 * PHASE_TOL
+
+
+def test_datum_adjustment_wrapped_and_unwrapped(tmp_path):
+    # adjustMiniStacks.py after a sequential run: wrapped phases multiplied by
+    # the mini-stack's datum phase as GDAL's "mul" pixel function does it (in
+    # double, rounded to CFloat32); unwrapped phases added, with the
+    # time-series VRT's ScaleOffset = -phase at the reference pixel
+    from fringe_amd.cli import adjust_ministacks
+    from fringe_amd.sequential import run_sequential
+    import datetime
+    import re
+    n, rows, cols, Nx, Ny, L = 8, 12, 14, 2, 2, 4
+    slc = synthetic.stack(n, rows, cols, seed=52)
+    t0 = datetime.date(2022, 1, 1)
+    dates = []
+    os.makedirs(tmp_path / "vrts")
+    for t in range(n):
+        d = (t0 + datetime.timedelta(12 * t)).strftime("%Y%m%d")
+        os.makedirs(tmp_path / "SLC" / d)
+        f = str(tmp_path / "SLC" / d / (d + ".slc"))
+        raster.write_envi(f, slc[t])
+        open(str(tmp_path / "vrts" / (d + ".vrt")), "w").write(raster.raw_vrt(f, cols, rows))
+        dates.append(d)
+    wts = synthetic.full_mask(rows, cols, Nx, Ny)
+    wf = str(tmp_path / "wts")
+    raster.write_envi(wf, wts, meta={"HALFWINDOWX": Nx, "HALFWINDOWY": Ny})
+    print("case: datum adjustment", flush=True)
+    res = run_sequential(str(tmp_path / "SLC"), wf, str(tmp_path / "seq"), miniStackSize=L,
+                         halfWindowX=Nx, halfWindowY=Ny, minNeighbors=5, verbose=False)
+    assert all(rc == 0 for _, rc in res)
+    ms, dc = str(tmp_path / "seq" / "miniStacks"), str(tmp_path / "seq" / "Datum_connection")
+    out = str(tmp_path / "adjusted")
+    assert adjust_ministacks.main(["-s", str(tmp_path / "vrts"), "-m", ms, "-d", dc, "-M", str(L), "-o", out]) == 0
+    for i, d in enumerate(dates):
+        s0 = (i // L) * L
+        last = dates[min(n, s0 + L) - 1]
+        a = raster.read(os.path.join(ms, f"{dates[s0]}_{last}", "EVD", d + ".slc"))
+        b = raster.read(os.path.join(dc, "EVD", last + ".slc"))
+        want = (a.astype(np.complex128) * b.astype(np.complex128)).astype(np.complex64)
+        got = raster.read(os.path.join(out, d + ".slc"))
+        assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+        assert np.array_equal(raster.read(os.path.join(out, d + ".slc.vrt")), got)
+    # unwrapped: synthetic unwrapped phases beside the EVD folders
+    rng = np.random.default_rng(3)
+    for s0 in range(0, n, L):
+        last = dates[min(n, s0 + L) - 1]
+        os.makedirs(os.path.join(ms, f"{dates[s0]}_{last}", "unwrap"))
+        for d in dates[s0:s0 + L]:
+            raster.write_envi(os.path.join(ms, f"{dates[s0]}_{last}", "unwrap", d + ".unw"),
+                              rng.normal(size=(rows, cols)).astype(np.float32) * 10)
+    os.makedirs(os.path.join(dc, "unwrap"))
+    for s0 in range(0, n, L):
+        last = dates[min(n, s0 + L) - 1]
+        raster.write_envi(os.path.join(dc, "unwrap", last + ".unw"), rng.normal(size=(rows, cols)).astype(np.float32))
+    out2 = str(tmp_path / "adjusted_unw")
+    ts = str(tmp_path / "ts.vrt")
+    assert adjust_ministacks.main(["-s", str(tmp_path / "vrts"), "-m", ms, "-d", dc, "-M", str(L), "-o", out2,
+                                   "--unwrapped", "-t", ts, "-x", "3", "-y", "5"]) == 0
+    offsets = [float(v) for v in re.findall(r"<ScaleOffset>([^<]+)</ScaleOffset>", open(ts).read())]
+    assert len(offsets) == n
+    for i, d in enumerate(dates):
+        s0 = (i // L) * L
+        last = dates[min(n, s0 + L) - 1]
+        a = raster.read(os.path.join(ms, f"{dates[s0]}_{last}", "unwrap", d + ".unw"), dtype=np.float32)
+        b = raster.read(os.path.join(dc, "unwrap", last + ".unw"), dtype=np.float32)
+        got = raster.read(os.path.join(out2, d + ".phase"), dtype=np.float32)
+        assert np.array_equal(got, (a.astype(np.float64) + b.astype(np.float64)).astype(np.float32))
+        assert offsets[i] == pytest.approx(-float(got[5, 3]), rel=1e-6)
+
+
+@pytest.mark.parametrize("bands,coh", [([1], False), ([1, 2], False), ([1, 2], True)])
+def test_despeck_module_matches_oracle(tmp_path, bands, coh):
+    # despeck.py through the GPU module, several row blocks (memsize 1 MB),
+    # against the oracle's whole-image sums: bit-exact
+    from fringe_amd.cli import despeck as despeck_cli
+    n, rows, cols, Nx, Ny = 3, 70, 40, 3, 2
+    print(f"case despeck bands={bands} coherence={coh}", flush=True)
+    slc = synthetic.stack(n, rows, cols, seed=61, patches=5)
+    sv, _ = write_stack(tmp_path, slc)
+    wts, _ = oracle.nmap(slc, Nx, Ny, 0.05)
+    wf = str(tmp_path / "wts")
+    raster.write_envi(wf, wts, meta={"HALFWINDOWX": Nx, "HALFWINDOWY": Ny})
+    out = str(tmp_path / "despeck" / "out.bin")
+    argv = ["-i", sv, "-o", out, "-w", wf, "-x", str(Nx), "-y", str(Ny), "-r", "1", "-l", "8",
+            "-b"] + [str(b) for b in bands] + (["-c"] if coh else [])
+    assert despeck_cli.main(argv) == 0
+    want = oracle.despeck(slc[bands[0] - 1], wts, Nx, Ny, b2=slc[bands[1] - 1] if len(bands) == 2 else None,
+                          coherence=coh)
+    if len(bands) == 1:
+        got = raster.read(out, dtype=np.float32)
+        assert np.array_equal(got, want.real)
+    else:
+        got = raster.read(out)
+        assert np.array_equal(got.view(np.uint64), want.view(np.uint64))

@@ -189,3 +189,36 @@ def test_ad2_threshold_decides_like_the_oracle_pvalue(prob):
                 a, b = np.round(a * 3) / 3, np.round(b * 3) / 3
             a, b = np.sort(a.astype(np.float32)), np.sort(b.astype(np.float32))
             assert (oracle.ad2_statistic(a, b) <= thr) == (oracle.ad2_pvalue(a, b) >= prob)
+
+
+def test_despeck_module_errors_before_gpu_work(tmp_path):
+    from fringe_amd.despecklib import Despeck
+    d = Despeck()
+    d.verbose = False
+    d.inputDS = str(tmp_path / "missing.vrt")
+    assert d.run() == 102
+    rows, cols, n = 6, 7, 3
+    slc = synthetic.stack(n, rows, cols, seed=2)
+    srcs = []
+    for t in range(n):
+        fn = str(tmp_path / f"s{t}.slc")
+        raster.write_envi(fn, slc[t])
+        srcs.append((fn, f"2022010{t + 1}", False))
+    sv = str(tmp_path / "stack.vrt")
+    open(sv, "w").write(raster.stack_vrt(srcs, cols, rows))
+    d.inputDS = sv
+    d.band1 = 4
+    assert d.run() == 102                    # band outside the stack
+    d.band1, d.band2 = 1, 5
+    assert d.run() == 102
+    d.band2 = 2
+    d.weightsDS = ""                         # the reference needs the mask
+    assert d.run() == 105
+    wts = synthetic.full_mask(rows, cols, 2, 2)
+    wf = str(tmp_path / "wts")
+    raster.write_envi(wf, wts, meta={"HALFWINDOWX": 2, "HALFWINDOWY": 2})
+    d.weightsDS = wf
+    d.halfWindowX, d.halfWindowY = 3, 2
+    assert d.run() == 109                    # last failing check wins, as in despeck.cpp:90-140
+    d.halfWindowX, d.halfWindowY = 2, 1
+    assert d.run() == 110

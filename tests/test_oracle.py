@@ -377,3 +377,41 @@ def test_reference_date_and_per_date_scaling():
     ok = a["tcorr"] > 0
     assert np.abs(np.angle(a["phase"] * b["phase"].conj()))[:, ok].max() < 1e-5
     assert np.abs(a["tcorr"] - b["tcorr"])[ok].max() < 1e-6
+
+
+# ------------------------------------------------------------- despeck --
+def test_despeck_oracle_closed_forms():
+    rows, cols, Nx, Ny = 9, 11, 2, 1
+    full = synthetic.full_mask(rows, cols, Nx, Ny)
+    # a constant amplitude field stays constant (exact: 2.0 sums exactly)
+    z = np.full((rows, cols), 2.0 * np.exp(0.3j), np.complex64)
+    amp = oracle.despeck(z, full, Nx, Ny)
+    assert np.allclose(amp.real, np.abs(z).astype(np.float32), rtol=0, atol=2e-7) and not amp.imag.any()
+    # with only the centre bit, every output is the pixel's own value
+    centre = np.zeros_like(full)
+    f = Ny * (2 * Nx + 1) + Nx
+    centre[..., f >> 5] = np.uint32(1 << (f & 31))
+    s = synthetic.stack(2, rows, cols, seed=7)
+    a, b = s[0].astype(np.complex64), s[1].astype(np.complex64)
+    assert np.array_equal(oracle.despeck(a, centre, Nx, Ny).real, _amp(a))
+    assert np.allclose(oracle.despeck(a, centre, Nx, Ny, b2=b), a * np.conj(b), rtol=1e-6)
+    coh = oracle.despeck(a, centre, Nx, Ny, b2=b, coherence=True)
+    assert np.allclose(np.abs(coh), 1.0, atol=1e-6)
+    # the coherence of a band with itself is one
+    assert np.allclose(oracle.despeck(a, full, Nx, Ny, b2=a, coherence=True), 1.0, atol=1e-6)
+    # a sparse mask averages exactly the selected, in-image neighbours
+    rng = np.random.default_rng(1)
+    sparse = full & rng.integers(0, 2 ** 32, full.shape, dtype=np.uint64).astype(np.uint32)
+    sparse[..., f >> 5] |= np.uint32(1 << (f & 31))
+    got = oracle.despeck(a, b2=b, wts=sparse, Nx=Nx, Ny=Ny)
+    ifg = (a.astype(np.complex128) * np.conj(b.astype(np.complex128)))
+    for r in range(rows):
+        for c in range(cols):
+            acc, k = 0, 0
+            for ii in range(-Ny, Ny + 1):
+                for jj in range(-Nx, Nx + 1):
+                    ff = (ii + Ny) * (2 * Nx + 1) + jj + Nx
+                    if 0 <= r + ii < rows and 0 <= c + jj < cols and (sparse[r, c, ff >> 5] >> (ff & 31)) & 1:
+                        acc += ifg[r + ii, c + jj]
+                        k += 1
+            assert abs(got[r, c] - acc / k) <= 1e-5 * max(1.0, abs(acc / k))

@@ -28,6 +28,7 @@ EXPORTS = [
     "fringe_server_wait", "fringe_server_result", "fringe_server_times", "fringe_server_release",
     "fringe_server_destroy", "fringe_adjust_wrapped", "fringe_adjust_unwrapped",
     "fringe_despeck_options_init", "fringe_despeck_process",
+    "fringe_calamp_options_init", "fringe_calamp_process",
 ]
 FRINGE_REQ_QUEUED, FRINGE_REQ_RUNNING, FRINGE_REQ_DONE = 0, 1, 2
 
@@ -67,6 +68,12 @@ class DespeckOptions(ctypes.Structure):
                 ("blocksize", ctypes.c_int), ("memsize", ctypes.c_int), ("band1", ctypes.c_int),
                 ("band2", ctypes.c_int), ("Nx", ctypes.c_int), ("Ny", ctypes.c_int),
                 ("computeCoherence", ctypes.c_int), ("verbose", ctypes.c_int)]
+
+
+class CalampOptions(ctypes.Structure):
+    _fields_ = [("inputDS", ctypes.c_char_p), ("maskDS", ctypes.c_char_p), ("outputDS", ctypes.c_char_p),
+                ("defaultValue", ctypes.c_double), ("applySqrt", ctypes.c_int), ("blocksize", ctypes.c_int),
+                ("memsize", ctypes.c_int), ("verbose", ctypes.c_int)]
 
 
 class ServerRequest(ctypes.Structure):
@@ -125,7 +132,8 @@ def lib():
                      ("fringe_phase_link_process", EvdOptions),
                      ("fringe_nmap_options_init", NmapOptions), ("fringe_nmap_process", NmapOptions),
                      ("fringe_despeck_options_init", DespeckOptions),
-                     ("fringe_despeck_process", DespeckOptions)):
+                     ("fringe_despeck_process", DespeckOptions),
+                     ("fringe_calamp_options_init", CalampOptions), ("fringe_calamp_process", CalampOptions)):
         if hasattr(L, name):
             getattr(L, name).argtypes = [ctypes.POINTER(st)]
     _LIB = L

@@ -156,3 +156,13 @@ class _DespeckOptions(_PlainOptions):
             "memsize": "memsize", "halfWindowX": "Nx", "halfWindowY": "Ny", "band1": "band1",
             "band2": "band2", "coherenceFlag": "computeCoherence"}
     _bool_attrs = ("coherenceFlag",)
+
+
+class _CalampOptions(_PlainOptions):
+    # attribute name -> field of fringe_calamp_options (calamplib.pyx:21-100)
+    _struct_cls = _lib.CalampOptions
+    _init = "fringe_calamp_options_init"
+    _entry = "fringe_calamp_process"
+    _map = {"inputDS": "inputDS", "outputDS": "outputDS", "maskDS": "maskDS", "defaultValue": "defaultValue",
+            "applySqrt": "applySqrt", "blocksize": "blocksize", "memsize": "memsize"}
+    _bool_attrs = ("applySqrt",)

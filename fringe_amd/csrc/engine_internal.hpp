@@ -52,6 +52,11 @@ struct LinkPlan {
 int despeck_launch(const float2* b1, const float2* b2, const uint32_t* wts, int rows, int cols, int Nx, int Ny,
                    int mode, int row_begin, int row_end, float2* c1, float2* c2, float2* out, hipStream_t s);
 
+// Per-workgroup (sum of |z|, count of valid pixels) of one band for calamp
+// (calamp.cpp:180-262); partial has calamp_partials() entries.
+int calamp_partials();
+int calamp_launch(const float2* z, const uint8_t* mask, long long n, double2* partial, hipStream_t s);
+
 // 0, or the code fringe_link_run would return for these options.
 int link_check(const fringe_link_tile* t);
 void set_last_error(const std::string& what);

@@ -13,8 +13,14 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <sstream>
 #include <string>
 #include <vector>
+
+#include <stdlib.h>
+#include <strings.h>
 
 #include "engine_internal.hpp"
 #include "fringe_amd.h"
@@ -384,6 +390,123 @@ int despeck_module(const fringe_despeck_options* o) {
     return status;
 }
 
+// calamp_process (calamp.cpp:10-262): same checks and codes; the per-band
+// sums run on the GPU; the output is a VRT of the input whose bands carry the
+// constants, formatted as the reference's std::ostream does (6 significant
+// digits).
+int calamp_module(const fringe_calamp_options* o) {
+    const bool verbose = o->verbose != 0;
+    std::string err;
+    auto in = fringe::io::open_raster(nz(o->inputDS), &err);
+    if (!in) {
+        std::printf("Cannot open stack file { %s } (%s)\nExiting with error code .... (102)\n", nz(o->inputDS),
+                    err.c_str());
+        return 102;
+    }
+    const int cols = in->xsize, rows = in->ysize, nbands = (int)in->bands.size();
+    if (verbose) std::printf("Number of rows  = %d\nNumber of cols  = %d\nNumber of bands = %d\n", rows, cols, nbands);
+    std::shared_ptr<Raster> msk;
+    const std::string mname = nz(o->maskDS);
+    if (!mname.empty() && strcasecmp(mname.c_str(), "None") != 0) {
+        msk = fringe::io::open_raster(mname, &err);
+        if (!msk) {
+            std::printf("Cannot open mask file { %s }\nExiting with error code .... (103)\n", mname.c_str());
+            return 103;
+        }
+        int code = 0;
+        if (msk->xsize != cols) code = 104;
+        if (msk->ysize != rows) code = 105;
+        if ((int)msk->bands.size() != 1) code = 106;
+        if (code) {
+            std::printf("Mask does not match the stack\nExiting with error code .... (%d)\n", code);
+            return code;
+        }
+    }
+    // blockysize = blocksize * (int(memsize 1e6 / cols) / (blocksize * 8)) (calamp.cpp:112-120)
+    int blockysize = o->blocksize * (int((o->memsize * 1.0e6) / cols) / (o->blocksize * 8));
+    if (blockysize < o->blocksize) blockysize = o->blocksize;
+    if (blockysize > rows) blockysize = rows;
+    if (verbose) std::printf("Block size = %d lines\n", blockysize);
+    const size_t px = (size_t)blockysize * cols;
+    const int np = fringe::calamp_partials();
+    DevBuf d_z, d_m, d_part;
+    if (!d_z.alloc(px * 8) || (msk && !d_m.alloc(px)) || !d_part.alloc((size_t)np * sizeof(double2)))
+        return FRINGE_ERR_DEVICE;
+    hipStream_t stream;
+    if (hipStreamCreate(&stream) != hipSuccess) return FRINGE_ERR_DEVICE;
+    std::vector<std::complex<float>> h_z(px);
+    std::vector<uint8_t> h_m(msk ? px : 1);
+    std::vector<double2> h_part(np);
+    std::vector<double> total(nbands, 0.0), norms(nbands, 0.0);
+    int status = 0;
+    for (int yoff = 0; yoff < rows && status == 0; yoff += blockysize) {
+        const int h = std::min(blockysize, rows - yoff);
+        const size_t n = (size_t)h * cols;
+        if (msk) {
+            if (!fringe::io::read_window(*msk, 1, 0, yoff, cols, h, h_m.data())) {
+                std::printf("Error reading mask at line %d\nExiting with error code .... (107)\n", yoff);
+                status = 107;
+                break;
+            }
+            if (hipMemcpyAsync(d_m.p, h_m.data(), n, hipMemcpyHostToDevice, stream) != hipSuccess) {
+                status = FRINGE_ERR_DEVICE;
+                break;
+            }
+        }
+        for (int b = 0; b < nbands; ++b) {
+            if (!fringe::io::read_window(*in, b + 1, 0, yoff, cols, h, h_z.data())) {
+                std::printf("Error reading data from band %d at line %d\nExiting with error code .... (108)\n",
+                            b + 1, yoff);
+                status = 108;
+                break;
+            }
+            if (hipMemcpyAsync(d_z.p, h_z.data(), n * 8, hipMemcpyHostToDevice, stream) != hipSuccess ||
+                fringe::calamp_launch((const float2*)d_z.p, msk ? (const uint8_t*)d_m.p : nullptr, (long long)n,
+                                      (double2*)d_part.p, stream) != 0 ||
+                hipMemcpyAsync(h_part.data(), d_part.p, (size_t)np * sizeof(double2), hipMemcpyDeviceToHost,
+                               stream) != hipSuccess ||
+                hipStreamSynchronize(stream) != hipSuccess) {
+                status = FRINGE_ERR_DEVICE;
+                break;
+            }
+            double bs = 0.0, bn = 0.0;
+            for (const double2& q : h_part) {
+                bs += q.x;
+                bn += q.y;
+            }
+            total[b] += bs;
+            norms[b] += bn;
+        }
+    }
+    hipStreamDestroy(stream);
+    if (status) return status;
+    std::vector<std::map<std::string, std::map<std::string, std::string>>> meta(nbands);
+    for (int b = 0; b < nbands; ++b) {
+        if (norms[b] == 0) {
+            if (verbose) std::printf("No valid data found in Band %d. Set to 1.0\n", b + 1);
+            total[b] = 1.0;
+        } else {
+            total[b] /= norms[b];
+            if (verbose) std::printf("Band %d:%g\n", b + 1, total[b]);
+        }
+        meta[b] = in->bands[b].metadata;
+        std::ostringstream strs;
+        strs << total[b];
+        meta[b]["slc"]["amplitudeConstant"] = strs.str();
+    }
+    char absbuf[4096];
+    const std::string src = realpath(nz(o->inputDS), absbuf) ? std::string(absbuf) : std::string(nz(o->inputDS));
+    FILE* f = std::fopen(nz(o->outputDS), "w");
+    if (!f) {
+        std::printf("Could not create output VRT %s\n", nz(o->outputDS));
+        return 104;
+    }
+    const std::string text = fringe::io::vrt_band_copy(*in, src, meta);
+    const bool ok = std::fwrite(text.data(), 1, text.size(), f) == text.size();
+    std::fclose(f);
+    return ok ? 0 : 104;
+}
+
 }  // namespace
 
 extern "C" {
@@ -571,6 +694,22 @@ void fringe_despeck_options_init(fringe_despeck_options* o) {
 
 int fringe_despeck_process(const fringe_despeck_options* o) {
     return o ? despeck_module(o) : FRINGE_ERR_ARGUMENT;
+}
+
+void fringe_calamp_options_init(fringe_calamp_options* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->inputDS = "";
+    o->maskDS = "";
+    o->outputDS = "";
+    o->defaultValue = 1.0;
+    o->applySqrt = 0;
+    o->blocksize = 128;
+    o->memsize = 256;
+    o->verbose = 1;
+}
+
+int fringe_calamp_process(const fringe_calamp_options* o) {
+    return o ? calamp_module(o) : FRINGE_ERR_ARGUMENT;
 }
 
 }  // extern "C"

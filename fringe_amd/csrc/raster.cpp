@@ -678,5 +678,40 @@ std::string vrt_for_envi(const std::string& file, int xsize, int ysize, int nban
     return o.str();
 }
 
+static std::string xml_escape(const std::string& v) {
+    std::string o;
+    for (char c : v) {
+        if (c == '&') o += "&amp;";
+        else if (c == '<') o += "&lt;";
+        else if (c == '>') o += "&gt;";
+        else if (c == '"') o += "&quot;";
+        else o += c;
+    }
+    return o;
+}
+
+std::string vrt_band_copy(const Raster& in, const std::string& src_path,
+                          const std::vector<std::map<std::string, std::map<std::string, std::string>>>& meta) {
+    std::ostringstream o;
+    o << "<VRTDataset rasterXSize=\"" << in.xsize << "\" rasterYSize=\"" << in.ysize << "\">\n";
+    for (size_t b = 0; b < in.bands.size(); ++b) {
+        o << "  <VRTRasterBand dataType=\"" << gdal_name(in.bands[b].dtype) << "\" band=\"" << b + 1 << "\">\n";
+        if (b < meta.size())
+            for (const auto& dom : meta[b]) {
+                o << "    <Metadata";
+                if (!dom.first.empty()) o << " domain=\"" << xml_escape(dom.first) << "\"";
+                o << ">\n";
+                for (const auto& kv : dom.second)
+                    o << "      <MDI key=\"" << xml_escape(kv.first) << "\">" << xml_escape(kv.second) << "</MDI>\n";
+                o << "    </Metadata>\n";
+            }
+        o << "    <SimpleSource>\n      <SourceFilename relativeToVRT=\"0\">" << xml_escape(src_path)
+          << "</SourceFilename>\n      <SourceBand>" << b + 1 << "</SourceBand>\n    </SimpleSource>\n";
+        o << "  </VRTRasterBand>\n";
+    }
+    o << "</VRTDataset>\n";
+    return o.str();
+}
+
 }  // namespace io
 }  // namespace fringe

@@ -82,6 +82,12 @@ struct EnviWriter {
 // of that file yields for our purposes).
 std::string vrt_for_envi(const std::string& file, int xsize, int ysize, int nbands, DType t);
 
+// VRT whose band b reads band b of src_path whole and carries meta[b]
+// (domain -> key -> value): what GDAL's CreateCopy of the dataset followed by
+// SetMetadataItem calls reads back as.
+std::string vrt_band_copy(const Raster& in, const std::string& src_path,
+                          const std::vector<std::map<std::string, std::map<std::string, std::string>>>& meta);
+
 bool path_exists(const std::string& p, bool* is_dir);
 bool make_dir(const std::string& p);
 std::string join_path(const std::string& a, const std::string& b);

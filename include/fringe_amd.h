@@ -186,6 +186,24 @@ typedef struct fringe_despeck_options {
 void fringe_despeck_options_init(fringe_despeck_options* o);
 int fringe_despeck_process(const fringe_despeck_options* o);   /* despeck.cpp:18 */
 
+/* ---- module level: replaces calamp_process (calamp.cpp:10); fields follow
+ *      calampOptions (calamp.hpp:24-40).  Writes outputDS, a VRT of the
+ *      input stack whose bands carry amplitudeConstant (domain "slc"), the
+ *      mean |z| of the band's valid pixels (1.0 when it has none), for nmap. */
+typedef struct fringe_calamp_options {
+    const char* inputDS;
+    const char* maskDS;    /* "" or "None": no mask */
+    const char* outputDS;
+    double defaultValue;   /* 1.0 (accepted; the reference does not use it) */
+    int applySqrt;         /* 0   (accepted; the reference does not use it) */
+    int blocksize;         /* 128 */
+    int memsize;           /* 256 */
+    int verbose;
+} fringe_calamp_options;
+
+void fringe_calamp_options_init(fringe_calamp_options* o);
+int fringe_calamp_process(const fringe_calamp_options* o);     /* calamp.cpp:10 */
+
 /* ---- datum adjustment of sequential mini-stacks -------------------------
  * Replaces the VRT pixel functions of adjustMiniStacks.py:157-228: wrapped
  * phases are multiplied by the mini-stack's datum phase ("mul", CFloat32),

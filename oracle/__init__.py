@@ -16,6 +16,8 @@ Parity status of each oracle function (what pins it, see tests/):
   nmap                          -> brute-force numpy window sweep + _ref KS
   eigen_index, pd_inverse       -> reference EigenLapack.hpp via _ref, the
                                    3x3 known-answer case of tests/eigen/test_eig.cpp
+  calamp                        -> closed forms (constant amplitude, empty mask,
+                                   NaN and zero pixels excluded)
   despeck                       -> closed forms (constant fields, single-bit
                                    masks, coherence of a band with itself)
   link (Gram/coherence)         -> numpy FP64 brute force on tiny windows
@@ -64,6 +66,7 @@ def lib():
         L.oracle_nmap.argtypes = [_f32p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_double, ctypes.c_int, _u32p, _i32p, ctypes.c_int]
+        L.oracle_calamp.argtypes = [_f32p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p]
         L.oracle_despeck.argtypes = [_f32p, ctypes.c_void_p, _u32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_int, _f32p]
         L.oracle_ad2_statistic.argtypes = [_f32p, _f32p, ctypes.c_int]
@@ -161,6 +164,15 @@ def nmap(slc, Nx, Ny, prob=0.05, mask=None, amp_const=None, threads=0, method="K
                       None if ac is None else ac.ctypes.data, Nx, Ny, float(prob),
                       {"KS2": 0, "AD2": 1}[method], wts.reshape(-1), cnt.reshape(-1), int(threads))
     return wts, cnt
+
+
+def calamp(slc, mask=None):
+    """Per-band mean |z| over valid pixels (calamp.cpp:180-262)."""
+    nb, rows, cols = slc.shape
+    out = np.zeros(nb, np.float64)
+    m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+    lib().oracle_calamp(_slc_f32(slc), None if m is None else m.ctypes.data, rows, cols, nb, out)
+    return out
 
 
 def despeck(b1, wts, Nx, Ny, b2=None, coherence=False):

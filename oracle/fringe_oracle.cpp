@@ -360,6 +360,29 @@ extern "C" int oracle_nmap(const float* slc, const uint8_t* mask, int rows, int 
 }
 
 // --------------------------------------------------------------------------
+// calamp (calamp.cpp:180-262): per band, the mean of |z| (rounded to float as
+// std::abs of complex<float>, summed in double) over pixels with |z| != 0,
+// |z| not NaN and mask > 0; 1.0 for a band without valid pixels.
+// --------------------------------------------------------------------------
+extern "C" int oracle_calamp(const float* slc, const uint8_t* mask, int rows, int cols, int nbands,
+                             double* out) {
+    const size_t npx = (size_t)rows * cols;
+    for (int b = 0; b < nbands; ++b) {
+        double sum = 0.0, norm = 0.0;
+        for (size_t p = 0; p < npx; ++p) {
+            const float* z = slc + 2 * ((size_t)b * npx + p);
+            double a = std::abs(zf(z[0], z[1]));
+            a = std::isnan(a) ? 0.0 : a;
+            const int valid = (a != 0.0) * ((mask ? mask[p] : 1) > 0);
+            sum += valid * a;
+            norm += valid;
+        }
+        out[b] = norm == 0 ? 1.0 : sum / norm;
+    }
+    return 0;
+}
+
+// --------------------------------------------------------------------------
 // despeck: SHP-masked neighbourhood mean (despeck.cpp:300-433) over a whole
 // image (the reference's blocks only bound memory: windows are truncated at
 // the image edges, as here).  mode 0: |b1|; 1: b1 conj(b2); 2: b1 conj(b2)

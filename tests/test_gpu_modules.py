@@ -244,3 +244,31 @@ def test_despeck_module_matches_oracle(tmp_path, bands, coh):
     else:
         got = raster.read(out)
         assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+
+
+def test_calamp_module_feeds_nmap(tmp_path):
+    # calamp.py writes a VRT of the stack whose bands carry amplitudeConstant
+    # (6 significant digits); nmap.py reads it back (nmap.cpp:204-233)
+    from fringe_amd.cli import calamp as calamp_cli
+    n, rows, cols, Nx, Ny = 6, 40, 36, 3, 2
+    print("case calamp", flush=True)
+    slc = synthetic.stack(n, rows, cols, seed=71, patches=6)
+    slc *= np.linspace(0.5, 3.0, n)[:, None, None].astype(np.float32)
+    sv, dates = write_stack(tmp_path, slc)
+    mask = (np.arange(rows * cols).reshape(rows, cols) % 7 != 0).astype(np.uint8)
+    mf = str(tmp_path / "mask")
+    raster.write_envi(mf, mask)
+    out = str(tmp_path / "calib.vrt")
+    assert calamp_cli.main(["-i", sv, "-o", out, "-m", mf, "-r", "1", "-l", "8"]) == 0
+    want = oracle.calamp(slc, mask)
+    got = [raster.metadata(out, "amplitudeConstant", "slc", band=b + 1) for b in range(n)]
+    assert got == [f"{v:g}" for v in want]
+    assert [raster.metadata(out, "Date", "slc", band=b + 1) for b in range(n)] == dates
+    assert np.array_equal(raster.read(out, band=3), raster.read(sv, band=3))
+    ac = np.array([float(v) for v in got])
+    wr, cr = oracle.nmap(slc, Nx, Ny, 0.05, amp_const=ac)
+    wf, cf = str(tmp_path / "nmap" / "wts"), str(tmp_path / "nmap" / "count")
+    assert nmap_cli.main(["-i", out, "-o", wf, "-c", cf, "-x", str(Nx), "-y", str(Ny)]) == 0
+    nul = oracle.mask_words(Nx, Ny)
+    wg = np.stack([raster.read(wf, band=k + 1, dtype=np.uint32) for k in range(nul)], axis=-1)
+    assert np.array_equal(wg, wr)

@@ -222,3 +222,28 @@ def test_despeck_module_errors_before_gpu_work(tmp_path):
     assert d.run() == 109                    # last failing check wins, as in despeck.cpp:90-140
     d.halfWindowX, d.halfWindowY = 2, 1
     assert d.run() == 110
+
+
+def test_calamp_module_errors_before_gpu_work(tmp_path):
+    from fringe_amd.calamplib import Calamp
+    c = Calamp()
+    c.verbose = False
+    c.inputDS = str(tmp_path / "missing.vrt")
+    c.outputDS = str(tmp_path / "out.vrt")
+    assert c.run() == 102
+    rows, cols = 5, 6
+    slc = synthetic.stack(2, rows, cols, seed=4)
+    srcs = []
+    for t in range(2):
+        fn = str(tmp_path / f"s{t}.slc")
+        raster.write_envi(fn, slc[t])
+        srcs.append((fn, f"2022010{t + 1}", False))
+    sv = str(tmp_path / "stack.vrt")
+    open(sv, "w").write(raster.stack_vrt(srcs, cols, rows))
+    c.inputDS = sv
+    c.maskDS = str(tmp_path / "nomask")
+    assert c.run() == 103
+    mf = str(tmp_path / "mask")
+    raster.write_envi(mf, np.ones((rows, cols + 1), np.uint8))
+    c.maskDS = mf
+    assert c.run() == 104

@@ -415,3 +415,23 @@ def test_despeck_oracle_closed_forms():
                         acc += ifg[r + ii, c + jj]
                         k += 1
             assert abs(got[r, c] - acc / k) <= 1e-5 * max(1.0, abs(acc / k))
+
+
+# -------------------------------------------------------------- calamp --
+def test_calamp_oracle_closed_forms():
+    rows, cols = 6, 7
+    z = np.zeros((3, rows, cols), np.complex64)
+    z[0] = 3.0 * np.exp(1j * np.linspace(0, 2, rows * cols)).reshape(rows, cols)
+    z[1, :2] = 2.0j
+    z[1, 2, 3] = np.nan
+    z[2] = 5.0
+    mask = np.ones((rows, cols), np.uint8)
+    mask[:, 0] = 0
+    c = oracle.calamp(z, mask)
+    assert abs(c[0] - 3.0) < 1e-6           # |3 e^{i t}| rounded to float, averaged
+    assert c[1] == 2.0                       # zeros and NaN are not valid pixels
+    assert c[2] == 5.0
+    assert np.array_equal(oracle.calamp(z, np.zeros_like(mask)), np.ones(3))   # nothing valid -> 1.0
+    s = synthetic.stack(2, rows, cols, seed=9)
+    want = [float(np.mean(_amp(s[b])[mask > 0].astype(np.float64))) for b in range(2)]
+    assert np.allclose(oracle.calamp(s, mask), want, rtol=1e-12)

@@ -161,6 +161,78 @@ def serve(args):
     print(json.dumps(out))
 
 
+def cfg3(args):
+    """Covariance-dominated case: N = 64 dates, 4096 columns, half-window
+    10 x 5 (231 neighbours, all-ones mask).  On a band of output rows (with its
+    Ny halo), the per-pixel Frobenius error of Gamma from the fp16 and bf16
+    hi/lo MFMA Grams against the FP64 Gram, and the phase error of the EVD
+    top eigenvector (evd module, EVD) against the FP64 path."""
+    import torch
+    from fringe_amd import synthetic, tile
+    dev = torch.device("cuda:0")
+    n, cols, Nx, Ny = 64, 4096, 10, 5
+    out_rows = args.cfg3_rows
+    rows = out_rows + 2 * Ny
+    torch.cuda.manual_seed(103)
+    slc = synthetic.stack_torch(n, rows, cols, 103, device=dev)
+    W = (2 * Nx + 1) * (2 * Ny + 1)
+    words = np.zeros((W + 31) // 32, np.uint32)
+    for f in range(W):
+        words[f >> 5] |= np.uint32(1 << (f & 31))
+    wts = torch.from_numpy(np.broadcast_to(words.view(np.int32), (rows, cols, len(words))).copy()).to(dev)
+    rb, re_ = Ny, Ny + out_rows
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, (time.perf_counter() - t0) * 1e3
+
+    (g64, np64), t64 = timed(lambda: tile.coherence(slc, wts, Nx, Ny, precision="fp64", module="evd",
+                                                    row_begin=rb, row_end=re_))
+    ok = np64 >= 0
+    nrm = torch.linalg.matrix_norm(g64[ok]).double()
+    res = {}
+    for prec in ("mixed", "mixed_bf16"):
+        (g, npx_), tg = timed(lambda: tile.coherence(slc, wts, Nx, Ny, precision=prec, module="evd",
+                                                     row_begin=rb, row_end=re_))
+        err = torch.linalg.matrix_norm(g[ok].to(torch.complex128) - g64[ok]).double()
+        rel = err / nrm
+        res[prec] = {"gamma_frob_err_p50": float(err.median()), "gamma_frob_err_p99": float(torch.quantile(err, 0.99)),
+                     "gamma_frob_err_max": float(err.max()), "gamma_frob_rel_max": float(rel.max()),
+                     "gram_ms": tg}
+        del g
+    del g64
+    torch.cuda.empty_cache()
+    phases = {}
+    for prec in ("fp64", "mixed", "mixed_bf16"):
+        p = tile.link_params(rows, cols, n, Nx, Ny, module="evd", method="EVD", precision=prec,
+                             row_begin=rb, row_end=re_)
+        r = tile.LinkRunner(p, device=dev)
+        r.run(slc, wts)
+        torch.cuda.synchronize()
+        (_, tl) = timed(lambda: r.run(slc, wts))
+        phases[prec] = (r.phase[:, rb:re_].clone(), r.tcorr[rb:re_].clone(), tl)
+        del r
+    ph64, tc64, t_link64 = phases["fp64"]
+    okp = tc64 > 0
+    for prec in ("mixed", "mixed_bf16"):
+        ph, tc, tl = phases[prec]
+        e = torch.angle(ph * torch.conj(ph64)).abs().amax(dim=0)[okp].double()
+        res[prec].update({"evd_phase_err_p99_rad": float(torch.quantile(e.float(), 0.99)),
+                          "evd_phase_err_max_rad": float(e.max()),
+                          "frac_phase_err_gt_1e-3": float((e > 1e-3).double().mean()),
+                          "tcorr_err_max": float((tc - tc64).abs()[okp].max()), "link_ms": tl})
+    out = {"metric": "cfg3: Gamma Frobenius error and EVD phase error of hi/lo MFMA Grams vs FP64 Gram (N=64)",
+           "config": {"workload": f"N={n}, {out_rows} output rows x {cols} cols (+{Ny}-row halos) of a 4096^2 "
+                                  f"tile, window {2 * Nx + 1}x{2 * Ny + 1}, all-ones mask, evd EVD",
+                      "pixels": int(ok.sum())},
+           "data": "synthetic (seeded, generated on the GPU)", "fp64_gram_ms": t64, "fp64_link_ms": t_link64,
+           "fp16_hilo": res["mixed"], "bf16_hilo": res["mixed_bf16"]}
+    print(json.dumps(out))
+
+
 def parse():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,6 +249,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=102)
     ap.add_argument("--copy-probe", action="store_true", help="run only the copy kernel used to calibrate PMC bytes")
     ap.add_argument("--serve", action="store_true", help="serving workload: Poisson stream of mixed tiles")
+    ap.add_argument("--cfg3", action="store_true",
+                    help="N=64, 21x11 window: coherence and EVD phase errors of the hi/lo Grams against FP64")
+    ap.add_argument("--cfg3-rows", type=int, default=32, help="output rows of the N=64 sample (4096 columns)")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--rate", type=float, default=40.0, help="mean arrivals per second")
     ap.add_argument("--budget-gb", type=float, default=200.0)
@@ -193,6 +268,9 @@ def main():
         return
     if args.serve:
         serve(args)
+        return
+    if args.cfg3:
+        cfg3(args)
         return
     import torch
     from fringe_amd import _lib, synthetic, tile

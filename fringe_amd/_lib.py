@@ -29,6 +29,7 @@ EXPORTS = [
     "fringe_server_destroy", "fringe_adjust_wrapped", "fringe_adjust_unwrapped",
     "fringe_despeck_options_init", "fringe_despeck_process",
     "fringe_calamp_options_init", "fringe_calamp_process",
+    "fringe_coherence_workspace_bytes", "fringe_coherence_run",
 ]
 FRINGE_REQ_QUEUED, FRINGE_REQ_RUNNING, FRINGE_REQ_DONE = 0, 1, 2
 
@@ -96,6 +97,9 @@ def lib():
     L.fringe_link_workspace_bytes.argtypes = [ctypes.POINTER(LinkTile)]
     L.fringe_link_workspace_bytes.restype = sz
     L.fringe_link_run.argtypes = [ctypes.POINTER(LinkTile), vp, vp, vp, vp, vp, vp, sz, vp]
+    L.fringe_coherence_workspace_bytes.argtypes = [ctypes.POINTER(LinkTile)]
+    L.fringe_coherence_workspace_bytes.restype = sz
+    L.fringe_coherence_run.argtypes = [ctypes.POINTER(LinkTile), vp, vp, vp, vp, vp, sz, vp]
     L.fringe_nmap_workspace_bytes.argtypes = [ctypes.POINTER(NmapTile)]
     L.fringe_nmap_workspace_bytes.restype = sz
     L.fringe_nmap_run.argtypes = [ctypes.POINTER(NmapTile), vp, vp, vp, vp, vp, vp, sz, vp]

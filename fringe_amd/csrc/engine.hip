@@ -257,6 +257,39 @@ int fringe_link_run(const fringe_link_tile* t, const void* d_slc, const uint32_t
     return 0;
 }
 
+size_t fringe_coherence_workspace_bytes(const fringe_link_tile* t) {
+    if (link_check(t) != 0) return 0;
+    return link_uses_mfma(t->nbands, t->precision) ? align256(mfma_pack_bytes(t->rows, t->cols, t->nbands)) : 256;
+}
+
+int fringe_coherence_run(const fringe_link_tile* t, const void* d_slc, const uint32_t* d_wts, void* d_gamma,
+                         int* d_npix, void* d_work, size_t work_bytes, void* stream) {
+    const int st = link_check(t);
+    if (st != 0) return st;
+    if (!d_slc || !d_wts || !d_gamma || !d_npix || !d_work) return FRINGE_ERR_ARGUMENT;
+    if (work_bytes < fringe_coherence_workspace_bytes(t)) return FRINGE_ERR_WORKSPACE;
+    LinkPlan P{};
+    P.rows = t->rows;
+    P.cols = t->cols;
+    P.n = t->nbands;
+    P.Nx = t->Nx;
+    P.Ny = t->Ny;
+    P.nul = shp_words(t->Nx, t->Ny);
+    P.module = t->module;
+    P.method = t->method;
+    P.min_nb = (t->module == FRINGE_MODULE_EVD) ? 2 : t->min_neighbors;
+    P.precision = t->precision;
+    P.row_begin = t->row_begin;
+    P.row_end = t->row_end;
+    P.npx = (long long)t->rows * t->cols;
+    P.d_pack = (uint32_t*)d_work;
+    if (coherence_launch(P, (const float2*)d_slc, d_wts, d_gamma, d_npix, (hipStream_t)stream) != 0) {
+        set_last_error("fringe_coherence_run", hipGetLastError());
+        return FRINGE_ERR_DEVICE;
+    }
+    return 0;
+}
+
 double fringe_ks_table(int n, double prob, uint8_t* h_table) {
     const double rn = n;
     const double scale = std::sqrt(rn * rn / (rn + rn));   // KS2sample.hpp:139

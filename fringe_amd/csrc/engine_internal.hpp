@@ -61,6 +61,8 @@ int calamp_launch(const float2* z, const uint8_t* mask, long long n, double2* pa
 int link_check(const fringe_link_tile* t);
 void set_last_error(const std::string& what);
 
+int coherence_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, void* gamma, int* npix,
+                     hipStream_t s);
 int link_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, float2* phase,
                 float* tcorr, float2* comp, hipStream_t s);
 size_t link_gamma_bytes_per_pixel(int n, int precision);

@@ -210,6 +210,43 @@ size_t link_gamma_bytes_per_pixel(int n, int precision) {
     return (size_t)n * n * e + sizeof(int);
 }
 
+// Coherence matrices only (the Gram stage of link_launch), into a caller
+// buffer covering the output rows: complex64 from the MFMA Gram, complex128
+// from the FP64 one.
+int coherence_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, void* gamma, int* npix,
+                     hipStream_t s) {
+    const long long first = (long long)P.row_begin * P.cols;
+    const int slots = (P.row_end - P.row_begin) * P.cols;
+    if (slots <= 0) return 0;
+    if (link_uses_mfma(P.n, P.precision)) {
+        if (pack_launch(slc, P.rows, P.cols, P.n, P.precision != FRINGE_PREC_MIXED_BF16, P.d_pack, s) != 0)
+            return -1;
+        LinkPlan Q = P;
+        Q.d_gamma = gamma;
+        Q.d_npix = npix;
+        return gram_mfma_launch(Q, P.d_pack, wts, first, slots, s);
+    }
+    GramArgs g{};
+    g.slc = slc;
+    g.wts = wts;
+    g.rows = P.rows;
+    g.cols = P.cols;
+    g.n = P.n;
+    g.Nx = P.Nx;
+    g.Ny = P.Ny;
+    g.nul = P.nul;
+    g.min_nb = P.min_nb;
+    g.npx = P.npx;
+    g.first = first;
+    g.slots = slots;
+    g.gamma = gamma;
+    g.npix = npix;
+    if (P.n <= 32) launch_gram_f64<32, 1, 32>(g, s);
+    else if (P.n <= 64) launch_gram_f64<64, 1, 32>(g, s);
+    else launch_gram_f64<64, 2, 16>(g, s);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int link_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, float2* phase,
                 float* tcorr, float2* comp, hipStream_t s) {
     const long long first_out = (long long)P.row_begin * P.cols;

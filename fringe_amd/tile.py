@@ -92,6 +92,29 @@ def link(slc, wts, Nx, Ny, **kw):
     return runner.run(slc, wts)
 
 
+def coherence(slc, wts, Nx, Ny, precision="mixed", module="phase_link", min_neighbors=2, row_begin=0,
+              row_end=None, stream=None):
+    """Coherence matrices of a device tile's output rows: (gamma [pixels, n, n]
+    complex64 (MFMA Gram) or complex128, npix [pixels] int32, -1 = not
+    linked)."""
+    n, rows, cols = slc.shape
+    p = link_params(rows, cols, n, Nx, Ny, module=module, min_neighbors=min_neighbors, precision=precision,
+                    row_begin=row_begin, row_end=row_end)
+    L = _lib.lib()
+    wb = L.fringe_coherence_workspace_bytes(ctypes.byref(p))
+    if wb == 0:
+        raise RuntimeError("invalid or unsupported coherence parameters")
+    work = torch.empty(wb, dtype=torch.uint8, device=slc.device)
+    npx = (p.row_end - p.row_begin) * cols
+    c64 = p.precision != _lib.FRINGE_PREC_FP64 and n <= 64
+    gam = torch.empty((npx, n, n), dtype=torch.complex64 if c64 else torch.complex128, device=slc.device)
+    npix = torch.empty(npx, dtype=torch.int32, device=slc.device)
+    rc = L.fringe_coherence_run(ctypes.byref(p), _ptr(slc.contiguous()), _ptr(wts.contiguous()), _ptr(gam),
+                                _ptr(npix), _ptr(work), wb, _stream(stream))
+    _check(rc, "fringe_coherence_run")
+    return gam, npix
+
+
 def nmap(slc, Nx, Ny, prob=0.05, mask=None, amp_const=None, row_begin=0, row_end=None,
          stream=None, method="KS2"):
     """SHP map of a device tile (method "KS2" or "AD2").  Returns

@@ -134,6 +134,17 @@ int fringe_link_run(const fringe_link_tile* t, const void* d_slc, const uint32_t
                     void* d_phase, float* d_tcorr, void* d_comp, void* d_work,
                     size_t work_bytes, void* stream);
 
+/* Coherence matrices alone (the first stage of fringe_link_run,
+ * evd.cpp:530-582): for every output pixel, slot = (row - row_begin) * cols
+ * + col, d_gamma[slot][i][j] = Gamma_ij (row-major, Hermitian, unit
+ * diagonal) and d_npix[slot] = number of neighbours, or -1 when the pixel is
+ * not linked (centre bit clear, too few neighbours).  Elements are complex64
+ * when the precision uses the MFMA Gram (MIXED / MIXED_BF16 with nbands <=
+ * 64) and complex128 otherwise. */
+size_t fringe_coherence_workspace_bytes(const fringe_link_tile* t);
+int fringe_coherence_run(const fringe_link_tile* t, const void* d_slc, const uint32_t* d_wts, void* d_gamma,
+                         int* d_npix, void* d_work, size_t work_bytes, void* stream);
+
 /* ---- tile engine: SHP map (KS2 two-sample test) -------------------------
  * d_mask (optional, uint8 [row][col]) marks usable pixels.  h_amp_const holds
  * the per-band amplitudeConstant metadata (<= 0 or NULL: missing), normalised

@@ -137,3 +137,38 @@ def test_mixed_precision_tcorr_and_phase_at_cfg1_scale():
         print(precision, st)
         assert st["n_well"] == rows * cols
         assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL
+
+
+@pytest.mark.parametrize("n", [10, 30, 48])
+def test_coherence_matrices(n):
+    # fringe_coherence_run: FP64 Gram against a numpy FP64 brute force of
+    # evd.cpp:530-582 (complex<float> products, so 1e-6), the fp16 hi/lo MFMA
+    # Gram against the FP64 one
+    rows, cols, Nx, Ny = 10, 12, 3, 2
+    print(f"case coherence n={n}", flush=True)
+    slc = synthetic.stack(n, rows, cols, seed=110 + n, patches=4)
+    wts, _ = oracle.nmap(slc, Nx, Ny, 0.05)
+    g64, np64 = tile.coherence(to_dev(slc, torch), wts_dev(wts, torch), Nx, Ny, precision="fp64", min_neighbors=2)
+    g16, np16 = tile.coherence(to_dev(slc, torch), wts_dev(wts, torch), Nx, Ny, precision="mixed", min_neighbors=2)
+    torch.cuda.synchronize()
+    g64, np64, g16, np16 = g64.cpu().numpy(), np64.cpu().numpy(), g16.cpu().numpy(), np16.cpu().numpy()
+    assert np.array_equal(np64, np16)
+    for r in range(rows):
+        for c in range(cols):
+            k = r * cols + c
+            sel = []
+            for ii in range(-Ny, Ny + 1):
+                for jj in range(-Nx, Nx + 1):
+                    f = (ii + Ny) * (2 * Nx + 1) + jj + Nx
+                    if 0 <= r + ii < rows and 0 <= c + jj < cols and (wts[r, c, f >> 5] >> (f & 31)) & 1:
+                        sel.append(slc[:, r + ii, c + jj])
+            if not (wts[r, c, (Ny * (2 * Nx + 1) + Nx) >> 5] >> ((Ny * (2 * Nx + 1) + Nx) & 31)) & 1 or len(sel) < 2:
+                assert np64[k] == -1
+                continue
+            Z = np.array(sel, np.complex128)
+            G = Z.T @ Z.conj()
+            d = np.sqrt(np.real(np.diag(G)))
+            want = G / np.outer(d, d)
+            assert np64[k] == len(sel)
+            assert np.abs(g64[k] - want).max() < 1e-6
+            assert np.abs(g16[k] - g64[k]).max() < 1e-4

@@ -233,6 +233,81 @@ def cfg3(args):
     print(json.dumps(out))
 
 
+def cfg4(args):
+    """Sequential estimator on device-resident data (sequential.py:144-255):
+    120 dates of 1024 x 1024 (12-day spacing, same coherence model), split into
+    mini-stacks of lengths drawn from 8..20; mini-stack k = compressed SLCs of
+    mini-stacks 1..k-1 + its own dates, linked by the evd module (MLE) with
+    reference index k-1; its compressed SLC joins the next ones; the stack of
+    compressed SLCs is linked last (datum connection).  SHP mask: nmap KS2
+    over all 120 dates, half-window 7 x 5.  Timed for the mixed and the plain
+    FP64 path."""
+    import torch
+    from fringe_amd import synthetic, tile
+    dev = torch.device("cuda:0")
+    n_all, side, Nx, Ny = 120, 1024, 7, 5
+    rng = np.random.default_rng(104)
+    sizes = []
+    while sum(sizes) < n_all:
+        sizes.append(int(rng.integers(8, 21)))
+    sizes[-1] -= sum(sizes) - n_all
+    if sizes[-1] < 1:
+        sizes.pop()
+    torch.cuda.manual_seed(104)
+    slc = synthetic.stack_torch(n_all, side, side, 104, device=dev, patches=32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    wts, _ = tile.nmap(slc, Nx, Ny, 0.05)
+    torch.cuda.synchronize()
+    t_nmap = time.perf_counter() - t0
+    print(f"nmap over {n_all} dates: {t_nmap:.2f} s", file=sys.stderr, flush=True)
+
+    def run(prec):
+        comps, outs, per = [], [], []
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        s0 = 0
+        for k, L in enumerate(sizes, start=1):
+            stack = torch.cat([torch.stack(comps), slc[s0:s0 + L]]) if comps else slc[s0:s0 + L].contiguous()
+            t1 = time.perf_counter()
+            ph, tc, co = tile.link(stack, wts, Nx, Ny, module="evd", method="MLE", ref_index=k - 1,
+                                   precision=prec)
+            torch.cuda.synchronize()
+            per.append((stack.shape[0], (time.perf_counter() - t1) * 1e3))
+            comps.append(co)
+            outs.append(ph[k - 1:])
+            s0 += L
+        t1 = time.perf_counter()
+        ph, tc, co = tile.link(torch.stack(comps), wts, Nx, Ny, module="evd", method="MLE", ref_index=0,
+                               precision=prec)
+        torch.cuda.synchronize()
+        per.append((len(comps), (time.perf_counter() - t1) * 1e3))
+        total = time.perf_counter() - t_start
+        return total, per, outs, ph
+
+    res = {}
+    for prec in ("mixed", "fp64"):
+        run(prec)   # warm-up (kernel loading for every N)
+        res[prec] = run(prec)
+        print(f"{prec}: {res[prec][0]:.2f} s", file=sys.stderr, flush=True)
+    err = 0.0
+    for a, b in zip(res["mixed"][2] + [res["mixed"][3]], res["fp64"][2] + [res["fp64"][3]]):
+        ok = (b.abs() > 0).all(dim=0)
+        err = max(err, float(torch.angle(a * torch.conj(b)).abs().amax(dim=0)[ok].quantile(0.999)))
+    links = len(sizes) + 1
+    out = {"metric": "cfg4: sequential mini-stacks (pixels/s over all mini-stack and datum links)",
+           "value": links * side * side / res["mixed"][0], "unit": "pixels/s",
+           "fp64_value": links * side * side / res["fp64"][0], "speedup_vs_fp64": res["fp64"][0] / res["mixed"][0],
+           "mini_stack_sizes": sizes, "links": links, "mixed_s": res["mixed"][0], "fp64_s": res["fp64"][0],
+           "per_link_ms_mixed": [[nk, round(ms, 2)] for nk, ms in res["mixed"][1]],
+           "per_link_ms_fp64": [[nk, round(ms, 2)] for nk, ms in res["fp64"][1]],
+           "phase_err_p999_rad_vs_fp64": err, "nmap_s": t_nmap,
+           "config": {"workload": f"{n_all} dates, {side}x{side}, mini-stacks U{{8..20}}, evd MLE, window "
+                                  f"{2 * Nx + 1}x{2 * Ny + 1}, nmap KS2 over all dates"},
+           "data": "synthetic (seeded, generated on the GPU)"}
+    print(json.dumps(out))
+
+
 def parse():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -252,6 +327,8 @@ def parse():
     ap.add_argument("--cfg3", action="store_true",
                     help="N=64, 21x11 window: coherence and EVD phase errors of the hi/lo Grams against FP64")
     ap.add_argument("--cfg3-rows", type=int, default=32, help="output rows of the N=64 sample (4096 columns)")
+    ap.add_argument("--cfg4", action="store_true",
+                    help="sequential mini-stacks of uneven length (120 dates, 1024^2), on device-resident data")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--rate", type=float, default=40.0, help="mean arrivals per second")
     ap.add_argument("--budget-gb", type=float, default=200.0)
@@ -271,6 +348,9 @@ def main():
         return
     if args.cfg3:
         cfg3(args)
+        return
+    if args.cfg4:
+        cfg4(args)
         return
     import torch
     from fringe_amd import _lib, synthetic, tile

@@ -1,15 +1,16 @@
 // solve32.hpp -- register-resident per-pixel estimator for n <= 32 dates.
 //
 // A wave holds two pixels; lane r of a 32-lane half owns row r of every
-// matrix of its pixel in registers.  LDS (about 3 KB per pixel) is used only
+// matrix of its pixel in registers.  LDS (about 3.5 KB per pixel) is used only
 // to publish one row or one vector to the other lanes, so many waves fit on
 // a CU and the arithmetic is not serialised behind LDS round trips.
 //
 // Stages (same semantics as the generic LDS solver in link_solve.hpp):
-//   * inverse of the real |Gamma| by a Gauss-Jordan sweep in FP64.  The
-//     sweep keeps the invariant M_rc = s_r s_c M_cr with s_k = -1 for pivots
-//     already swept, so the multiplier M_rj a lane needs at step j is read
-//     from the published pivot row instead of from a runtime register index;
+//   * inverse of the real |Gamma| by the symmetric sweep operator in FP64,
+//     with the matrix in 4 x 8 blocks (lane (a, b) of a half holds rows
+//     4a..4a+3, columns 8b..8b+7): per pivot a lane reads 4 multipliers and
+//     8 pivot-row entries from LDS instead of a whole row, then the blocks go
+//     back to one row per lane a block column at a time;
 //   * H = |Gamma|^-1 o Gamma (or Gamma itself for EVD) into registers;
 //   * Householder tridiagonalisation: the owner of row j publishes it, the
 //     reflector v_j = conj(row j) * scal_j is applied, and every lane parks
@@ -33,8 +34,9 @@ template <typename T> struct Lds32 {
     T D[32], E[32], E2[32];
     T B[32];          // inverse-iteration right-hand side
     T FP[32], FQ[32], FS[32], FX[32];   // eliminated rows
-    double TR[32];    // sweep: scaled pivot row
+    double TR[32];    // sweep: published pivot row
     double PV;        // sweep: pivot
+    double XT[4][32]; // sweep: one block column on its way to the row layout
     cx<float> OUT[32];
 };
 
@@ -96,51 +98,96 @@ template <typename T> __device__ inline T max32(T x) { return red32<OpMax>(x); }
 template <typename T> __device__ inline T min32(T x) { return red32<OpMin>(x); }
 template <typename T> __device__ inline cx<T> sum32(cx<T> x) { return cx<T>{sum32(x.re), sum32(x.im)}; }
 
-// ---- FP64 sweep inverse of |Gamma| - shift*I, rows in registers ----------
-// m[c] (c != r) holds M_rc, dr holds M_rr; m[r] is a free slot.
+// ---- FP64 inverse of |Gamma| - shift*I in 4 x 8 blocks --------------------
+// Lane (a, b) = (r / 4, r % 4) of the half holds M[i][j] = A(4a + i, 8b + j).
+// Rows and columns >= n hold the identity (never swept, never mixed in).
 template <typename Tg>
-__device__ inline void abs_gamma_row(double (&m)[32], double& dr, const cx<Tg>* grow, int n, int r,
-                                     bool act, double shift) {
+__device__ inline bool abs_gamma_block(double (&M)[4][8], const cx<Tg>* gam, int n, int r, bool act,
+                                       double shift) {
+    const int a = r >> 2, b = r & 3;
+    bool fin = true;
 #pragma unroll
-    for (int c = 0; c < 32; ++c) {
-        double v = 0.0;
-        if (act && c < n && r < n && c != r) {
-            const cx<Tg> g = grow[c];
-            v = (double)sqrt(g.re * g.re + g.im * g.im);
+    for (int i = 0; i < 4; ++i) {
+        const int row = 4 * a + i;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int col = 8 * b + j;
+            double v = row == col ? 1.0 - shift : 0.0;
+            if (act && row < n && col < n && row != col) {
+                const cx<Tg> g = gam[(size_t)row * n + col];
+                v = (double)sqrt(g.re * g.re + g.im * g.im);
+                fin = fin && isfinite(v);
+            }
+            M[i][j] = v;
         }
-        m[c] = v;
     }
-    dr = 1.0 - shift;
+    return fin;
 }
 
+// Symmetric sweep operator: sweeping pivot k maps
+//   A_kk -> -1/A_kk,  A_rk -> A_rk/A_kk,  A_kc -> A_kc/A_kk,
+//   A_rc -> A_rc - A_rk A_kc / A_kk                        (r, c != k)
+// and keeps A symmetric, so the multiplier of row r is entry r of the
+// published pivot row P.  With P_k = A_kk - 1 every entry but (k, k) is
+// A_rc - (P_r / p) P_c; (k, k) is set to -1/p.  After pivots 0..n-1 the
+// blocks hold -A^-1.  Returns whether every pivot was positive (A is
+// positive definite).
 template <typename T>
-__device__ inline bool sweep32(Lds32<T>& L, double (&m)[32], double& dr, int n, int r) {
+__device__ inline bool sweep_block(Lds32<T>& L, double (&M)[4][8], int n, int r) {
+    const int a = r >> 2, b = r & 3;
     bool ok = true;
-    for (int j = 0; j < n; ++j) {
-        // the pivot row is published unscaled; every lane folds 1/p into its
-        // own multiplier (T = M_j. / p is never formed)
-        if (r == j) {
+    for (int q = 0; 8 * q < n; ++q) {
 #pragma unroll
-            for (int c = 0; c < 32; ++c) L.TR[c] = m[c];
-            L.TR[j] = dr + 1.0;   // (p + 1) / p = 1 + 1/p after the scaling
-            L.PV = dr;
+        for (int kj = 0; kj < 8; ++kj) {
+            const int k = 8 * q + kj;
+            if (k >= n) break;
+            const int ki = kj & 3;   // row k is row ki of the blocks with a == k / 4
+            if (a == (k >> 2)) {
+                double pr[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) pr[j] = M[ki][j];
+                if (b == q) {   // column k is column kj of the blocks with b == q
+                    L.PV = pr[kj];
+                    pr[kj] = pr[kj] - 1.0;
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j) L.TR[8 * b + j] = pr[j];
+            }
+            wsync();
+            const double p = L.PV;
+            ok = ok && (p > 0.0);
+            const double piv = fast_rcp(p);
+            double pc[8], g[4];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) pc[j] = L.TR[8 * b + j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) g[i] = L.TR[4 * a + i] * piv;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) M[i][j] = fma(-g[i], pc[j], M[i][j]);
+            if (a == (k >> 2) && b == q) M[ki][kj] = -piv;
+            wsync();
         }
-        wsync();
-        const double p = L.PV;
-        ok = ok && (p > 0.0);
-        const double piv = fast_rcp(p);
-        const double mjr = L.TR[r];
-        // rows r != j: M_rc -= M_rj T_c with M_rj = s_r M_jr read from the
-        // published row; the pivot row becomes M_j. / p = M_jc - ((p - 1) / p) M_jc,
-        // so one FMA form covers every lane
-        const bool me = (r == j);
-        const double g = (me ? p - 1.0 : (r < j ? -mjr : mjr)) * piv;
-#pragma unroll
-        for (int c = 0; c < 32; ++c) m[c] = fma(-g, L.TR[c], m[c]);
-        dr = me ? piv : fma(-g, mjr, dr);
-        wsync();
     }
     return ok;
+}
+
+// Blocks -> rows: m[c] = M(r, c).  Block column j (columns j, 8 + j, 16 + j,
+// 24 + j) passes through LDS at a time, so the registers it frees are the
+// ones the row layout fills.
+template <typename T>
+__device__ inline void blocks_to_rows(Lds32<T>& L, const double (&M)[4][8], double (&m)[32], int r) {
+    const int a = r >> 2, b = r & 3;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) L.XT[b][4 * a + i] = M[i][j];
+        wsync();
+#pragma unroll
+        for (int k = 0; k < 4; ++k) m[8 * k + j] = L.XT[k][r];
+        wsync();
+    }
 }
 
 // ---- Householder tridiagonalisation of the rows in h ---------------------
@@ -362,18 +409,16 @@ __device__ inline bool load_rows(cx<T> (&h)[32], const cx<Tg>* grow, int n, int 
 }
 
 template <typename T, typename Tg>
-__device__ inline bool hadamard_rows(cx<T> (&h)[32], const double (&m)[32], double dr,
-                                     const cx<Tg>* grow, int n, int r, bool act) {
+__device__ inline bool hadamard_rows(cx<T> (&h)[32], const double (&m)[32], const cx<Tg>* grow, int n,
+                                     int r, bool act) {
     bool fin = true;
-    // eight columns at a time, so that the row of Gamma is not loaded whole
-    // while the FP64 row of the inverse is still live
 #pragma unroll
     for (int c0 = 0; c0 < 32; c0 += 8) {
 #pragma unroll
         for (int c = c0; c < c0 + 8; ++c) {
             cx<T> v = {T(0), T(0)};
             if (r < n && c < n) {
-                const double mm = (c == r) ? dr : m[c];
+                const double mm = -m[c];   // m holds -|Gamma|^-1
                 cx<Tg> g = (c == r) ? cx<Tg>{Tg(1), Tg(0)} : cx<Tg>{Tg(0), Tg(0)};
                 if (act && c != r) g = grow[c];
                 const double hr = mm * (double)g.re, hi = mm * (double)g.im;
@@ -382,7 +427,6 @@ __device__ inline bool hadamard_rows(cx<T> (&h)[32], const double (&m)[32], doub
             }
             h[c] = v;
         }
-        __builtin_amdgcn_sched_barrier(0);
     }
     return min32(fin ? 1.0f : 0.0f) > 0.5f;
 }
@@ -418,14 +462,15 @@ __global__ __launch_bounds__(64) void solve32_kernel(SolveArgs a) {
         const T lmin = bisect32<T>(L, n, 0, r, tn);
         if (!fin) { code = -1.0f; failed = true; }
         else if (!(lmin >= T(1.0e-6))) { code = -2.0f; failed = true; }
-        double m[32], dr;
-        abs_gamma_row<Tg>(m, dr, grow, n, r, act, 1.0e-6);
-        const bool pd_shift = sweep32<T>(L, m, dr, n, r);
+        double M[4][8], m[32];
+        abs_gamma_block<Tg>(M, gam, n, r, act, 1.0e-6);
+        const bool pd_shift = sweep_block<T>(L, M, n, r);
         if (!failed && !pd_shift) { code = -4.0f; failed = true; }
-        abs_gamma_row<Tg>(m, dr, grow, n, r, act, 0.0);
-        const bool pd = sweep32<T>(L, m, dr, n, r);
+        abs_gamma_block<Tg>(M, gam, n, r, act, 0.0);
+        const bool pd = sweep_block<T>(L, M, n, r);
         if (!failed && !pd) { code = -5.0f; failed = true; }
-        const bool hfin = hadamard_rows<T, Tg>(h, m, dr, grow, n, r, act);
+        blocks_to_rows<T>(L, M, m, r);
+        const bool hfin = hadamard_rows<T, Tg>(h, m, grow, n, r, act);
         tridiag32<T>(L, h, n, r);
         const T lam = bisect32<T>(L, n, 0, r, tn);
         const T yr = tri_eigvec32<T>(L, n, r, lam, tn);
@@ -442,14 +487,12 @@ __global__ __launch_bounds__(64) void solve32_kernel(SolveArgs a) {
         if (!fin) { code = -6.0f; failed = true; }
         else if (!(lam >= T(1.0e-6))) { code = -7.0f; failed = true; }
     } else {
-        double m[32], dr;
-        abs_gamma_row<Tg>(m, dr, grow, n, r, act, 0.0);
-        bool fin = true;
-#pragma unroll
-        for (int c = 0; c < 32; ++c) fin = fin && isfinite(m[c]);
+        double M[4][8], m[32];
+        bool fin = abs_gamma_block<Tg>(M, gam, n, r, act, 0.0);
         fin = min32(fin ? 1.0f : 0.0f) > 0.5f;
-        const bool pd = sweep32<T>(L, m, dr, n, r);
-        const bool hfin = hadamard_rows<T, Tg>(h, m, dr, grow, n, r, act);
+        const bool pd = sweep_block<T>(L, M, n, r);
+        blocks_to_rows<T>(L, M, m, r);
+        const bool hfin = hadamard_rows<T, Tg>(h, m, grow, n, r, act);
         const bool run_evd = !pd || !hfin;
         tridiag32<T>(L, h, n, r);
         const T lam = bisect32<T>(L, n, 0, r, tn);
@@ -502,17 +545,26 @@ __global__ __launch_bounds__(64) void solve32_kernel(SolveArgs a) {
         const cx<T> ci = unit_or(cx<T>{(T)of.re, -(T)of.im}, cx<T>{T(1), T(0)}, okx);
         const int half = (n - 1) / 2;
         const int dmax = ((n & 1) == 0 && r < n / 2) ? half + 1 : half;
-        for (int d = 1; d <= dmax; ++d) {
-            int j = r + d;
-            if (j >= n) j -= n;
+        // all gathers first, then the sums; jd[d-1] = (r + d) mod n, stepped
+        // so that it stays in [0, n) for every n >= 1
+        int jd[16];
+        cx<Tg> gv[16];
+#pragma unroll
+        for (int d = 1; d <= 16; ++d) {
+            const int prev = d == 1 ? r : jd[d - 2];
+            jd[d - 1] = prev + 1 == n ? 0 : prev + 1;
+            gv[d - 1] = grow[jd[d - 1]];
+        }
+#pragma unroll
+        for (int d = 1; d <= 16; ++d) {
+            const int j = jd[d - 1];
             const int lo = min(r, j), hi = max(r, j);
-            if (stbas && hi - lo > a.bw) continue;
-            const cx<Tg> gv = grow[j];
-            const cx<T> ph = unit_or(cx<T>{(T)gv.re, (T)gv.im}, cx<T>{T(1), T(0)}, okx);
+            const bool use = d <= dmax && !(stbas && hi - lo > a.bw);
+            const cx<T> ph = unit_or(cx<T>{(T)gv[d - 1].re, (T)gv[d - 1].im}, cx<T>{T(1), T(0)}, okx);
             const cx<float> oj = L.OUT[j];
             cx<T> t = cmul(cmul(ph, ci), cx<T>{(T)oj.re, (T)oj.im});
             if (r > j) t = cconj(t);
-            tc = cadd(tc, t);
+            if (use) tc = cadd(tc, t);
         }
     }
     tc = sum32(tc);

@@ -182,6 +182,9 @@ static int link_chunk_pixels(const fringe_link_tile* t) {
     const size_t target = (size_t)192 << 20;   // keep Gamma of one round near the last-level cache
     long long rows = (long long)(target / (per * (size_t)t->cols));
     if (rows < 1) rows = 1;
+    // whole pixel tiles of the staged Gram kernel
+    const int th = link_uses_mfma(t->nbands, t->precision) ? gram_stage_rows(t->nbands, t->Nx, t->Ny) : 1;
+    if (rows > th) rows -= rows % th;
     const long long out_rows = t->row_end - t->row_begin;
     if (rows > out_rows) rows = out_rows > 0 ? out_rows : 1;
     return (int)(rows * t->cols);

@@ -71,5 +71,7 @@ int pack_launch(const float2* slc, int rows, int cols, int n, bool f16, uint32_t
 int gram_mfma_launch(const LinkPlan& P, const uint32_t* pk, const uint32_t* wts, long long first,
                      int slots, hipStream_t s);
 bool link_uses_mfma(int n, int precision);
+// pixel rows of one workgroup tile of the staged Gram kernel (1 when it is not used)
+int gram_stage_rows(int n, int Nx, int Ny);
 
 }  // namespace fringe

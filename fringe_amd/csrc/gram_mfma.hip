@@ -338,9 +338,12 @@ __global__ __launch_bounds__(256) void gram_mfma_lds(MfmaArgs a) {
     const int TH = a.TH, TW = a.TW;
     const int WR = TH + 2 * Ny, WC = TW + 2 * Nx, NPOS = WR * WC;
     uint2* rec = (uint2*)smem;                             // [NPOS + 1][32] (re, im) words
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int* list = (int*)(rec + (size_t)(NPOS + 1) * 32) + wave * (a.list_cap + 32);
+    // wave index in a scalar register, so the pixel loop below is uniform
+    // control flow (scalar branches) rather than exec-masked
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    int* list = (int*)(rec + (size_t)(NPOS + 1) * 32) + wave * (a.list_cap + 32 + 64);
     float* dg = (float*)(list + a.list_cap);                // diagonal of Re G, 32
+    uint32_t* wbuf = (uint32_t*)(dg + 32);                  // next pixel's mask words, 64
 
     // XCD-aware tile order (blocks are dealt to the 8 XCDs round-robin)
     const unsigned nb = gridDim.x * gridDim.y, bid = blockIdx.y * gridDim.x + blockIdx.x;
@@ -352,15 +355,15 @@ __global__ __launch_bounds__(256) void gram_mfma_lds(MfmaArgs a) {
     const int rbase = R0 - Ny, cbase = C0 - Nx;
 
     // ---- stage the records, padded to 32 dates (zeros beyond n and outside
-    //      the image), four loads in flight ----
+    //      the image), eight loads in flight ----
     {
         const int total = (NPOS + 1) * 32;
         const uint2* src = (const uint2*)a.pk;
-        for (int e0 = threadIdx.x; e0 < total; e0 += 4 * 256) {
-            uint2 v[4];
-            bool in[4];
+        for (int e0 = threadIdx.x; e0 < total; e0 += 8 * 256) {
+            uint2 v[8];
+            bool in[8];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < 8; ++u) {
                 const int e = e0 + u * 256;
                 const int pos = e >> 5, d = e & 31;
                 const int rr = rbase + pos / WC, cc = cbase + pos % WC;
@@ -370,7 +373,7 @@ __global__ __launch_bounds__(256) void gram_mfma_lds(MfmaArgs a) {
                 v[u] = src[at];
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < 8; ++u) {
                 const int e = e0 + u * 256;
                 if (e < total) rec[e] = in[u] ? v[u] : make_uint2(0u, 0u);
             }
@@ -390,13 +393,28 @@ __global__ __launch_bounds__(256) void gram_mfma_lds(MfmaArgs a) {
         offx[it] = f - offy[it] * Wx;
     }
 
-    for (int pid = wave; pid < TH * TW; pid += 4) {
+    // mask words of a pixel in lanes 0..nul-1 (one load).  The next pixel's
+    // are requested at the top of an iteration and parked in LDS before this
+    // pixel's Gamma stores, so no wait for them ever waits for the stores.
+    auto mask_word = [&](int pid) -> uint32_t {
         const int r = R0 + pid / TW, c = C0 + pid % TW;
-        if (r >= row_end || r >= a.rows || c >= a.cols) continue;   // wave-uniform
+        const bool ok = pid < TH * TW && r < row_end && r < a.rows && c < a.cols;
+        // always one load, of pixel 0 when there is no pixel (such a pixel is
+        // skipped before its word is used)
+        const long long px = ok ? (long long)r * a.cols + c : 0;
+        return a.wts[px * a.nul + (lane < a.nul ? lane : 0)];
+    };
+    wbuf[lane] = mask_word(wave);
+    for (int pid = wave; pid < TH * TW; pid += 4) {
+        const uint32_t wword = wbuf[lane];
+        const uint32_t wnext = mask_word(pid + 4);
+        const int r = R0 + pid / TW, c = C0 + pid % TW;
+        if (r >= row_end || r >= a.rows || c >= a.cols) {   // wave-uniform
+            wbuf[lane] = wnext;
+            continue;
+        }
         const long long pix = (long long)r * a.cols + c;
         const int slot = (r - a.row0) * a.cols + c;
-        // mask words in lanes 0..nul-1, one load for the pixel
-        const uint32_t wword = a.wts[pix * a.nul + (lane < a.nul ? lane : 0)];
         const int fc = shp_flat(0, 0, Nx, Ny);
         const bool cen = (__shfl(wword, fc >> 5) >> (fc & 31)) & 1u;
         // ---- neighbour list (staged record indices) ----
@@ -425,6 +443,7 @@ __global__ __launch_bounds__(256) void gram_mfma_lds(MfmaArgs a) {
             cnt += __popcll(bal);
         }
         if (!(cen && cnt >= a.min_nb)) {
+            wbuf[lane] = wnext;
             if (lane == 0) a.npix[slot] = -1;
             continue;
         }
@@ -474,34 +493,63 @@ __global__ __launch_bounds__(256) void gram_mfma_lds(MfmaArgs a) {
             for (int e = 0; e < 8; ++e) x[e] = nx[e];
         }
 
+        wbuf[lane] = wnext;   // waits for the load here, before the stores below
+
         // ---- fold P into Gamma ----
+        // Re G_mm sits in register 4 (m / 8) + m % 4 of the lanes whose half
+        // holds row m ((m / 4) % 2 == hf); it is picked out by selects, and
+        // every later LDS read is issued before any result is used.
+        float dv = 0.0f;
 #pragma unroll
-        for (int v = 0; v < 16; ++v)
-            if (acc_row(v, lane) == m) dg[m] = RR[v] + II[v];
+        for (int v = 0; v < 16; ++v) dv = (acc_row(v, lane) == m) ? RR[v] + II[v] : dv;
+        if (((m >> 2) & 1) == hf) dg[m] = dv;
         wave_sync_lds();
-        float* gout = a.gamma + (size_t)slot * n * n * 2;
+        float di[16];
+#pragma unroll
+        for (int v = 0; v < 16; ++v) di[v] = dg[acc_row(v, lane)];
         const float dj = dg[m];
+        float2* gout = reinterpret_cast<float2*>(a.gamma) + (size_t)slot * n * n;
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const int i = acc_row(v, lane);
-            if (i >= n || m >= n) continue;
-            float gr = 1.0f, gi = 0.0f;
-            if (i != m) {
-                const float sc = rsqrtf(dg[i] * dj);
-                gr = (RR[v] + II[v]) * sc;
-                gi = (IR[v] - RI[v]) * sc;   // P(im_i, re_j) - P(re_i, im_j)
-            }
-            reinterpret_cast<float2*>(gout)[(size_t)i * n + m] = make_float2(gr, gi);
+            const float sc = rsqrtf(di[v] * dj);
+            const bool off = i != m;
+            const float gr = off ? (RR[v] + II[v]) * sc : 1.0f;
+            const float gi = off ? (IR[v] - RI[v]) * sc : 0.0f;   // P(im_i, re_j) - P(re_i, im_j)
+            if (i < n && m < n) gout[(size_t)i * n + m] = make_float2(gr, gi);
         }
         if (lane == 0) a.npix[slot] = cnt;
         wave_sync_lds();
     }
 }
 
-static size_t gram_lds_bytes(int n, int Nx, int Ny, int TH, int TW, int list_cap) {
-    (void)n;   // records are staged padded to 32 dates
+static size_t gram_lds_bytes(int Nx, int Ny, int TH, int TW, int list_cap) {
+    // records are staged padded to 32 dates
     const size_t stage = (size_t)((TH + 2 * Ny) * (TW + 2 * Nx) + 1) * 32 * 8;
-    return stage + (size_t)4 * (list_cap + 32) * 4;
+    return stage + (size_t)4 * (list_cap + 32 + 64) * 4;
+}
+
+static int gram_list_cap(int Nx, int Ny) {
+    const int Wn = (2 * Ny + 1) * (2 * Nx + 1);
+    return ((Wn + 15) / 16) * 16 + 16;
+}
+
+// largest TH x TW tile (TH <= 4, TW <= 8) whose stage keeps two workgroups
+// on a CU; returns the stage's LDS bytes
+static size_t gram_stage_tile(int Nx, int Ny, int& TH, int& TW) {
+    const size_t budget = 78 * 1024;
+    const int cap = gram_list_cap(Nx, Ny);
+    TH = 4;
+    TW = 8;
+    while (TH > 1 && gram_lds_bytes(Nx, Ny, TH, TW, cap) > budget) --TH;
+    while (TW > 1 && gram_lds_bytes(Nx, Ny, TH, TW, cap) > budget) TW >>= 1;
+    return gram_lds_bytes(Nx, Ny, TH, TW, cap);
+}
+
+int gram_stage_rows(int n, int Nx, int Ny) {
+    if (n > 32) return 1;
+    int TH, TW;
+    return gram_stage_tile(Nx, Ny, TH, TW) <= 160 * 1024 ? TH : 1;
 }
 
 // packed records followed by the per-date maxima (256-byte aligned)
@@ -555,16 +603,11 @@ int gram_mfma_launch(const LinkPlan& P, const uint32_t* pk, const uint32_t* wts,
     a.TW = 8;
     a.gamma = (float*)P.d_gamma;
     a.npix = P.d_npix;
-    const int Wn = (2 * P.Ny + 1) * (2 * P.Nx + 1);
-    a.list_cap = ((Wn + 15) / 16) * 16 + 16;
+    a.list_cap = gram_list_cap(P.Nx, P.Ny);
     const bool f16 = (P.precision != FRINGE_PREC_MIXED_BF16);
     if (P.n <= 32) {
-        // largest tile whose stage keeps two workgroups on a CU
-        const size_t budget = 78 * 1024;
-        int TH = 4, TW = 8;
-        while (TH > 1 && gram_lds_bytes(P.n, P.Nx, P.Ny, TH, TW, a.list_cap) > budget) --TH;
-        while (TW > 1 && gram_lds_bytes(P.n, P.Nx, P.Ny, TH, TW, a.list_cap) > budget) TW >>= 1;
-        const size_t bytes = gram_lds_bytes(P.n, P.Nx, P.Ny, TH, TW, a.list_cap);
+        int TH, TW;
+        const size_t bytes = gram_stage_tile(P.Nx, P.Ny, TH, TW);
         if (bytes <= 160 * 1024) {
             a.TH = TH;
             a.TW = TW;

@@ -115,7 +115,10 @@ __device__ inline bool abs_gamma_block(double (&M)[4][8], const cx<Tg>* gam, int
             double v = row == col ? 1.0 - shift : 0.0;
             if (act && row < n && col < n && row != col) {
                 const cx<Tg> g = gam[(size_t)row * n + col];
-                v = (double)sqrt(g.re * g.re + g.im * g.im);
+                if constexpr (sizeof(Tg) == 4)   // float Gamma: one v_sqrt_f32 (~1 ulp)
+                    v = (double)__builtin_amdgcn_sqrtf(g.re * g.re + g.im * g.im);
+                else
+                    v = sqrt(g.re * g.re + g.im * g.im);
                 fin = fin && isfinite(v);
             }
             M[i][j] = v;
@@ -191,6 +194,8 @@ __device__ inline void blocks_to_rows(Lds32<T>& L, const double (&M)[4][8], doub
 }
 
 // ---- Householder tridiagonalisation of the rows in h ---------------------
+// Step j of the reduction, for C0 - 1 <= j <= C0 + 2: columns below C0 hold
+// only finished entries and parked reflectors, so they are skipped.
 template <typename T, int C0>
 __device__ inline void tri_step(Lds32<T>& L, cx<T> (&h)[32], int j, int n, int r) {
     // row j was published at the end of the previous step (or before the first)
@@ -201,7 +206,11 @@ __device__ inline void tri_step(Lds32<T>& L, cx<T> (&h)[32], int j, int n, int r
     cx<T> tau = {T(0), T(0)}, scal = {T(0), T(0)};
     T beta = ar;
     if (xn2 != T(0) || ai != T(0)) {
-        const T nrm = sqrt(ar * ar + ai * ai + xn2);
+        T nrm;
+        if constexpr (sizeof(T) == 4)   // one v_sqrt_f32 (~1 ulp): the reflector stays unitary to rounding
+            nrm = __builtin_amdgcn_sqrtf(ar * ar + ai * ai + xn2);
+        else
+            nrm = sqrt(ar * ar + ai * ai + xn2);
         beta = (ar >= T(0)) ? -nrm : nrm;
         const T ib = fast_rcp(beta);
         tau = cx<T>{(beta - ar) * ib, -ai * ib};
@@ -235,7 +244,7 @@ __device__ inline void tri_step(Lds32<T>& L, cx<T> (&h)[32], int j, int n, int r
     // lane's entry of v_j there for the back-transformation (no later step
     // reads it: V and W vanish at column j from here on)
 #pragma unroll
-    for (int c = (C0 > 0 ? C0 - 1 : 0); c < (C0 + 8 < 32 ? C0 + 8 : 32); ++c)
+    for (int c = (C0 > 0 ? C0 - 1 : 0); c < (C0 + 3 < 32 ? C0 + 3 : 32); ++c)
         if (c == j && r > j + 1) h[c] = vr;
     // publish row j + 1 for the next step (everyone is past reading row j)
     if (r == j + 1) {
@@ -252,14 +261,18 @@ __device__ inline void tridiag32(Lds32<T>& L, cx<T> (&h)[32], int n, int r) {
         for (int c = 0; c < 32; ++c) L.R[c] = h[c];
     }
     wsync();
-    // columns below j+1 carry zero reflector entries: one loop per octet of
+    // columns below j+1 carry zero reflector entries: one loop per quad of
     // columns left to update (separate loops keep the register allocation of
     // each variant to itself)
     int j = 0;
-    for (; j < min(7, n - 1); ++j) tri_step<T, 0>(L, h, j, n, r);
-    for (; j < min(15, n - 1); ++j) tri_step<T, 8>(L, h, j, n, r);
-    for (; j < min(23, n - 1); ++j) tri_step<T, 16>(L, h, j, n, r);
-    for (; j < n - 1; ++j) tri_step<T, 24>(L, h, j, n, r);
+    for (; j < min(3, n - 1); ++j) tri_step<T, 0>(L, h, j, n, r);
+    for (; j < min(7, n - 1); ++j) tri_step<T, 4>(L, h, j, n, r);
+    for (; j < min(11, n - 1); ++j) tri_step<T, 8>(L, h, j, n, r);
+    for (; j < min(15, n - 1); ++j) tri_step<T, 12>(L, h, j, n, r);
+    for (; j < min(19, n - 1); ++j) tri_step<T, 16>(L, h, j, n, r);
+    for (; j < min(23, n - 1); ++j) tri_step<T, 20>(L, h, j, n, r);
+    for (; j < min(27, n - 1); ++j) tri_step<T, 24>(L, h, j, n, r);
+    for (; j < n - 1; ++j) tri_step<T, 28>(L, h, j, n, r);
     // last diagonal entry: row n - 1 is the one published last
     if (r == 0) L.D[n - 1] = L.R[n - 1].re;
     wsync();
@@ -295,8 +308,8 @@ __device__ inline T bisect32(Lds32<T>& L, int n, int k, int r, T& tnorm) {
     // 10 rounds reach FP64 resolution
     const int rounds = sizeof(T) == 4 ? 5 : 10;
     for (int it = 0; it < rounds; ++it) {
-        const T w = hi - lo;
-        const T sig = lo + (w * T(r + 1)) / T(33);
+        const T w33 = (hi - lo) * T(1.0 / 33.0);   // the same expression gives every section point
+        const T sig = lo + w33 * T(r + 1);
         T q = L.D[0] - sig;
         if (fabs(q) < pivmin) q = -pivmin;
         int cnt = q < T(0);
@@ -307,8 +320,8 @@ __device__ inline T bisect32(Lds32<T>& L, int n, int k, int r, T& tnorm) {
         }
         const uint64_t b = group_ballot<32>(cnt > k);
         const int f = b ? (int)__builtin_ctzll(b) : 32;
-        const T nhi = (f < 32) ? lo + (w * T(f + 1)) / T(33) : hi;
-        const T nlo = (f > 0) ? lo + (w * T(f)) / T(33) : lo;
+        const T nhi = (f < 32) ? lo + w33 * T(f + 1) : hi;
+        const T nlo = (f > 0) ? lo + w33 * T(f) : lo;
         lo = nlo;
         hi = nhi;
     }
@@ -413,20 +426,25 @@ __device__ inline bool hadamard_rows(cx<T> (&h)[32], const double (&m)[32], cons
                                      int r, bool act) {
     bool fin = true;
 #pragma unroll
-    for (int c0 = 0; c0 < 32; c0 += 8) {
-#pragma unroll
-        for (int c = c0; c < c0 + 8; ++c) {
-            cx<T> v = {T(0), T(0)};
-            if (r < n && c < n) {
-                const double mm = -m[c];   // m holds -|Gamma|^-1
-                cx<Tg> g = (c == r) ? cx<Tg>{Tg(1), Tg(0)} : cx<Tg>{Tg(0), Tg(0)};
-                if (act && c != r) g = grow[c];
+    for (int c = 0; c < 32; ++c) {
+        cx<T> v = {T(0), T(0)};
+        if (r < n && c < n) {
+            cx<Tg> g = (c == r) ? cx<Tg>{Tg(1), Tg(0)} : cx<Tg>{Tg(0), Tg(0)};
+            if (act && c != r) g = grow[c];
+            if constexpr (sizeof(T) == 4) {
+                // FP32 solver: the inverse entry rounded to float, one product
+                // each; an infinite or NaN part makes the sum non-finite
+                const float mm = (float)(-m[c]);   // m holds -|Gamma|^-1
+                v = cx<T>{mm * (float)g.re, mm * (float)g.im};
+                fin = fin && isfinite(v.re + v.im);
+            } else {
+                const double mm = -m[c];
                 const double hr = mm * (double)g.re, hi = mm * (double)g.im;
                 fin = fin && isfinite(hr) && isfinite(hi);
                 v = cx<T>{(T)hr, (T)hi};
             }
-            h[c] = v;
         }
+        h[c] = v;
     }
     return min32(fin ? 1.0f : 0.0f) > 0.5f;
 }

@@ -424,7 +424,7 @@ def main():
                            "TFLOP/s", g_ms, g_calls, "8*K_eff*N^2 flops per linked pixel (hi/lo passes not counted)")
     if "solve" in per:
         s_ms, s_calls = per["solve"]
-        rl["solve"] = _roof("solve32_kernel", "valu", solve_flops(n) * linked / (s_ms / 1e3) / 1e12, PEAK_F32_VALU,
+        rl["solve"] = _roof("solve_reg_kernel", "valu", solve_flops(n) * linked / (s_ms / 1e3) / 1e12, PEAK_F32_VALU,
                             "TFLOP/s", s_ms, s_calls,
                             "(19/3)N^3 + 16N^2 flops per linked pixel: |Gamma| inverse, Hermitian tridiagonal "
                             "reduction, back-transform, Hadamard product, temporal coherence")

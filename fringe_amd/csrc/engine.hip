@@ -168,8 +168,8 @@ int link_check(const fringe_link_tile* t) {
     if (link_uses_mfma(t->nbands, t->precision) &&
         ((long long)t->rows * t->cols + 1) * 2 * t->nbands >= (1ll << 32))
         return FRINGE_ERR_UNSUPPORTED;
-    // LDS footprint of the generic solver used above 32 dates (one wave per workgroup)
-    if (t->nbands > 32) {
+    // LDS footprint of the generic solver used above 64 dates (one wave per workgroup)
+    if (t->nbands > 64) {
         const size_t need = t->precision == FRINGE_PREC_FP64 ? solve_group_bytes<double>(t->nbands)
                                                              : solve_group_bytes<float>(t->nbands);
         if (need > (size_t)device_lds_limit()) return FRINGE_ERR_UNSUPPORTED;

@@ -193,13 +193,21 @@ static void dispatch_solve(const SolveArgs& a, hipStream_t s) {
     if (a.n <= 32) {
         const unsigned blocks = (unsigned)((a.slots + 1) / 2);
         if (a.module == FRINGE_MODULE_PHASE_LINK)
-            hipLaunchKernelGGL((solve32_kernel<T, Tg, MODE_EMI>), dim3(blocks), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_EMI, 32>), dim3(blocks), dim3(64), 0, s, a);
         else if (a.method == FRINGE_METHOD_MLE)
-            hipLaunchKernelGGL((solve32_kernel<T, Tg, MODE_MLE>), dim3(blocks), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_MLE, 32>), dim3(blocks), dim3(64), 0, s, a);
         else
-            hipLaunchKernelGGL((solve32_kernel<T, Tg, MODE_EVD>), dim3(blocks), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_EVD, 32>), dim3(blocks), dim3(64), 0, s, a);
     }
-    else if (a.n <= 64) launch_solve<T, Tg, 64, 1>(a, s);
+    else if (a.n <= 64) {
+        const unsigned blocks = (unsigned)a.slots;
+        if (a.module == FRINGE_MODULE_PHASE_LINK)
+            hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_EMI, 64>), dim3(blocks), dim3(64), 0, s, a);
+        else if (a.method == FRINGE_METHOD_MLE)
+            hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_MLE, 64>), dim3(blocks), dim3(64), 0, s, a);
+        else
+            hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_EVD, 64>), dim3(blocks), dim3(64), 0, s, a);
+    }
     else launch_solve<T, Tg, 64, 2>(a, s);
 }
 

@@ -1,7 +1,8 @@
-// solve32.hpp -- register-resident per-pixel estimator for n <= 32 dates.
+// solve32.hpp -- register-resident per-pixel estimator for n <= 64 dates.
 //
-// A wave holds two pixels; lane r of a 32-lane half owns row r of every
-// matrix of its pixel in registers.  LDS (about 3.5 KB per pixel) is used only
+// NR = 32 (n <= 32): a wave holds two pixels and lane r of a 32-lane half
+// owns row r of every matrix of its pixel in registers.  NR = 64
+// (32 < n <= 64): one pixel per wave, lane r owns row r.  LDS (about 3.5 KB per pixel) is used only
 // to publish one row or one vector to the other lanes, so many waves fit on
 // a CU and the arithmetic is not serialised behind LDS round trips.
 //
@@ -25,31 +26,37 @@
 
 namespace fringe {
 
-template <typename T> struct Lds32 {
-    cx<T> R[32];      // published row
-    cx<T> V[32];
-    cx<T> W[32];
-    cx<T> TAU[32];
+template <typename T, int NR> struct LdsR {
+    cx<T> R[NR];      // published row
+    cx<T> V[NR];
+    cx<T> W[NR];
+    cx<T> TAU[NR];
     cx<T> YREF;
-    T D[32], E[32], E2[32];
-    T B[32];          // inverse-iteration right-hand side
-    T FP[32], FQ[32], FS[32], FX[32];   // eliminated rows
-    double TR[32];    // sweep: published pivot row
+    T D[NR], E[NR], E2[NR];
+    T B[NR];          // inverse-iteration right-hand side
+    T FP[NR], FQ[NR], FS[NR], FX[NR];   // eliminated rows
+    double TR[NR];    // sweep: published pivot row
     double PV;        // sweep: pivot
-    double XT[4][32]; // sweep: one block column on its way to the row layout
-    cx<float> OUT[32];
+    double XT[NR / 8][NR];   // sweep: one block column on its way to the row layout
+    cx<float> OUT[NR];
 };
 
-__device__ inline void wsync() {
+// Blocks of the FP64 sweep: 8 columns wide, NR / 8 rows high; lane r holds
+// row block r / (NR / 8) and column block r % (NR / 8).
+template <int NR> struct Blk {
+    static constexpr int BC = 8, NCB = NR / 8, BR = NR / 8;
+};
+
+__device__ __forceinline__ void wsync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename T> __device__ inline T fast_rcp(T x) { return T(1) / x; }
-template <> __device__ inline float fast_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <typename T> __device__ __forceinline__ T fast_rcp(T x) { return T(1) / x; }
+template <> __device__ __forceinline__ float fast_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 // 1/x in double: hardware estimate refined by two Newton steps (~1 ulp).
-template <> __device__ inline double fast_rcp<double>(double x) {
+template <> __device__ __forceinline__ double fast_rcp<double>(double x) {
     double r = __builtin_amdgcn_rcp(x);
     r = fma(r, fma(-x, r, 1.0), r);
     r = fma(r, fma(-x, r, 1.0), r);
@@ -61,14 +68,14 @@ template <> __device__ inline double fast_rcp<double>(double x) {
 // combine 16 lanes through DPP; v_permlane16_swap then exchanges the two
 // 16-lane rows of the half.  Every lane ends with the same, bitwise
 // identical value (each stage adds two equal-rank partials commutatively).
-template <int CTRL> __device__ inline uint32_t dpp32(uint32_t v) {
+template <int CTRL> __device__ __forceinline__ uint32_t dpp32(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
 }
 struct OpAdd { template <typename T> __device__ static T f(T a, T b) { return a + b; } };
 struct OpMax { template <typename T> __device__ static T f(T a, T b) { return max(a, b); } };
 struct OpMin { template <typename T> __device__ static T f(T a, T b) { return min(a, b); } };
 
-template <typename Op> __device__ inline float red32(float x) {
+template <typename Op> __device__ __forceinline__ float red32(float x) {
     x = Op::f(x, __uint_as_float(dpp32<0xB1>(__float_as_uint(x))));
     x = Op::f(x, __uint_as_float(dpp32<0x4E>(__float_as_uint(x))));
     x = Op::f(x, __uint_as_float(dpp32<0x141>(__float_as_uint(x))));
@@ -76,12 +83,12 @@ template <typename Op> __device__ inline float red32(float x) {
     const auto s = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
     return Op::f(__uint_as_float(s[0]), __uint_as_float(s[1]));
 }
-template <int CTRL> __device__ inline double dpp64(double v) {
+template <int CTRL> __device__ __forceinline__ double dpp64(double v) {
     const uint64_t u = __builtin_bit_cast(uint64_t, v);
     const uint32_t lo = dpp32<CTRL>((uint32_t)u), hi = dpp32<CTRL>((uint32_t)(u >> 32));
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
-template <typename Op> __device__ inline double red32(double x) {
+template <typename Op> __device__ __forceinline__ double red32(double x) {
     x = Op::f(x, dpp64<0xB1>(x));
     x = Op::f(x, dpp64<0x4E>(x));
     x = Op::f(x, dpp64<0x141>(x));
@@ -93,22 +100,46 @@ template <typename Op> __device__ inline double red32(double x) {
     const double b = __builtin_bit_cast(double, ((uint64_t)hi[1] << 32) | lo[1]);
     return Op::f(a, b);
 }
-template <typename T> __device__ inline T sum32(T x) { return red32<OpAdd>(x); }
-template <typename T> __device__ inline T max32(T x) { return red32<OpMax>(x); }
-template <typename T> __device__ inline T min32(T x) { return red32<OpMin>(x); }
-template <typename T> __device__ inline cx<T> sum32(cx<T> x) { return cx<T>{sum32(x.re), sum32(x.im)}; }
+// over the NR lanes of a pixel: for NR = 64 the two half-wave results are
+// combined once more through v_permlane32_swap
+template <typename Op, int NR> __device__ __forceinline__ float redR(float x) {
+    x = red32<Op>(x);
+    if constexpr (NR == 64) {
+        const auto s = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+        x = Op::f(__uint_as_float(s[0]), __uint_as_float(s[1]));
+    }
+    return x;
+}
+template <typename Op, int NR> __device__ __forceinline__ double redR(double x) {
+    x = red32<Op>(x);
+    if constexpr (NR == 64) {
+        const uint64_t u = __builtin_bit_cast(uint64_t, x);
+        const auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)u, (uint32_t)u, false, false);
+        const auto hi = __builtin_amdgcn_permlane32_swap((uint32_t)(u >> 32), (uint32_t)(u >> 32), false, false);
+        const double a = __builtin_bit_cast(double, ((uint64_t)hi[0] << 32) | lo[0]);
+        const double b = __builtin_bit_cast(double, ((uint64_t)hi[1] << 32) | lo[1]);
+        x = Op::f(a, b);
+    }
+    return x;
+}
+template <int NR, typename T> __device__ __forceinline__ T sumR(T x) { return redR<OpAdd, NR>(x); }
+template <int NR, typename T> __device__ __forceinline__ T maxR(T x) { return redR<OpMax, NR>(x); }
+template <int NR, typename T> __device__ __forceinline__ T minR(T x) { return redR<OpMin, NR>(x); }
+template <int NR, typename T> __device__ __forceinline__ cx<T> sumR(cx<T> x) { return cx<T>{sumR<NR>(x.re), sumR<NR>(x.im)}; }
 
-// ---- FP64 inverse of |Gamma| - shift*I in 4 x 8 blocks --------------------
-// Lane (a, b) = (r / 4, r % 4) of the half holds M[i][j] = A(4a + i, 8b + j).
+// ---- FP64 inverse of |Gamma| - shift*I in BR x 8 blocks -------------------
+// Lane (a, b) = (r / NCB, r % NCB) holds M[i][j] = A(BR a + i, 8b + j)
+// (BR = 4, NCB = 4 for NR = 32; BR = 8, NCB = 8 for NR = 64).
 // Rows and columns >= n hold the identity (never swept, never mixed in).
-template <typename Tg>
-__device__ inline bool abs_gamma_block(double (&M)[4][8], const cx<Tg>* gam, int n, int r, bool act,
+template <int NR, typename Tg>
+__device__ __forceinline__ bool abs_gamma_block(double (&M)[Blk<NR>::BR][8], const cx<Tg>* gam, int n, int r, bool act,
                                        double shift) {
-    const int a = r >> 2, b = r & 3;
+    constexpr int BR = Blk<NR>::BR, NCB = Blk<NR>::NCB;
+    const int a = r / NCB, b = r % NCB;
     bool fin = true;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int row = 4 * a + i;
+    for (int i = 0; i < BR; ++i) {
+        const int row = BR * a + i;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int col = 8 * b + j;
@@ -135,17 +166,18 @@ __device__ inline bool abs_gamma_block(double (&M)[4][8], const cx<Tg>* gam, int
 // A_rc - (P_r / p) P_c; (k, k) is set to -1/p.  After pivots 0..n-1 the
 // blocks hold -A^-1.  Returns whether every pivot was positive (A is
 // positive definite).
-template <typename T>
-__device__ inline bool sweep_block(Lds32<T>& L, double (&M)[4][8], int n, int r) {
-    const int a = r >> 2, b = r & 3;
+template <typename T, int NR>
+__device__ __forceinline__ bool sweep_block(LdsR<T, NR>& L, double (&M)[Blk<NR>::BR][8], int n, int r) {
+    constexpr int BR = Blk<NR>::BR, NCB = Blk<NR>::NCB;
+    const int a = r / NCB, b = r % NCB;
     bool ok = true;
     for (int q = 0; 8 * q < n; ++q) {
 #pragma unroll
         for (int kj = 0; kj < 8; ++kj) {
             const int k = 8 * q + kj;
             if (k >= n) break;
-            const int ki = kj & 3;   // row k is row ki of the blocks with a == k / 4
-            if (a == (k >> 2)) {
+            const int ki = kj % BR;   // row k is row ki of the blocks with a == k / BR
+            if (a == k / BR) {
                 double pr[8];
 #pragma unroll
                 for (int j = 0; j < 8; ++j) pr[j] = M[ki][j];
@@ -160,16 +192,16 @@ __device__ inline bool sweep_block(Lds32<T>& L, double (&M)[4][8], int n, int r)
             const double p = L.PV;
             ok = ok && (p > 0.0);
             const double piv = fast_rcp(p);
-            double pc[8], g[4];
+            double pc[8], g[BR];
 #pragma unroll
             for (int j = 0; j < 8; ++j) pc[j] = L.TR[8 * b + j];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) g[i] = L.TR[4 * a + i] * piv;
+            for (int i = 0; i < BR; ++i) g[i] = L.TR[BR * a + i] * piv;
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < BR; ++i)
 #pragma unroll
                 for (int j = 0; j < 8; ++j) M[i][j] = fma(-g[i], pc[j], M[i][j]);
-            if (a == (k >> 2) && b == q) M[ki][kj] = -piv;
+            if (a == k / BR && b == q) M[ki][kj] = -piv;
             wsync();
         }
     }
@@ -177,31 +209,32 @@ __device__ inline bool sweep_block(Lds32<T>& L, double (&M)[4][8], int n, int r)
 }
 
 // Blocks -> rows: m[c] = M(r, c).  Block column j (columns j, 8 + j, 16 + j,
-// 24 + j) passes through LDS at a time, so the registers it frees are the
-// ones the row layout fills.
-template <typename T>
-__device__ inline void blocks_to_rows(Lds32<T>& L, const double (&M)[4][8], double (&m)[32], int r) {
-    const int a = r >> 2, b = r & 3;
+// ...) passes through LDS at a time, so the registers it frees are the ones
+// the row layout fills.
+template <typename T, int NR>
+__device__ __forceinline__ void blocks_to_rows(LdsR<T, NR>& L, const double (&M)[Blk<NR>::BR][8], double (&m)[NR], int r) {
+    constexpr int BR = Blk<NR>::BR, NCB = Blk<NR>::NCB;
+    const int a = r / NCB, b = r % NCB;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) L.XT[b][4 * a + i] = M[i][j];
+        for (int i = 0; i < BR; ++i) L.XT[b][BR * a + i] = M[i][j];
         wsync();
 #pragma unroll
-        for (int k = 0; k < 4; ++k) m[8 * k + j] = L.XT[k][r];
+        for (int k = 0; k < NCB; ++k) m[8 * k + j] = L.XT[k][r];
         wsync();
     }
 }
 
 // ---- Householder tridiagonalisation of the rows in h ---------------------
-// Step j of the reduction, for C0 - 1 <= j <= C0 + 2: columns below C0 hold
-// only finished entries and parked reflectors, so they are skipped.
-template <typename T, int C0>
-__device__ inline void tri_step(Lds32<T>& L, cx<T> (&h)[32], int j, int n, int r) {
+// Step j of the reduction, for C0 - 1 <= j <= C0 + STEP - 2: columns below
+// C0 hold only finished entries and parked reflectors, so they are skipped.
+template <typename T, int NR, int C0, int STEP>
+__device__ __forceinline__ void tri_step(LdsR<T, NR>& L, cx<T> (&h)[NR], int j, int n, int r) {
     // row j was published at the end of the previous step (or before the first)
     const cx<T> alpha = cconj(L.R[j + 1]);
     const cx<T> xr = (r > j + 1 && r < n) ? cconj(L.R[r]) : cx<T>{T(0), T(0)};
-    const T xn2 = sum32(cabs2(xr));
+    const T xn2 = sumR<NR>(cabs2(xr));
     const T ar = alpha.re, ai = alpha.im;
     cx<T> tau = {T(0), T(0)}, scal = {T(0), T(0)};
     T beta = ar;
@@ -228,15 +261,15 @@ __device__ inline void tri_step(Lds32<T>& L, cx<T> (&h)[32], int j, int n, int r
     wsync();
     cx<T> p = {T(0), T(0)};
 #pragma unroll
-    for (int c = C0; c < 32; ++c) cfma(p, h[c], L.V[c]);
+    for (int c = C0; c < NR; ++c) cfma(p, h[c], L.V[c]);
     p = cmul(tau, p);
-    const cx<T> s = sum32(cmulc(vr, p));
+    const cx<T> s = sumR<NR>(cmulc(vr, p));
     const cx<T> a2 = cscale(cmul(tau, s), T(-0.5));
     const cx<T> wr = (r > j && r < n) ? cadd(p, cmul(a2, vr)) : cx<T>{T(0), T(0)};
     L.W[r] = wr;
     wsync();
 #pragma unroll
-    for (int c = C0; c < 32; ++c) {
+    for (int c = C0; c < NR; ++c) {
         cfms_c(h[c], vr, L.W[c]);
         cfms_c(h[c], wr, L.V[c]);
     }
@@ -244,43 +277,54 @@ __device__ inline void tri_step(Lds32<T>& L, cx<T> (&h)[32], int j, int n, int r
     // lane's entry of v_j there for the back-transformation (no later step
     // reads it: V and W vanish at column j from here on)
 #pragma unroll
-    for (int c = (C0 > 0 ? C0 - 1 : 0); c < (C0 + 3 < 32 ? C0 + 3 : 32); ++c)
+    for (int c = (C0 > 0 ? C0 - 1 : 0); c < (C0 + STEP - 1 < NR ? C0 + STEP - 1 : NR); ++c)
         if (c == j && r > j + 1) h[c] = vr;
     // publish row j + 1 for the next step (everyone is past reading row j)
     if (r == j + 1) {
 #pragma unroll
-        for (int c = 0; c < 32; ++c) L.R[c] = h[c];
+        for (int c = 0; c < NR; ++c) L.R[c] = h[c];
     }
     wsync();
 }
 
-template <typename T>
-__device__ inline void tridiag32(Lds32<T>& L, cx<T> (&h)[32], int n, int r) {
+template <typename T, int NR>
+__device__ __forceinline__ void tridiag(LdsR<T, NR>& L, cx<T> (&h)[NR], int n, int r) {
     if (r == 0) {
 #pragma unroll
-        for (int c = 0; c < 32; ++c) L.R[c] = h[c];
+        for (int c = 0; c < NR; ++c) L.R[c] = h[c];
     }
     wsync();
-    // columns below j+1 carry zero reflector entries: one loop per quad of
-    // columns left to update (separate loops keep the register allocation of
-    // each variant to itself)
+    // columns below j+1 carry zero reflector entries: one loop per quad (NR =
+    // 32) or octet (NR = 64) of columns left to update (separate loops keep
+    // the register allocation of each variant to itself)
     int j = 0;
-    for (; j < min(3, n - 1); ++j) tri_step<T, 0>(L, h, j, n, r);
-    for (; j < min(7, n - 1); ++j) tri_step<T, 4>(L, h, j, n, r);
-    for (; j < min(11, n - 1); ++j) tri_step<T, 8>(L, h, j, n, r);
-    for (; j < min(15, n - 1); ++j) tri_step<T, 12>(L, h, j, n, r);
-    for (; j < min(19, n - 1); ++j) tri_step<T, 16>(L, h, j, n, r);
-    for (; j < min(23, n - 1); ++j) tri_step<T, 20>(L, h, j, n, r);
-    for (; j < min(27, n - 1); ++j) tri_step<T, 24>(L, h, j, n, r);
-    for (; j < n - 1; ++j) tri_step<T, 28>(L, h, j, n, r);
+    if constexpr (NR == 32) {
+        for (; j < min(3, n - 1); ++j) tri_step<T, NR, 0, 4>(L, h, j, n, r);
+        for (; j < min(7, n - 1); ++j) tri_step<T, NR, 4, 4>(L, h, j, n, r);
+        for (; j < min(11, n - 1); ++j) tri_step<T, NR, 8, 4>(L, h, j, n, r);
+        for (; j < min(15, n - 1); ++j) tri_step<T, NR, 12, 4>(L, h, j, n, r);
+        for (; j < min(19, n - 1); ++j) tri_step<T, NR, 16, 4>(L, h, j, n, r);
+        for (; j < min(23, n - 1); ++j) tri_step<T, NR, 20, 4>(L, h, j, n, r);
+        for (; j < min(27, n - 1); ++j) tri_step<T, NR, 24, 4>(L, h, j, n, r);
+        for (; j < n - 1; ++j) tri_step<T, NR, 28, 4>(L, h, j, n, r);
+    } else {
+        for (; j < min(7, n - 1); ++j) tri_step<T, NR, 0, 8>(L, h, j, n, r);
+        for (; j < min(15, n - 1); ++j) tri_step<T, NR, 8, 8>(L, h, j, n, r);
+        for (; j < min(23, n - 1); ++j) tri_step<T, NR, 16, 8>(L, h, j, n, r);
+        for (; j < min(31, n - 1); ++j) tri_step<T, NR, 24, 8>(L, h, j, n, r);
+        for (; j < min(39, n - 1); ++j) tri_step<T, NR, 32, 8>(L, h, j, n, r);
+        for (; j < min(47, n - 1); ++j) tri_step<T, NR, 40, 8>(L, h, j, n, r);
+        for (; j < min(55, n - 1); ++j) tri_step<T, NR, 48, 8>(L, h, j, n, r);
+        for (; j < n - 1; ++j) tri_step<T, NR, 56, 8>(L, h, j, n, r);
+    }
     // last diagonal entry: row n - 1 is the one published last
     if (r == 0) L.D[n - 1] = L.R[n - 1].re;
     wsync();
 }
 
 // ---- eigenvalue k (0-based ascending) of tridiag(D, E) --------------------
-template <typename T>
-__device__ inline T bisect32(Lds32<T>& L, int n, int k, int r, T& tnorm) {
+template <typename T, int NR>
+__device__ __forceinline__ T bisect(LdsR<T, NR>& L, int n, int k, int r, T& tnorm) {
     T lo = T(3.0e38), hi = T(-3.0e38), em2 = T(0), tn = T(0);
     if (r < n) {
         const T a = (r > 0) ? fabs(L.E[r - 1]) : T(0);
@@ -293,22 +337,22 @@ __device__ inline T bisect32(Lds32<T>& L, int n, int k, int r, T& tnorm) {
             em2 = L.E2[r];
         }
     }
-    lo = min32(lo);
-    hi = max32(hi);
-    em2 = max32(em2);
-    tn = max32(tn);
+    lo = minR<NR>(lo);
+    hi = maxR<NR>(hi);
+    em2 = maxR<NR>(em2);
+    tn = maxR<NR>(tn);
     wsync();
     tnorm = tn;
     const T pivmin = Lim<T>::tiny() * max(T(1), em2);
     const T pad = T(4) * Lim<T>::eps() * tn + T(2) * pivmin;
     lo -= pad;
     hi += pad;
-    // 33-section shrinks the Gershgorin interval 33-fold per round: 5 rounds
-    // reach ~1e-7 |T| (enough for the inverse-iteration shift in FP32),
-    // 10 rounds reach FP64 resolution
-    const int rounds = sizeof(T) == 4 ? 5 : 10;
+    // (NR+1)-section shrinks the Gershgorin interval (NR+1)-fold per round:
+    // 5 rounds of 33 or 4 of 65 reach ~1e-7 |T| (enough for the
+    // inverse-iteration shift in FP32), 10 of 33 or 9 of 65 FP64 resolution
+    const int rounds = sizeof(T) == 4 ? (NR == 32 ? 5 : 4) : (NR == 32 ? 10 : 9);
     for (int it = 0; it < rounds; ++it) {
-        const T w33 = (hi - lo) * T(1.0 / 33.0);   // the same expression gives every section point
+        const T w33 = (hi - lo) * T(1.0 / (NR + 1));   // the same expression gives every section point
         const T sig = lo + w33 * T(r + 1);
         T q = L.D[0] - sig;
         if (fabs(q) < pivmin) q = -pivmin;
@@ -318,9 +362,9 @@ __device__ inline T bisect32(Lds32<T>& L, int n, int k, int r, T& tnorm) {
             if (fabs(q) < pivmin) q = -pivmin;
             cnt += q < T(0);
         }
-        const uint64_t b = group_ballot<32>(cnt > k);
-        const int f = b ? (int)__builtin_ctzll(b) : 32;
-        const T nhi = (f < 32) ? lo + w33 * T(f + 1) : hi;
+        const uint64_t b = group_ballot<NR>(cnt > k);
+        const int f = b ? (int)__builtin_ctzll(b) : NR;
+        const T nhi = (f < NR) ? lo + w33 * T(f + 1) : hi;
         const T nlo = (f > 0) ? lo + w33 * T(f) : lo;
         lo = nlo;
         hi = nhi;
@@ -332,8 +376,8 @@ __device__ inline T bisect32(Lds32<T>& L, int n, int k, int r, T& tnorm) {
 // Every lane of the half runs the same elimination chain on broadcast
 // operands; lane 0 records the eliminated rows for the back substitution,
 // and lane i keeps component i of the result.
-template <typename T>
-__device__ inline T tri_eigvec32(Lds32<T>& L, int n, int r, T lam, T tnorm) {
+template <typename T, int NR>
+__device__ __forceinline__ T tri_eigvec(LdsR<T, NR>& L, int n, int r, T lam, T tnorm) {
     const T pert = max(Lim<T>::eps() * tnorm, Lim<T>::tiny());
     L.B[r] = T(0.5) + T((r * 37 + 11) & 63) * T(1.0 / 64.0);
     wsync();
@@ -380,7 +424,7 @@ __device__ inline T tri_eigvec32(Lds32<T>& L, int n, int r, T lam, T tnorm) {
             x2 = x1;
             x1 = xi;
         }
-        T mx = max32((r < n) ? fabs(xme) : T(0));
+        T mx = maxR<NR>((r < n) ? fabs(xme) : T(0));
         const T sc = (mx > T(0) && isfinite(mx)) ? T(1) / mx : T(1);
         xme *= sc;
         wsync();
@@ -392,23 +436,23 @@ __device__ inline T tri_eigvec32(Lds32<T>& L, int n, int r, T lam, T tnorm) {
 
 // y <- Q y with Q = H_0 ... H_{n-2}; entry r > j+1 of v_j is h[j] of lane r
 // (unrolled over j so the register index is static).
-template <typename T>
-__device__ inline cx<T> back_transform32(Lds32<T>& L, const cx<T> (&h)[32], int n, int r, cx<T> y) {
+template <typename T, int NR>
+__device__ __forceinline__ cx<T> back_transform(LdsR<T, NR>& L, const cx<T> (&h)[NR], int n, int r, cx<T> y) {
 #pragma unroll
-    for (int j = 30; j >= 0; --j) {
+    for (int j = NR - 2; j >= 0; --j) {
         if (j > n - 2) continue;   // uniform
         const cx<T> v = (r == j + 1) ? cx<T>{T(1), T(0)} : (r > j + 1 && r < n) ? h[j] : cx<T>{T(0), T(0)};
-        const cx<T> s = sum32(cmulc(y, v));
+        const cx<T> s = sumR<NR>(cmulc(y, v));
         y = csub(y, cmul(v, cmul(L.TAU[j], s)));
     }
     return y;
 }
 
-template <typename T, typename Tg>
-__device__ inline bool load_rows(cx<T> (&h)[32], const cx<Tg>* grow, int n, int r, bool act, int band) {
+template <typename T, typename Tg, int NR>
+__device__ __forceinline__ bool load_rows(cx<T> (&h)[NR], const cx<Tg>* grow, int n, int r, bool act, int band) {
     bool fin = true;
 #pragma unroll
-    for (int c = 0; c < 32; ++c) {
+    for (int c = 0; c < NR; ++c) {
         cx<Tg> v = {Tg(0), Tg(0)};
         if (r < n && c < n) {
             if (act) v = grow[c];
@@ -418,15 +462,15 @@ __device__ inline bool load_rows(cx<T> (&h)[32], const cx<Tg>* grow, int n, int 
         fin = fin && isfinite(v.re) && isfinite(v.im);
         h[c] = cx<T>{(T)v.re, (T)v.im};
     }
-    return min32(fin ? 1.0f : 0.0f) > 0.5f;
+    return minR<NR>(fin ? 1.0f : 0.0f) > 0.5f;
 }
 
-template <typename T, typename Tg>
-__device__ inline bool hadamard_rows(cx<T> (&h)[32], const double (&m)[32], const cx<Tg>* grow, int n,
+template <typename T, typename Tg, int NR>
+__device__ __forceinline__ bool hadamard_rows(cx<T> (&h)[NR], const double (&m)[NR], const cx<Tg>* grow, int n,
                                      int r, bool act) {
     bool fin = true;
 #pragma unroll
-    for (int c = 0; c < 32; ++c) {
+    for (int c = 0; c < NR; ++c) {
         cx<T> v = {T(0), T(0)};
         if (r < n && c < n) {
             cx<Tg> g = (c == r) ? cx<Tg>{Tg(1), Tg(0)} : cx<Tg>{Tg(0), Tg(0)};
@@ -446,7 +490,7 @@ __device__ inline bool hadamard_rows(cx<T> (&h)[32], const double (&m)[32], cons
         }
         h[c] = v;
     }
-    return min32(fin ? 1.0f : 0.0f) > 0.5f;
+    return minR<NR>(fin ? 1.0f : 0.0f) > 0.5f;
 }
 
 enum { MODE_EMI = 0, MODE_EVD = 1, MODE_MLE = 2 };
@@ -454,12 +498,16 @@ enum { MODE_EMI = 0, MODE_EVD = 1, MODE_MLE = 2 };
 // MODE_EMI: phase_link (EMI with EVD fallback, phase_link.cpp:540-618)
 // MODE_EVD: evd EVD / STBAS (evd.cpp:689-732)
 // MODE_MLE: evd MLE with its failure codes (evd.cpp:584-688)
-template <typename T, typename Tg, int MODE>
-__global__ __launch_bounds__(64) void solve32_kernel(SolveArgs a) {
-    __shared__ Lds32<T> sm[2];
-    const int grp = threadIdx.x >> 5, r = threadIdx.x & 31;
-    Lds32<T>& L = sm[grp];
-    const int slot = blockIdx.x * 2 + grp;
+// NR = 32: two pixels per wave (n <= 32); NR = 64: one pixel per wave
+// (32 < n <= 64).  The body is the kernel itself: handing the LDS array to a
+// shared device function instead costs the NR = 32 FP32 kernel 56 VGPRs and
+// half its occupancy.
+template <typename T, typename Tg, int MODE, int NR>
+__global__ __launch_bounds__(64) void solve_reg_kernel(SolveArgs a) {
+    __shared__ LdsR<T, NR> sm[64 / NR];
+    const int grp = threadIdx.x / NR, r = threadIdx.x % NR;
+    LdsR<T, NR>& L = sm[grp];
+    const int slot = blockIdx.x * (64 / NR) + grp;
     const int n = a.n;
     const bool inslot = slot < a.slots;
     const int np = inslot ? a.npix[slot] : -1;
@@ -470,58 +518,58 @@ __global__ __launch_bounds__(64) void solve32_kernel(SolveArgs a) {
 
     float code = 0.0f;
     bool failed = false;
-    cx<T> h[32];
+    cx<T> h[NR];
     cx<T> y = {T(0), T(0)};
     T tn;
 
     if constexpr (MODE == MODE_MLE) {
-        const bool fin = load_rows<T, Tg>(h, grow, n, r, act, -1);
-        tridiag32<T>(L, h, n, r);
-        const T lmin = bisect32<T>(L, n, 0, r, tn);
+        const bool fin = load_rows<T, Tg, NR>(h, grow, n, r, act, -1);
+        tridiag<T, NR>(L, h, n, r);
+        const T lmin = bisect<T, NR>(L, n, 0, r, tn);
         if (!fin) { code = -1.0f; failed = true; }
         else if (!(lmin >= T(1.0e-6))) { code = -2.0f; failed = true; }
-        double M[4][8], m[32];
-        abs_gamma_block<Tg>(M, gam, n, r, act, 1.0e-6);
-        const bool pd_shift = sweep_block<T>(L, M, n, r);
+        double M[Blk<NR>::BR][8], m[NR];
+        abs_gamma_block<NR, Tg>(M, gam, n, r, act, 1.0e-6);
+        const bool pd_shift = sweep_block<T, NR>(L, M, n, r);
         if (!failed && !pd_shift) { code = -4.0f; failed = true; }
-        abs_gamma_block<Tg>(M, gam, n, r, act, 0.0);
-        const bool pd = sweep_block<T>(L, M, n, r);
+        abs_gamma_block<NR, Tg>(M, gam, n, r, act, 0.0);
+        const bool pd = sweep_block<T, NR>(L, M, n, r);
         if (!failed && !pd) { code = -5.0f; failed = true; }
-        blocks_to_rows<T>(L, M, m, r);
-        const bool hfin = hadamard_rows<T, Tg>(h, m, grow, n, r, act);
-        tridiag32<T>(L, h, n, r);
-        const T lam = bisect32<T>(L, n, 0, r, tn);
-        const T yr = tri_eigvec32<T>(L, n, r, lam, tn);
-        y = back_transform32<T>(L, h, n, r, cx<T>{yr, T(0)});
+        blocks_to_rows<T, NR>(L, M, m, r);
+        const bool hfin = hadamard_rows<T, Tg, NR>(h, m, grow, n, r, act);
+        tridiag<T, NR>(L, h, n, r);
+        const T lam = bisect<T, NR>(L, n, 0, r, tn);
+        const T yr = tri_eigvec<T, NR>(L, n, r, lam, tn);
+        y = back_transform<T, NR>(L, h, n, r, cx<T>{yr, T(0)});
         if (!failed && !hfin) { code = -6.0f; failed = true; }
         if (!failed && !(lam >= T(1.0e-6))) { code = -7.0f; failed = true; }
     } else if constexpr (MODE == MODE_EVD) {
         const int band = (a.method == FRINGE_METHOD_STBAS) ? a.bw : -1;
-        const bool fin = load_rows<T, Tg>(h, grow, n, r, act, band);
-        tridiag32<T>(L, h, n, r);
-        const T lam = bisect32<T>(L, n, n - 1, r, tn);
-        const T yr = tri_eigvec32<T>(L, n, r, lam, tn);
-        y = back_transform32<T>(L, h, n, r, cx<T>{yr, T(0)});
+        const bool fin = load_rows<T, Tg, NR>(h, grow, n, r, act, band);
+        tridiag<T, NR>(L, h, n, r);
+        const T lam = bisect<T, NR>(L, n, n - 1, r, tn);
+        const T yr = tri_eigvec<T, NR>(L, n, r, lam, tn);
+        y = back_transform<T, NR>(L, h, n, r, cx<T>{yr, T(0)});
         if (!fin) { code = -6.0f; failed = true; }
         else if (!(lam >= T(1.0e-6))) { code = -7.0f; failed = true; }
     } else {
-        double M[4][8], m[32];
-        bool fin = abs_gamma_block<Tg>(M, gam, n, r, act, 0.0);
-        fin = min32(fin ? 1.0f : 0.0f) > 0.5f;
-        const bool pd = sweep_block<T>(L, M, n, r);
-        blocks_to_rows<T>(L, M, m, r);
-        const bool hfin = hadamard_rows<T, Tg>(h, m, grow, n, r, act);
+        double M[Blk<NR>::BR][8], m[NR];
+        bool fin = abs_gamma_block<NR, Tg>(M, gam, n, r, act, 0.0);
+        fin = minR<NR>(fin ? 1.0f : 0.0f) > 0.5f;
+        const bool pd = sweep_block<T, NR>(L, M, n, r);
+        blocks_to_rows<T, NR>(L, M, m, r);
+        const bool hfin = hadamard_rows<T, Tg, NR>(h, m, grow, n, r, act);
         const bool run_evd = !pd || !hfin;
-        tridiag32<T>(L, h, n, r);
-        const T lam = bisect32<T>(L, n, 0, r, tn);
-        const T yr = tri_eigvec32<T>(L, n, r, lam, tn);
-        y = back_transform32<T>(L, h, n, r, cx<T>{yr, T(0)});
+        tridiag<T, NR>(L, h, n, r);
+        const T lam = bisect<T, NR>(L, n, 0, r, tn);
+        const T yr = tri_eigvec<T, NR>(L, n, r, lam, tn);
+        y = back_transform<T, NR>(L, h, n, r, cx<T>{yr, T(0)});
         if (__any(act && fin && run_evd)) {
-            load_rows<T, Tg>(h, grow, n, r, act, -1);
-            tridiag32<T>(L, h, n, r);
-            const T le = bisect32<T>(L, n, n - 1, r, tn);
-            const T ye = tri_eigvec32<T>(L, n, r, le, tn);
-            const cx<T> yv = back_transform32<T>(L, h, n, r, cx<T>{ye, T(0)});
+            load_rows<T, Tg, NR>(h, grow, n, r, act, -1);
+            tridiag<T, NR>(L, h, n, r);
+            const T le = bisect<T, NR>(L, n, n - 1, r, tn);
+            const T ye = tri_eigvec<T, NR>(L, n, r, le, tn);
+            const cx<T> yv = back_transform<T, NR>(L, h, n, r, cx<T>{ye, T(0)});
             if (run_evd) y = yv;
         }
         if (!fin) { code = -1.0f; failed = true; }
@@ -548,7 +596,7 @@ __global__ __launch_bounds__(64) void solve32_kernel(SolveArgs a) {
         cs.re = (double)(z.x * of.re + z.y * of.im);
         cs.im = (double)(z.y * of.re - z.x * of.im);
     }
-    cs = sum32(cs);
+    cs = sumR<NR>(cs);
     // ---- temporal coherence (evd.cpp:770-786) ----
     // sum over pairs i < j (j <= i + bw for STBAS) of
     //   exp(i(arg Gamma_ij - arg out_i + arg out_j)).
@@ -565,16 +613,16 @@ __global__ __launch_bounds__(64) void solve32_kernel(SolveArgs a) {
         const int dmax = ((n & 1) == 0 && r < n / 2) ? half + 1 : half;
         // all gathers first, then the sums; jd[d-1] = (r + d) mod n, stepped
         // so that it stays in [0, n) for every n >= 1
-        int jd[16];
-        cx<Tg> gv[16];
+        int jd[NR / 2];
+        cx<Tg> gv[NR / 2];
 #pragma unroll
-        for (int d = 1; d <= 16; ++d) {
+        for (int d = 1; d <= NR / 2; ++d) {
             const int prev = d == 1 ? r : jd[d - 2];
             jd[d - 1] = prev + 1 == n ? 0 : prev + 1;
             gv[d - 1] = grow[jd[d - 1]];
         }
 #pragma unroll
-        for (int d = 1; d <= 16; ++d) {
+        for (int d = 1; d <= NR / 2; ++d) {
             const int j = jd[d - 1];
             const int lo = min(r, j), hi = max(r, j);
             const bool use = d <= dmax && !(stbas && hi - lo > a.bw);
@@ -585,7 +633,7 @@ __global__ __launch_bounds__(64) void solve32_kernel(SolveArgs a) {
             if (use) tc = cadd(tc, t);
         }
     }
-    tc = sum32(tc);
+    tc = sumR<NR>(tc);
 
     if (!inslot) return;
     if (r < n) a.phase[(long long)r * a.npx + pix] = write_ok ? make_float2(of.re, of.im) : make_float2(0.0f, 0.0f);

@@ -78,7 +78,7 @@ def lib():
         L.oracle_ad2_sigma.argtypes = [ctypes.c_int, ctypes.c_int]
         L.oracle_ad2_sigma.restype = ctypes.c_double
         L.oracle_link.argtypes = [_f32p, _u32p] + [ctypes.c_int] * 12 + [
-            _f32p, _f32p, _f32p, ctypes.c_void_p, ctypes.c_int]
+            _f32p, _f32p, _f32p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         L.oracle_eigen_index.argtypes = [_f64p, ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]
         L.oracle_pd_inverse.argtypes = [_f64p, ctypes.c_int]
@@ -195,23 +195,28 @@ def link(slc, wts, Nx, Ny, module="phase_link", method="MLE", min_neighbors=2,
     """Per-pixel phase linking with reference semantics.
 
     Returns dict(phase=[nbands,rows,cols] c64, tcorr=[rows,cols] f32,
-    comp=[rows,cols] c64, status=int[, gap=[rows,cols] f64])."""
+    comp=[rows,cols] c64, status=int[, gap=[rows,cols] f64, vmin=[rows,cols] f64]).
+    gap: eigen-gap of the selected eigenvalue over the spectral spread;
+    vmin: sqrt(n) times the smallest modulus of the selected unit eigenvector."""
     nb, rows, cols = slc.shape
     r0, r1 = (0, rows) if rows_range is None else rows_range
     phase = np.zeros((nb, rows, cols), np.complex64)
     tcorr = np.zeros((rows, cols), np.float32)
     comp = np.zeros((rows, cols), np.complex64)
     gap = np.full((rows, cols), -1.0) if want_gap else None
+    vmin = np.full((rows, cols), -1.0) if want_gap else None
     mod = MOD_EVD if module == "evd" else MOD_PHASE_LINK
     st = lib().oracle_link(_slc_f32(slc), np.ascontiguousarray(wts, np.uint32).reshape(-1),
                            rows, cols, nb, Nx, Ny, mod, METHODS[method], int(min_neighbors),
                            int(mini_stack_count), int(bandwidth), r0, r1,
                            phase.view(np.float32).reshape(-1), tcorr.reshape(-1),
                            comp.view(np.float32).reshape(-1),
-                           None if gap is None else gap.ctypes.data, int(threads))
+                           None if gap is None else gap.ctypes.data,
+                           None if vmin is None else vmin.ctypes.data, int(threads))
     out = dict(phase=phase, tcorr=tcorr, comp=comp, status=st)
     if want_gap:
         out["gap"] = gap
+        out["vmin"] = vmin
     return out
 
 

@@ -447,6 +447,9 @@ extern "C" int oracle_despeck(const float* b1, const float* b2, const uint32_t* 
 // caller zero-fills them.  gap (optional, [rows][cols]) receives the distance
 // from the selected eigenvalue to its nearest neighbour divided by the
 // spread of the spectrum (diagnostic for near-degenerate pixels), or -1.
+// vmin (optional, [rows][cols]) receives sqrt(n) min_t |v_t| of the selected
+// unit eigenvector: the smallest component relative to the RMS one, whose
+// phase an eigenvector error of absolute size d fixes only to ~d / |v_t|.
 // Returns 0, or 101 for an invalid STBAS bandwidth (evd.cpp:74-91).
 // --------------------------------------------------------------------------
 enum { MOD_EVD = 0, MOD_PHASE_LINK = 1 };
@@ -456,7 +459,7 @@ extern "C" int oracle_link(const float* slc, const uint32_t* wts, int rows, int 
                            int nbands, int Nx, int Ny, int module, int method,
                            int minNeighbors, int miniStackCount, int bandWidth, int row0,
                            int row1, float* phase, float* tcorr, float* comp, double* gap,
-                           int nthreads) {
+                           double* vmin, int nthreads) {
     if (oracle_lapack_load(nullptr) != 0) return -100;
     const int n = nbands;
     const bool isstbas = (method == METH_STBAS);
@@ -655,6 +658,11 @@ extern "C" int oracle_link(const float* slc, const uint32_t* wts, int rows, int 
                 phase[2 * ((size_t)t * npx + pp) + 1] = out[t].imag();
             }
 
+            if (vmin) {
+                double mn = 1e300;
+                for (int t = 0; t < n; ++t) mn = std::min(mn, std::abs(ew.z[t]));
+                vmin[pp] = mn * std::sqrt((double)n);
+            }
             if (gap && sel_matrix) {
                 // Diagnostic: neighbour gap of the selected eigenvalue over the
                 // spectral spread, from the full FP64 spectrum.

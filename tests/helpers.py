@@ -4,6 +4,11 @@ import numpy as np
 PHASE_TOL = 1.0e-3   # rad, wrapped phase vs the FP64 oracle
 TCORR_TOL = 1.0e-4   # temporal coherence vs the FP64 oracle
 GAP_MIN = 1.0e-3     # relative eigen-gap below which a pixel's phase is ill-posed
+# Smallest eigenvector component, relative to the RMS one, whose phase an FP32
+# solver fixes to 1e-3 rad: its absolute eigenvector error is ~1e-7, so a
+# component of size |v_t| carries a phase error of ~1e-7 / |v_t| (FP64 fixes
+# such phases; FP32 in general does not).
+VMIN_MIN = 1.0e-2
 
 
 def wrapped_err(a, b):
@@ -22,14 +27,20 @@ def wts_dev(wts, torch):
     return torch.from_numpy(np.ascontiguousarray(wts).view(np.int32)).to("cuda")
 
 
-def compare_link(gpu, ref, gap=None, phase_tol=PHASE_TOL, tcorr_tol=TCORR_TOL, gap_min=GAP_MIN):
-    """Return a dict of mismatch statistics between GPU and oracle outputs."""
+def compare_link(gpu, ref, gap=None, phase_tol=PHASE_TOL, tcorr_tol=TCORR_TOL, gap_min=GAP_MIN, vmin=None,
+                 vmin_min=VMIN_MIN):
+    """Return a dict of mismatch statistics between GPU and oracle outputs.
+    Pixels count as well posed when processed, with eigen-gap >= gap_min and
+    (when vmin is given) eigenvector components no smaller than vmin_min of
+    the RMS component."""
     ph_g, tc_g, co_g = gpu
     ph_r, tc_r, co_r = ref["phase"], ref["tcorr"], ref["comp"]
     ok_r = tc_r > 0            # processed, not failed
     well = ok_r.copy()
     if gap is not None:
         well &= gap >= gap_min
+    if vmin is not None:
+        well &= vmin >= vmin_min
     err = wrapped_err(ph_g, ph_r).max(axis=0)
     out = {
         "n_ok": int(ok_r.sum()), "n_well": int(well.sum()),

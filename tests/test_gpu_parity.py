@@ -77,21 +77,86 @@ def test_nmap_bit_exact(n, method):
     assert np.array_equal(cg, cr)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed_bf16"])
-def test_link_generic_solver_n40(precision):
-    # above 32 dates the generic LDS-resident solver runs
+def _full_rank_gap(gap, rows, cols, Nx, Ny, n, precision):
+    """For the float-Gamma precisions: gap with pixels whose (full-mask)
+    window holds fewer than n samples marked ill-posed.  Their sample
+    coherence is rank deficient: |Gamma| is near singular, so MLE / EMI turn
+    the ~1e-6 error of a float Gamma into phase errors of any size (an FP64
+    solve of the same float Gamma misses as well), and the EVD eigen-gaps
+    shrink to where n * 1e-6 / gap exceeds 1e-3 rad.  FP64 keeps every pixel
+    the eigen-gap admits."""
+    if precision == "fp64":
+        return gap
+    r, c = np.arange(rows)[:, None], np.arange(cols)[None, :]
+    k = ((np.minimum(r + Ny, rows - 1) - np.maximum(r - Ny, 0) + 1)
+         * (np.minimum(c + Nx, cols - 1) - np.maximum(c - Nx, 0) + 1))
+    return np.where(k >= n, gap, -1.0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("module,method", [("phase_link", "MLE"), ("evd", "EVD"), ("evd", "MLE")])
+@pytest.mark.parametrize("n", [33, 48, 64])
+def test_link_64row_solver(n, module, method, precision):
+    # 32 < n <= 64: one pixel per wave, one row of every matrix per lane; an
+    # 11 x 9 window holds 99 samples in the interior, fewer at the edges
+    rows, cols, Nx, Ny = 14, 16, 5, 4
+    print(f"case 64-row n={n} {module}/{method} {precision}", flush=True)
+    slc = synthetic.stack(n, rows, cols, seed=300 + n)
+    wts = synthetic.full_mask(rows, cols, Nx, Ny)
+    g, r = _run_both(slc, wts, Nx, Ny, module, method, precision)
+    st = compare_link(g, r, _full_rank_gap(r["gap"], rows, cols, Nx, Ny, n, precision),
+                      vmin=None if precision == "fp64" else r["vmin"])
+    print(st)
+    assert st["untouched_equal"]
+    if precision == "fp64":
+        assert st["codes_equal"]   # evd MLE failure codes as the reference sets them
+    # at least the 6 x 6 pixels with whole windows; evd MLE's thresholds code
+    # most pixels at n = 64
+    assert st["n_well"] >= (1 if (module, method) == ("evd", "MLE") else 36)
+    assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_link_64row_solver_stbas_reference(precision):
+    n, rows, cols, Nx, Ny = 64, 14, 16, 5, 4
+    print(f"case 64-row stbas {precision}", flush=True)
+    slc = synthetic.stack(n, rows, cols, seed=364)
+    wts = synthetic.full_mask(rows, cols, Nx, Ny)
+    g, r = _run_both(slc, wts, Nx, Ny, "evd", "STBAS", precision, bw=10, ref=3)
+    st = compare_link(g, r, _full_rank_gap(r["gap"], rows, cols, Nx, Ny, n, precision),
+                      vmin=None if precision == "fp64" else r["vmin"])
+    print(st)
+    assert st["n_well"] >= 36
+    assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL
+    assert np.all(g[0][3][r["tcorr"] > 0] == 1 + 0j)
+
+
+def test_link_64row_solver_bf16():
     n, rows, cols, Nx, Ny = 40, 12, 14, 4, 3
-    print(f"case n=40 {precision}", flush=True)
+    print("case 64-row n=40 mixed_bf16", flush=True)
     slc = synthetic.stack(n, rows, cols, seed=103)
     wts = synthetic.full_mask(rows, cols, Nx, Ny)
-    g, r = _run_both(slc, wts, Nx, Ny, "phase_link", "MLE", precision)
+    g, r = _run_both(slc, wts, Nx, Ny, "phase_link", "MLE", "mixed_bf16")
     st = compare_link(g, r, r["gap"])
     print(st)
     assert st["untouched_equal"]
     # the bf16 hi/lo pair carries ~16 significant bits (fp16 with per-date
     # scaling ~22): its Gamma error is ~64x larger, so it gets 5x the bounds
-    ptol, ttol = (PHASE_TOL, TCORR_TOL) if precision != "mixed_bf16" else (5 * PHASE_TOL, 5 * TCORR_TOL)
-    assert st["phase_max"] <= ptol and st["tcorr_max"] <= ttol
+    assert st["phase_max"] <= 5 * PHASE_TOL and st["tcorr_max"] <= 5 * TCORR_TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_link_generic_solver_n72(precision):
+    # above 64 dates the generic LDS-resident solver runs
+    n, rows, cols, Nx, Ny = 72, 12, 14, 5, 4
+    print(f"case generic n=72 {precision}", flush=True)
+    slc = synthetic.stack(n, rows, cols, seed=372)
+    wts = synthetic.full_mask(rows, cols, Nx, Ny)
+    g, r = _run_both(slc, wts, Nx, Ny, "phase_link", "MLE", precision)
+    st = compare_link(g, r, _full_rank_gap(r["gap"], rows, cols, Nx, Ny, n, precision))
+    print(st)
+    assert st["untouched_equal"]
+    assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

@@ -278,7 +278,7 @@ def test_link_evd_matches_numpy_eigh():
     n, rows, cols, Nx, Ny = 8, 7, 9, 2, 2
     slc = synthetic.stack(n, rows, cols, seed=21)
     wts = synthetic.full_mask(rows, cols, Nx, Ny)
-    out = oracle.link(slc, wts, Nx, Ny, module="evd", method="EVD")
+    out = oracle.link(slc, wts, Nx, Ny, module="evd", method="EVD", want_gap=True)
     for r in range(rows):
         for c in range(cols):
             Z = _window_samples(slc, wts, r, c, Nx, Ny)
@@ -289,6 +289,11 @@ def test_link_evd_matches_numpy_eigh():
             v = V[:, -1]
             ph = np.exp(1j * (np.angle(v) - np.angle(v[0])))
             assert np.abs(np.angle(out["phase"][:, r, c] * ph.conj())).max() < 1e-6
+            # diagnostics: gap over spread and the smallest unit-eigenvector
+            # component relative to the RMS one (zheevr runs with the
+            # reference's ABSTOL = 1e-6)
+            assert abs(out["gap"][r, c] - (w[-1] - w[-2]) / (w[-1] - w[0])) < 1e-6
+            assert abs(out["vmin"][r, c] - np.sqrt(n) * np.abs(v).min()) < 1e-6
             # temporal coherence from its definition (evd.cpp:770-786)
             iu = np.triu_indices(n, 1)
             e = np.exp(1j * (np.angle(gam[iu]) - np.angle(ph[iu[0]]) + np.angle(ph[iu[1]])))

@@ -57,23 +57,43 @@ template <typename T> __device__ inline cx<T> crecip(cx<T> a) {
 }
 
 // ---- lane groups ---------------------------------------------------------
-// A wave of 64 lanes is split into 64/G groups of G lanes; each group
-// works on one pixel.  Group reductions use cross-lane shuffles limited to
-// the group's lanes.
-template <int G, typename T> __device__ inline T gsum(T x) {
+// G <= 64: a wave of 64 lanes is split into 64/G groups of G lanes, each
+// working on one pixel, and group reductions use cross-lane shuffles limited
+// to the group's lanes.  G > 64: a workgroup of G/64 waves works on one
+// pixel; each wave reduces by shuffles and the G/64 partials meet in LDS
+// (every lane of the workgroup must take part: the barriers are block-wide).
+// One 32-byte LDS scratch serves every multi-wave reduction of a kernel
+// (partials travel as double, exact for float and double operands), so the
+// reductions add 32 bytes of static LDS to a kernel whatever their number.
+__device__ inline double* greduce_scratch() {
+    __shared__ double part[4];
+    return part;
+}
+template <int G, typename T, typename F> __device__ inline T greduce(T x, F f) {
+    static_assert(G <= 256, "at most four waves per group");
+    const int W = G > 64 ? 64 : G;
 #pragma unroll
-    for (int o = G / 2; o >= 1; o >>= 1) x += __shfl_xor(x, o, G);
+    for (int o = W / 2; o >= 1; o >>= 1) x = f(x, __shfl_xor(x, o, W));
+    if constexpr (G > 64) {
+        static_assert(sizeof(T) <= sizeof(double), "floating-point partials");
+        double* part = greduce_scratch();
+        __syncthreads();   // the previous reduction's reads are done
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = (double)x;
+        __syncthreads();
+        x = (T)part[0];
+#pragma unroll
+        for (int w = 1; w < G / 64; ++w) x = f(x, (T)part[w]);
+    }
     return x;
+}
+template <int G, typename T> __device__ inline T gsum(T x) {
+    return greduce<G>(x, [](T a, T b) { return a + b; });
 }
 template <int G, typename T> __device__ inline T gmax(T x) {
-#pragma unroll
-    for (int o = G / 2; o >= 1; o >>= 1) x = max(x, __shfl_xor(x, o, G));
-    return x;
+    return greduce<G>(x, [](T a, T b) { return max(a, b); });
 }
 template <int G, typename T> __device__ inline T gmin(T x) {
-#pragma unroll
-    for (int o = G / 2; o >= 1; o >>= 1) x = min(x, __shfl_xor(x, o, G));
-    return x;
+    return greduce<G>(x, [](T a, T b) { return min(a, b); });
 }
 template <int G, typename T> __device__ inline cx<T> gsum(cx<T> x) {
     return cx<T>{gsum<G>(x.re), gsum<G>(x.im)};

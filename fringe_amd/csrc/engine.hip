@@ -168,10 +168,12 @@ int link_check(const fringe_link_tile* t) {
     if (link_uses_mfma(t->nbands, t->precision) &&
         ((long long)t->rows * t->cols + 1) * 2 * t->nbands >= (1ll << 32))
         return FRINGE_ERR_UNSUPPORTED;
-    // LDS footprint of the generic solver used above 64 dates (one wave per workgroup)
+    // LDS footprint of the generic solver used above 64 dates (one pixel per
+    // two-wave workgroup): the pixel's matrices plus the kernel's static LDS
     if (t->nbands > 64) {
-        const size_t need = t->precision == FRINGE_PREC_FP64 ? solve_group_bytes<double>(t->nbands)
-                                                             : solve_group_bytes<float>(t->nbands);
+        const size_t need = (t->precision == FRINGE_PREC_FP64 ? solve_group_bytes<double>(t->nbands)
+                                                              : solve_group_bytes<float>(t->nbands)) +
+                            solve_static_lds(t->precision);
         if (need > (size_t)device_lds_limit()) return FRINGE_ERR_UNSUPPORTED;
     }
     return 0;

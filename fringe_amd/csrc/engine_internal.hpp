@@ -73,5 +73,7 @@ int gram_mfma_launch(const LinkPlan& P, const uint32_t* pk, const uint32_t* wts,
 bool link_uses_mfma(int n, int precision);
 // pixel rows of one workgroup tile of the staged Gram kernel (1 when it is not used)
 int gram_stage_rows(int n, int Nx, int Ny);
+// static LDS bytes of the solver kernel that runs above 64 dates
+size_t solve_static_lds(int precision);
 
 }  // namespace fringe

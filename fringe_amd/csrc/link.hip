@@ -181,11 +181,20 @@ static void launch_gram_f64(GramArgs a, hipStream_t s) {
 // ---------------------------------------------------------------------------
 template <typename T, typename Tg, int G, int R>
 static void launch_solve(SolveArgs a, hipStream_t s) {
-    constexpr int GPW = 64 / G;
+    constexpr int GPW = G >= 64 ? 1 : 64 / G;
     a.group_bytes = solve_group_bytes<T>(a.n);
     const unsigned blocks = (unsigned)((a.slots + GPW - 1) / GPW);
-    hipLaunchKernelGGL((solve_kernel<T, Tg, G, R>), dim3(blocks), dim3(64), a.group_bytes * GPW, s,
+    hipLaunchKernelGGL((solve_kernel<T, Tg, G, R>), dim3(blocks), dim3(G > 64 ? G : 64), a.group_bytes * GPW, s,
                        a);
+}
+
+// Static LDS of the solver kernel that runs above 64 dates (its reduction
+// scratch); the launch check adds it to the dynamic LDS of a pixel.
+size_t solve_static_lds(int precision) {
+    hipFuncAttributes fa{};
+    const void* k = precision == FRINGE_PREC_FP64 ? (const void*)solve_kernel<double, double, 128, 1>
+                                                   : (const void*)solve_kernel<float, double, 128, 1>;
+    return hipFuncGetAttributes(&fa, k) == hipSuccess ? fa.sharedSizeBytes : 256;
 }
 
 template <typename T, typename Tg>
@@ -208,7 +217,7 @@ static void dispatch_solve(const SolveArgs& a, hipStream_t s) {
         else
             hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_EVD, 64>), dim3(blocks), dim3(64), 0, s, a);
     }
-    else launch_solve<T, Tg, 64, 2>(a, s);
+    else launch_solve<T, Tg, 128, 1>(a, s);   // 64 < n <= 128: two waves per pixel
 }
 
 bool link_uses_mfma(int n, int precision) { return precision != FRINGE_PREC_FP64 && n <= 64; }

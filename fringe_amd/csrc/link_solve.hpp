@@ -2,9 +2,10 @@
 // normalised coherence Gamma -> (|Gamma|^-1 o Gamma) -> extreme eigenpair ->
 // referenced phases, compressed SLC, temporal coherence.
 //
-// Lane groups: a wave holds 64/G pixels; lane g of a group owns matrix rows
-// g + G*q (q < R), so n <= G*R.  All control flow that contains a barrier is
-// uniform over the wave (it depends only on launch arguments or on wave-wide
+// Lane groups: a wave holds 64/G pixels (G <= 64), or a workgroup of G/64
+// waves holds one (G > 64); lane g of a group owns matrix rows g + G*q
+// (q < R), so n <= G*R.  All control flow that contains a barrier is uniform
+// over the workgroup (it depends only on launch arguments or on group-wide
 // votes); per-pixel decisions are carried as predicates.
 #pragma once
 #include "device_common.hpp"
@@ -83,7 +84,10 @@ template <typename T> __device__ inline SolveLds<T> solve_carve(char* base, int 
     return L;
 }
 
-template <int G> __device__ inline bool group_all(bool b) { return group_ballot<G>(!b) == 0ull; }
+template <int G> __device__ inline bool group_all(bool b) {
+    if constexpr (G > 64) return gmin<G>(b ? 1.0f : 0.0f) > 0.5f;   // through the shared scratch
+    else return group_ballot<G>(!b) == 0ull;
+}
 
 // ---- load Gamma (optionally banded for STBAS) into A --------------------
 template <typename T, typename Tg, int G, int R>
@@ -327,8 +331,15 @@ __device__ T select_eigenvalue(SolveLds<T>& L, int n, int k, T& tnorm) {
         const T w = hi - lo;
         const T sig = lo + (w * T(g + 1)) / T(G + 1);
         const int cnt = sturm_count(L.D, L.E2, n, sig, pivmin);
-        const uint64_t b = group_ballot<G>(cnt > k);
-        const int f = b ? (int)__builtin_ctzll(b) : G;
+        // first lane whose shift has more than k eigenvalues below it; the
+        // counts grow with the shift, so that is G minus the number of such lanes
+        int f;
+        if constexpr (G > 64) {
+            f = G - (int)gsum<G>(cnt > k ? 1.0f : 0.0f);
+        } else {
+            const uint64_t b = group_ballot<G>(cnt > k);
+            f = b ? (int)__builtin_ctzll(b) : G;
+        }
         const T nhi = (f < G) ? lo + (w * T(f + 1)) / T(G + 1) : hi;
         const T nlo = (f > 0) ? lo + (w * T(f)) / T(G + 1) : lo;
         lo = nlo;
@@ -437,8 +448,8 @@ template <typename T> __device__ inline cx<T> unit_or(cx<T> u, cx<T> fallback, b
 
 // ---------------------------------------------------------------------------
 template <typename T, typename Tg, int G, int R>
-__global__ __launch_bounds__(64) void solve_kernel(SolveArgs a) {
-    constexpr int GPW = 64 / G;
+__global__ __launch_bounds__(G > 64 ? G : 64) void solve_kernel(SolveArgs a) {
+    constexpr int GPW = G >= 64 ? 1 : 64 / G;
     const int grp = threadIdx.x / G, g = threadIdx.x % G;
     const int slot = blockIdx.x * GPW + grp;
     const int n = a.n, S = n | 1;

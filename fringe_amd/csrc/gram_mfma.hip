@@ -502,7 +502,9 @@ __global__ __launch_bounds__(256) void gram_mfma_lds(MfmaArgs a) {
         float dv = 0.0f;
 #pragma unroll
         for (int v = 0; v < 16; ++v) dv = (acc_row(v, lane) == m) ? RR[v] + II[v] : dv;
-        if (((m >> 2) & 1) == hf) dg[m] = dv;
+        // published as 1/sqrt(Re G_mm): one transcendental per lane, and the
+        // normalisation of entry (i, m) is a product
+        if (((m >> 2) & 1) == hf) dg[m] = rsqrtf(dv);
         wave_sync_lds();
         float di[16];
 #pragma unroll
@@ -512,7 +514,7 @@ __global__ __launch_bounds__(256) void gram_mfma_lds(MfmaArgs a) {
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const int i = acc_row(v, lane);
-            const float sc = rsqrtf(di[v] * dj);
+            const float sc = di[v] * dj;
             const bool off = i != m;
             const float gr = off ? (RR[v] + II[v]) * sc : 1.0f;
             const float gi = off ? (IR[v] - RI[v]) * sc : 0.0f;   // P(im_i, re_j) - P(re_i, im_j)

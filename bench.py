@@ -233,6 +233,63 @@ def cfg3(args):
     print(json.dumps(out))
 
 
+def cfg1(args):
+    """The reference's CPU-sized case: N = 10 dates of 256 x 256, phase ramp
+    phi_n(col) = n * 3 rad * col / (cols - 1), 11 x 11 all-ones window, evd
+    EVD.  GPU FP64 and mixed link times on the full tile, the oracle (CPU,
+    reference semantics, FP64 LAPACK) timed on the same full tile as the CPU
+    baseline, the GPU phases against the oracle's, and the RMSE of the
+    recovered ramp."""
+    import torch
+    from fringe_amd import synthetic, tile
+    dev = torch.device("cuda:0")
+    n, rows, cols, Nx, Ny = 10, 256, 256, 5, 5
+    slc_np = synthetic.stack(n, rows, cols, seed=101)
+    wts_np = synthetic.full_mask(rows, cols, Nx, Ny)
+    slc = torch.from_numpy(slc_np).to(dev)
+    wts = torch.from_numpy(wts_np.view(np.int32)).to(dev)
+    res, runs = {}, {}
+    for prec in ("fp64", "mixed"):
+        r = tile.LinkRunner(tile.link_params(rows, cols, n, Nx, Ny, module="evd", method="EVD", precision=prec),
+                            device=dev)
+        for _ in range(args.warmup):
+            r.run(slc, wts)
+        torch.cuda.synchronize()
+        t = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            r.run(slc, wts)
+            torch.cuda.synchronize()
+            t.append((time.perf_counter() - t0) * 1e3)
+        runs[prec] = (r.phase.cpu().numpy(), r.tcorr.cpu().numpy())
+        res[prec] = {"ms_median": float(np.median(t)), "pixels_per_s": rows * cols / (np.median(t) / 1e3)}
+        del r
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import oracle
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    ref = oracle.link(slc_np, wts_np, Nx, Ny, module="evd", method="EVD", want_gap=True, threads=threads)
+    dt = time.perf_counter() - t0
+    well = (ref["tcorr"] > 0) & (ref["gap"] >= 1e-3)
+    truth = np.exp(1j * synthetic.ramp_phase(n, cols))[:, None, :]   # referenced to date 0 (zero phase)
+    for prec, (ph, tc) in runs.items():
+        e = np.abs(np.angle(ph * np.conj(ref["phase"]))).max(axis=0)
+        res[prec].update({
+            "phase_err_vs_oracle_max_rad": float(e[well].max()),
+            "tcorr_err_vs_oracle_max": float(np.abs(tc - ref["tcorr"])[well].max()),
+            "ramp_rmse_deg": float(np.degrees(np.sqrt(np.mean(np.angle(ph * np.conj(truth)) ** 2))))})
+    out = {"metric": "cfg1: pixels/s linked (evd EVD, N=10, 256^2, 11x11 all-ones window), GPU vs the CPU oracle",
+           "config": {"workload": f"N={n}, {rows}x{cols} px, window {2 * Nx + 1}x{2 * Ny + 1}, all-ones mask, "
+                                  "phase ramp 0..3 rad per date step, evd EVD", "pixels": rows * cols},
+           "data": "synthetic (seeded numpy generator, seed 101)",
+           "fp64": res["fp64"], "mixed": res["mixed"],
+           "ramp_rmse_deg_oracle": float(np.degrees(np.sqrt(np.mean(np.angle(ref["phase"] * np.conj(truth)) ** 2)))),
+           "cpu_baseline": {"value": rows * cols / dt, "unit": "pixels/s", "cores": threads, "kind": "port",
+                            "sample": f"the full {rows}x{cols} tile ({dt:.2f} s), oracle/fringe_oracle.cpp, "
+                                      "FP64 LAPACK, OpenMP"}}
+    print(json.dumps(out))
+
+
 def cfg4(args):
     """Sequential estimator on device-resident data (sequential.py:144-255):
     120 dates of 1024 x 1024 (12-day spacing, same coherence model), split into
@@ -324,6 +381,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=102)
     ap.add_argument("--copy-probe", action="store_true", help="run only the copy kernel used to calibrate PMC bytes")
     ap.add_argument("--serve", action="store_true", help="serving workload: Poisson stream of mixed tiles")
+    ap.add_argument("--cfg1", action="store_true",
+                    help="N=10, 256^2, evd EVD: GPU FP64 and mixed against the CPU oracle on the full tile")
     ap.add_argument("--cfg3", action="store_true",
                     help="N=64, 21x11 window: coherence and EVD phase errors of the hi/lo Grams against FP64")
     ap.add_argument("--cfg3-rows", type=int, default=32, help="output rows of the N=64 sample (4096 columns)")
@@ -345,6 +404,9 @@ def main():
         return
     if args.serve:
         serve(args)
+        return
+    if args.cfg1:
+        cfg1(args)
         return
     if args.cfg3:
         cfg3(args)

@@ -290,6 +290,94 @@ def cfg1(args):
     print(json.dumps(out))
 
 
+def e2e(args):
+    """End-to-end time of the file-based modules on the headline shape (N = 30
+    dates, rows x cols, 15 x 11 window, gamma-patch amplitudes): the stack is
+    written as per-date ENVI files with raw VRTs and a stack VRT (the
+    tops2vrt layout), the nmap module writes the SHP mask, and the phase_link
+    module (EMI, minNeighbors 5, the reference's default block memory)
+    reads stack and mask, links and writes one ENVI raster per date plus
+    tcorr and the compressed SLC.  Also the host-buffer rate: the stack and
+    mask in host memory, copied to the GPU, linked, and the outputs copied
+    back (PCIe-inclusive, no file I/O)."""
+    import datetime
+    import shutil
+    import tempfile
+    import torch
+    from fringe_amd import raster, synthetic, tile
+    from fringe_amd.cli import nmap as nmap_cli, phase_link as pl_cli
+    dev = torch.device("cuda:0")
+    n, rows, cols, Nx, Ny = args.n, args.rows, args.cols, args.nx, args.ny
+    torch.cuda.manual_seed(args.seed)
+    slc = synthetic.stack_torch(n, rows, cols, args.seed, device=dev, patches=32).cpu().numpy()
+    root = tempfile.mkdtemp(prefix="fringe_e2e_")
+    out = {"metric": "end-to-end phase_link module time (files in, files out) and host-buffer rate",
+           "config": {"workload": f"phase_link EMI, N={n}, {rows}x{cols} px, window {2 * Nx + 1}x{2 * Ny + 1}, "
+                                  "gamma-patch amplitudes, nmap KS2 mask, minNeighbors=5", "pixels": rows * cols},
+           "data": "synthetic (seeded), written to local disk"}
+    try:
+        t = time.perf_counter()
+        srcs, d0 = [], datetime.date(2022, 1, 1)
+        os.makedirs(os.path.join(root, "slcs"))
+        for k in range(n):
+            d = (d0 + datetime.timedelta(12 * k)).strftime("%Y%m%d")
+            os.makedirs(os.path.join(root, "SLC", d))
+            f = os.path.join(root, "SLC", d, d + ".slc.full")
+            slc[k].tofile(f)
+            v = os.path.join(root, "slcs", d + ".vrt")
+            with open(v, "w") as fh:
+                fh.write(raster.raw_vrt(f, cols, rows))
+            srcs.append((v, d, True))
+        base = os.path.join(root, "slcs_base.vrt")
+        with open(base, "w") as fh:
+            fh.write(raster.stack_vrt(srcs, cols, rows))
+        out["write_stack_s"] = time.perf_counter() - t
+        print(f"stack written in {out['write_stack_s']:.1f} s", file=sys.stderr, flush=True)
+        wf, cf = os.path.join(root, "nmap", "wts"), os.path.join(root, "nmap", "count")
+        t = time.perf_counter()
+        assert nmap_cli.main(["-i", base, "-o", wf, "-c", cf, "-x", str(Nx), "-y", str(Ny)]) == 0
+        out["nmap_module_s"] = time.perf_counter() - t
+        print(f"nmap module {out['nmap_module_s']:.1f} s", file=sys.stderr, flush=True)
+        for prec in ("mixed", "fp64"):
+            od = os.path.join(root, "pl_" + prec)
+            t = time.perf_counter()
+            assert pl_cli.main(["-i", base, "-w", wf, "-o", od, "-x", str(Nx), "-y", str(Ny), "-n", "5",
+                                "--precision", prec]) == 0
+            dt = time.perf_counter() - t
+            out[f"phase_link_module_{prec}_s"] = dt
+            out[f"phase_link_module_{prec}_pixels_per_s"] = rows * cols / dt
+            print(f"phase_link module ({prec}) {dt:.1f} s", file=sys.stderr, flush=True)
+            shutil.rmtree(od)
+        # host buffers: H2D + link + D2H, no file I/O
+        nul = (((2 * Nx + 1) * (2 * Ny + 1)) + 31) // 32
+        wts = np.stack([raster.read(wf, b + 1, dtype=np.uint32) for b in range(nul)], axis=-1)
+        slc_h = torch.from_numpy(slc).pin_memory()
+        wts_h = torch.from_numpy(wts.view(np.int32)).pin_memory()
+        r = tile.LinkRunner(tile.link_params(rows, cols, n, Nx, Ny, module="phase_link", method="MLE",
+                                             min_neighbors=5, precision="mixed"), device=dev)
+        ph_h = torch.empty((n, rows, cols), dtype=torch.complex64).pin_memory()
+        tc_h = torch.empty((rows, cols), dtype=torch.float32).pin_memory()
+        co_h = torch.empty((rows, cols), dtype=torch.complex64).pin_memory()
+        times = []
+        for it in range(args.warmup + 3):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            sd, wd = slc_h.to(dev, non_blocking=True), wts_h.to(dev, non_blocking=True)
+            ph, tc, co = r.run(sd, wd)
+            ph_h.copy_(ph, non_blocking=True)
+            tc_h.copy_(tc, non_blocking=True)
+            co_h.copy_(co, non_blocking=True)
+            torch.cuda.synchronize()
+            if it >= args.warmup:
+                times.append(time.perf_counter() - t)
+        out["host_buffers_mixed_s"] = float(np.median(times))
+        out["host_buffers_mixed_pixels_per_s"] = rows * cols / float(np.median(times))
+        out["bytes_over_pcie"] = int(slc.nbytes + wts.nbytes + ph_h.numel() * 8 + tc_h.numel() * 4 + co_h.numel() * 8)
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+    print(json.dumps(out))
+
+
 def cfg4(args):
     """Sequential estimator on device-resident data (sequential.py:144-255):
     120 dates of 1024 x 1024 (12-day spacing, same coherence model), split into
@@ -381,6 +469,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=102)
     ap.add_argument("--copy-probe", action="store_true", help="run only the copy kernel used to calibrate PMC bytes")
     ap.add_argument("--serve", action="store_true", help="serving workload: Poisson stream of mixed tiles")
+    ap.add_argument("--e2e", action="store_true",
+                    help="file-based nmap + phase_link modules on the headline shape, and the host-buffer rate")
     ap.add_argument("--cfg1", action="store_true",
                     help="N=10, 256^2, evd EVD: GPU FP64 and mixed against the CPU oracle on the full tile")
     ap.add_argument("--cfg3", action="store_true",
@@ -407,6 +497,9 @@ def main():
         return
     if args.cfg1:
         cfg1(args)
+        return
+    if args.e2e:
+        e2e(args)
         return
     if args.cfg3:
         cfg3(args)

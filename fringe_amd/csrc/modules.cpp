@@ -9,6 +9,8 @@
 // return codes, as the reference.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -17,6 +19,7 @@
 #include <memory>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include <stdlib.h>
@@ -67,6 +70,45 @@ int block_rows(int memsize, int blocksize, int cols, int rows, int per_pixel_uni
     if (b > rows) b = rows;
     return b;
 }
+
+// f(i) for i in [0, n) on up to 16 host threads; every band of a stack or
+// output is a file of its own, so band reads and writes run side by side.
+template <typename F> static bool parallel_for(int n, F f) {
+    const int hw = std::max(1, (int)std::thread::hardware_concurrency());
+    const int nt = std::max(1, std::min({n, 16, hw}));
+    std::atomic<int> next{0};
+    std::atomic<bool> ok{true};
+    auto work = [&]() {
+        for (int i = next++; i < n && ok; i = next++)
+            if (!f(i)) ok = false;
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(work);
+    work();
+    for (auto& x : th) x.join();
+    return ok;
+}
+
+// Host staging memory for the PCIe copies: pinned (hipHostMalloc) when the
+// runtime grants it, pageable otherwise.
+template <typename T> struct HostBuf {
+    T* p = nullptr;
+    bool pinned = false;
+    std::vector<T> v;
+    void alloc(size_t count) {
+        if (hipHostMalloc((void**)&p, count * sizeof(T), hipHostMallocDefault) == hipSuccess) {
+            pinned = true;
+            return;
+        }
+        (void)hipGetLastError();
+        v.resize(count);
+        p = v.data();
+    }
+    T* data() { return p; }
+    ~HostBuf() {
+        if (pinned) (void)hipHostFree(p);
+    }
+};
 
 bool next_block(int& yoff, int& count, int blockysize, int rows, int Ny, BlockPlan& bp) {
     if (yoff >= rows) return false;
@@ -222,24 +264,38 @@ int link_module(const fringe_evd_options* o, int module) {
     hipStream_t stream;
     if (hipStreamCreate(&stream) != hipSuccess) return FRINGE_ERR_DEVICE;
 
-    std::vector<std::complex<float>> h_slc(px * nbands), h_ph(px * nbands), h_co(px), row_buf;
-    std::vector<uint32_t> h_wts(px * nulong), band_buf(px);
-    std::vector<float> h_tc(px);
+    HostBuf<std::complex<float>> h_slc, h_ph, h_co;
+    HostBuf<uint32_t> h_wts;
+    HostBuf<float> h_tc;
+    h_slc.alloc(px * nbands);
+    h_ph.alloc(px * nbands);
+    h_co.alloc(px);
+    h_wts.alloc(px * nulong);
+    h_tc.alloc(px);
+    std::vector<uint32_t> wts_planes(px * nulong);
     int yoff = 0, count = 0, status = 0;
     BlockPlan bp;
     while (status == 0 && next_block(yoff, count, blockysize, rows, Ny, bp)) {
         const size_t bpx = (size_t)bp.inysize * cols;
-        for (int b = 0; b < nbands && status == 0; ++b)
-            if (!fringe::io::read_window(*in, b + 1, 0, bp.yoff, cols, bp.inysize, h_slc.data() + (size_t)b * bpx))
-                status = FRINGE_ERR_READ_SLC;
-        for (int w = 0; w < nulong && status == 0; ++w) {
-            if (!fringe::io::read_window(*wts, w + 1, 0, bp.yoff, cols, bp.inysize, band_buf.data())) {
-                status = FRINGE_ERR_READ_WTS;
-                break;
-            }
-            for (size_t p = 0; p < bpx; ++p) h_wts[p * nulong + w] = band_buf[p];
+        if (!parallel_for(nbands, [&](int b) {
+                return fringe::io::read_window(*in, b + 1, 0, bp.yoff, cols, bp.inysize, h_slc.data() + (size_t)b * bpx);
+            })) {
+            status = FRINGE_ERR_READ_SLC;
+            break;
         }
-        if (status) break;
+        if (!parallel_for(nulong, [&](int w) {
+                return fringe::io::read_window(*wts, w + 1, 0, bp.yoff, cols, bp.inysize,
+                                               wts_planes.data() + (size_t)w * bpx);
+            })) {
+            status = FRINGE_ERR_READ_WTS;
+            break;
+        }
+        // band planes -> [pixel][word], in row chunks
+        parallel_for(bp.inysize, [&](int y) {
+            for (size_t p = (size_t)y * cols; p < (size_t)(y + 1) * cols; ++p)
+                for (int w = 0; w < nulong; ++w) h_wts.data()[p * nulong + w] = wts_planes[(size_t)w * bpx + p];
+            return true;
+        });
         t.rows = bp.inysize;
         t.row_begin = bp.first;
         t.row_end = bp.first + bp.lines;
@@ -263,12 +319,20 @@ int link_module(const fringe_evd_options* o, int module) {
             break;
         }
         const size_t off = (size_t)bp.first * cols;
-        for (int b = 0; b < nbands && status == 0; ++b)
-            if (!outs[b].write_rows(bp.yoff + bp.first, bp.lines, h_ph.data() + (size_t)b * bpx + off))
-                status = FRINGE_ERR_WRITE_SLC;
-        if (status == 0 && (!corr.write_rows(bp.yoff + bp.first, bp.lines, h_tc.data() + off) ||
-                            !comp.write_rows(bp.yoff + bp.first, bp.lines, h_co.data() + off)))
-            status = FRINGE_ERR_WRITE_AUX;
+        std::atomic<bool> slc_ok{true}, aux_ok{true};
+        parallel_for(nbands + 2, [&](int b) {
+            const int y0 = bp.yoff + bp.first;
+            if (b < nbands) {
+                if (!outs[b].write_rows(y0, bp.lines, h_ph.data() + (size_t)b * bpx + off)) slc_ok = false;
+            } else if (b == nbands) {
+                if (!corr.write_rows(y0, bp.lines, h_tc.data() + off)) aux_ok = false;
+            } else if (!comp.write_rows(y0, bp.lines, h_co.data() + off)) {
+                aux_ok = false;
+            }
+            return true;
+        });
+        if (!slc_ok) status = FRINGE_ERR_WRITE_SLC;
+        else if (!aux_ok) status = FRINGE_ERR_WRITE_AUX;
         advance(yoff, blockysize, rows, bp);
         if (verbose) std::printf("\rprogress %5.1f%%", 100.0 * yoff / rows), std::fflush(stdout);
     }
@@ -627,19 +691,23 @@ int fringe_nmap_process(const fringe_nmap_options* o) {
         return FRINGE_ERR_DEVICE;
     hipStream_t stream;
     if (hipStreamCreate(&stream) != hipSuccess) return FRINGE_ERR_DEVICE;
-    std::vector<std::complex<float>> h_slc(px * nbands);
+    HostBuf<std::complex<float>> h_slc;
+    HostBuf<uint32_t> h_wts;
+    HostBuf<int32_t> h_cnt;
+    h_slc.alloc(px * nbands);
+    h_wts.alloc(px * nulong);
+    h_cnt.alloc(px);
     std::vector<uint8_t> h_mask(px);
-    std::vector<uint32_t> h_wts(px * nulong);
-    std::vector<int32_t> h_cnt(px);
     std::vector<int16_t> cnt16(px);
     int yoff = 0, count = 0, status = 0;
     BlockPlan bp;
     while (status == 0 && next_block(yoff, count, blockysize, rows, Ny, bp)) {
         const size_t bpx = (size_t)bp.inysize * cols;
         if (msk && !fringe::io::read_window(*msk, 1, 0, bp.yoff, cols, bp.inysize, h_mask.data())) status = 108;
-        for (int b = 0; b < nbands && status == 0; ++b)
-            if (!fringe::io::read_window(*in, b + 1, 0, bp.yoff, cols, bp.inysize, h_slc.data() + (size_t)b * bpx))
-                status = 108;
+        if (status == 0 && !parallel_for(nbands, [&](int b) {
+                return fringe::io::read_window(*in, b + 1, 0, bp.yoff, cols, bp.inysize, h_slc.data() + (size_t)b * bpx);
+            }))
+            status = 108;
         if (status) break;
         t.rows = bp.inysize;
         t.row_begin = bp.first;
@@ -663,7 +731,7 @@ int fringe_nmap_process(const fringe_nmap_options* o) {
         }
         const size_t off = (size_t)bp.first * cols;
         for (size_t p = 0; p < (size_t)bp.lines * cols; ++p) {
-            const int32_t v = h_cnt[off + p];   // Int32 buffer written to an Int16 file (nmap.cpp:535-539)
+            const int32_t v = h_cnt.data()[off + p];   // Int32 buffer written to an Int16 file (nmap.cpp:535-539)
             cnt16[p] = (int16_t)(v > 32767 ? 32767 : v < -32768 ? -32768 : v);
         }
         if (!wcount.write_rows(bp.yoff + bp.first, bp.lines, cnt16.data()) ||

@@ -525,13 +525,33 @@ template <> uint32_t decode<uint32_t>(const unsigned char* src, DType t, bool ms
     }
 }
 
+// the file sample type a T is stored as without conversion
+template <typename T> struct NativeType { static constexpr DType value = DType::Unknown; };
+template <> struct NativeType<std::complex<float>> { static constexpr DType value = DType::CFloat32; };
+template <> struct NativeType<float> { static constexpr DType value = DType::Float32; };
+template <> struct NativeType<uint32_t> { static constexpr DType value = DType::UInt32; };
+template <> struct NativeType<uint8_t> { static constexpr DType value = DType::Byte; };
+
 template <typename T>
 static bool read_raw(const Source& s, int x0, int y0, int w, int h, T* out, int ldo) {
     FILE* f = std::fopen(s.file.c_str(), "rb");
     if (!f) return false;
     const int es = dtype_size(s.dtype);
-    std::vector<unsigned char> line;
     bool ok = true;
+    if (s.dtype == NativeType<T>::value && !s.msb && s.pixel_offset == es && es == (int)sizeof(T)) {
+        // packed samples of the output type in native byte order: straight
+        // into the output rows (what decode() would produce, element by element)
+        for (int y = 0; y < h && ok; ++y) {
+            const long long start = s.image_offset + (long long)(y0 + y) * s.line_offset + (long long)x0 * es;
+            T* row = out + (size_t)y * ldo;
+            if (fseeko(f, start, SEEK_SET) != 0) { ok = false; break; }
+            const size_t got = std::fread(row, sizeof(T), (size_t)w, f);
+            for (size_t x = got; x < (size_t)w; ++x) row[x] = T();
+        }
+        std::fclose(f);
+        return ok;
+    }
+    std::vector<unsigned char> line;
     for (int y = 0; y < h && ok; ++y) {
         const long long start = s.image_offset + (long long)(y0 + y) * s.line_offset + (long long)x0 * s.pixel_offset;
         const long long span = (long long)(w - 1) * s.pixel_offset + es;

@@ -56,7 +56,7 @@ __host__ __device__ inline size_t align16(size_t b) { return (b + 15) & ~(size_t
 template <typename T> __host__ __device__ inline size_t solve_group_bytes(int n) {
     const int S = n | 1;
     size_t b = align16((size_t)n * S * sizeof(cx<T>));
-    b += 5 * align16((size_t)n * sizeof(cx<T>));
+    b += 4 * align16((size_t)n * sizeof(cx<T>));
     b += 8 * align16((size_t)n * sizeof(T));
     b += align16((size_t)n * sizeof(cx<float>));
     return b;
@@ -71,7 +71,6 @@ template <typename T> __device__ inline SolveLds<T> solve_carve(char* base, int 
     L.W = (cx<T>*)p;   p += align16((size_t)n * sizeof(cx<T>));
     L.TAU = (cx<T>*)p; p += align16((size_t)n * sizeof(cx<T>));
     L.Y = (cx<T>*)p;   p += align16((size_t)n * sizeof(cx<T>));
-    p += align16((size_t)n * sizeof(cx<T>));   // spare
     L.D = (T*)p;   p += align16((size_t)n * sizeof(T));
     L.E = (T*)p;   p += align16((size_t)n * sizeof(T));
     L.E2 = (T*)p;  p += align16((size_t)n * sizeof(T));

@@ -416,7 +416,9 @@ __global__ __launch_bounds__(256) void gram_mfma_lds(MfmaArgs a) {
         const long long pix = (long long)r * a.cols + c;
         const int slot = (r - a.row0) * a.cols + c;
         const int fc = shp_flat(0, 0, Nx, Ny);
-        const bool cen = (__shfl(wword, fc >> 5) >> (fc & 31)) & 1u;
+        // mask words reach the lanes through readlane (scalar registers), not
+        // through an LDS-routed shuffle
+        const bool cen = (__builtin_amdgcn_readlane(wword, fc >> 5) >> (fc & 31)) & 1u;
         // ---- neighbour list (staged record indices) ----
         const int lr = r - R0, lc = c - C0;                  // window origin in the stage
         int cnt = 0;
@@ -424,7 +426,11 @@ __global__ __launch_bounds__(256) void gram_mfma_lds(MfmaArgs a) {
         for (int it = 0; it < kPass; ++it) {
             if (it * 64 >= Wn) break;   // uniform
             const int f = it * 64 + lane;
-            const uint32_t wf = __shfl(wword, (f < Wn ? f : 0) >> 5);
+            // lanes 0..31 test word 2it, lanes 32..63 word 2it+1 (lanes past
+            // the last word hold word 0 and only cover positions >= Wn)
+            const uint32_t wlo = __builtin_amdgcn_readlane(wword, 2 * it);
+            const uint32_t whi = __builtin_amdgcn_readlane(wword, 2 * it + 1);
+            const uint32_t wf = lane < 32 ? wlo : whi;
             bool take = false;
             int qpos = 0;
             if (cen && f < Wn) {

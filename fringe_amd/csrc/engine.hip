@@ -181,7 +181,11 @@ int link_check(const fringe_link_tile* t) {
 
 static int link_chunk_pixels(const fringe_link_tile* t) {
     const size_t per = link_gamma_bytes_per_pixel(t->nbands, t->precision);
-    const size_t target = (size_t)192 << 20;   // keep Gamma of one round near the last-level cache
+    // Gamma of one round: ~2 GB, so that one solver launch holds tens of
+    // generations of resident waves (their Gamma reads and launch tails then
+    // overlap other waves' work); smaller rounds that keep Gamma in the
+    // 256 MB last-level cache were measured slower
+    const size_t target = (size_t)2048 << 20;
     long long rows = (long long)(target / (per * (size_t)t->cols));
     if (rows < 1) rows = 1;
     // whole pixel tiles of the staged Gram kernel

@@ -312,12 +312,30 @@ double fringe_ks_table(int n, double prob, uint8_t* h_table) {
     return margin;
 }
 
+// Workspace of fringe_nmap_run: sorted amplitudes [rows*cols][n] float,
+// validity [rows*cols], the AD2 term table, then the forward-half mask
+// words and counts.  With P, points its device pointers into d_work.
+static size_t nmap_layout(const fringe_nmap_tile* t, NmapPlan* P, void* d_work = nullptr) {
+    const size_t npx = (size_t)t->rows * t->cols;
+    const size_t o_valid = align256(npx * t->nbands * sizeof(float));
+    const size_t o_ad2 = o_valid + align256(npx);
+    const size_t o_fwd = o_ad2 + (t->method == 1 ? align256((size_t)(2 * t->nbands - 1) * (t->nbands + 1) * sizeof(double)) : 0);
+    const size_t o_fcnt = o_fwd + align256(npx * nmap_fwd_words(t->Nx, t->Ny) * sizeof(uint32_t));
+    const size_t end = o_fcnt + align256(npx * sizeof(int32_t));
+    if (P) {
+        char* w = (char*)d_work;
+        P->d_amp = (float*)w;
+        P->d_valid = (uint8_t*)(w + o_valid);
+        P->d_ad2 = t->method == 1 ? (double*)(w + o_ad2) : nullptr;
+        P->d_fwd = (uint32_t*)(w + o_fwd);
+        P->d_fcnt = (int32_t*)(w + o_fcnt);
+    }
+    return end;
+}
+
 size_t fringe_nmap_workspace_bytes(const fringe_nmap_tile* t) {
     if (!t || t->rows <= 0 || t->cols <= 0 || t->nbands <= 0) return 0;
-    const size_t npx = (size_t)t->rows * t->cols;
-    size_t b = align256(npx * t->nbands * sizeof(float)) + align256(npx);
-    if (t->method == 1) b += align256((size_t)(2 * t->nbands - 1) * (t->nbands + 1) * sizeof(double));
-    return b;
+    return nmap_layout(t, nullptr);
 }
 
 double fringe_ad2_threshold(int n, double prob, double* h_sigma) {
@@ -362,14 +380,11 @@ int fringe_nmap_run(const fringe_nmap_tile* t, const void* d_slc, const uint8_t*
     std::memset(P.table.bits, 0, sizeof(P.table.bits));
     for (int k = 0; k <= t->nbands; ++k)
         if (tab[k]) P.table.bits[k >> 5] |= (1u << (k & 31));
-    const size_t npx = (size_t)t->rows * t->cols;
-    P.d_amp = (float*)d_work;
-    P.d_valid = (uint8_t*)d_work + align256(npx * t->nbands * sizeof(float));
+    nmap_layout(t, &P, d_work);
     P.method = t->method;
     if (t->method == 1) {
         P.ad2_sigma = ad2_sigma(t->nbands, t->nbands);
         P.ad2_max = ad2_threshold(t->prob);
-        P.d_ad2 = (double*)((char*)d_work + align256(npx * t->nbands * sizeof(float)) + align256(npx));
     }
     if (nmap_launch(P, (const float2*)d_slc, d_mask, d_wts, d_count, (hipStream_t)stream) != 0) {
         set_last_error("fringe_nmap_run", hipGetLastError());

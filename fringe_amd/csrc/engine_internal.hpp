@@ -30,7 +30,12 @@ struct NmapPlan {
     double ad2_sigma;  // AD2: standard deviation of A2_kN (k = 2, n + n samples)
     double ad2_max;    // AD2: similar iff the standardised statistic <= ad2_max
     double* d_ad2;     // AD2: [2n-1][n+1] per-position terms of the inner sum
+    uint32_t* d_fwd;   // forward-half mask words, [rows*cols][nul - centre/32]
+    int32_t* d_fcnt;   // forward-half counts, [rows*cols]
 };
+
+// uint32 words per pixel of the forward half of a mask (the centre's word on)
+int nmap_fwd_words(int Nx, int Ny);
 
 int nmap_launch(const NmapPlan& P, const float2* slc, const uint8_t* mask, uint32_t* wts,
                 int32_t* count, hipStream_t s);

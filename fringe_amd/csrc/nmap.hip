@@ -8,7 +8,12 @@
 // Stage 1 (nmap_prep): per pixel, amplitude |z| / alpha_b for every band,
 // validity, and an ascending sort held in registers (bitonic network).
 // Stage 2 (nmap_net for n <= 64, nmap_sweep above): per pixel, the
-// two-sample test against every valid in-image neighbour.  KS2: the statistic as an integer count k = D*n,
+// two-sample test against every valid in-image neighbour of the forward half
+// of the window (dy > 0, or dy = 0 and dx >= 0), as the reference's sweep
+// visits pairs (nmap.cpp:404-457); the bits go to a workspace.  Stage 3
+// (nmap_mirror) assembles every output pixel's full mask: bit (dy, dx) of
+// the backward half is bit (-dy, -dx) of neighbour p + (dy, dx), i.e. the
+// same test of the same ordered pair, so each pair is tested once.  KS2: the statistic as an integer count k = D*n,
 // decided through a host-built table (k -> similar) instead of evaluating the
 // Kolmogorov series on the device.  AD2: the standardised Anderson-Darling
 // statistic in double, bit-identical to the host's, compared with the largest
@@ -179,43 +184,51 @@ __device__ inline bool ks_similar(int k, const NmapPlan& P) {
 //        the first j+1 positions.
 // Pads (beyond n) get keys above every finite or infinite amplitude.
 // ---------------------------------------------------------------------------
+// Forward-half words of the pixels of rows [fwd_row0, row_end): the
+// output rows and the Ny rows above them whose forward tests reach into
+// them.  Pixel (r, c) keeps words centre/32 .. nul-1 of its mask at
+// F + ((r - fwd_row0) * cols + c) * nf, and its forward count (centre
+// included) at fcnt[(r - fwd_row0) * cols + c].
+__device__ inline int fwd_row0(const NmapPlan& P) { return max(0, P.row_begin - P.Ny); }
+__device__ inline int centre_flat(const NmapPlan& P) { return P.Ny * (2 * P.Nx + 1) + P.Nx; }
+
 template <int NP, bool AD2>
-__global__ __launch_bounds__(256) void nmap_net(const NmapPlan P, uint32_t* __restrict__ wts,
-                                                int32_t* __restrict__ count) {
+__global__ __launch_bounds__(256) void nmap_net(const NmapPlan P, uint32_t* __restrict__ F,
+                                                int32_t* __restrict__ fcnt) {
     const uint32_t* __restrict__ keys = reinterpret_cast<const uint32_t*>(P.d_amp);
     const uint8_t* __restrict__ valid = P.d_valid;
     const int n = P.n, rows = P.rows, cols = P.cols, Nx = P.Nx, Ny = P.Ny, nul = P.nul;
     const long long npx = (long long)rows * cols;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long nout = (long long)(P.row_end - P.row_begin) * cols;
-    if (t >= nout) return;
-    const int r = P.row_begin + (int)(t / cols);
+    const int h0 = fwd_row0(P);
+    if (t >= (long long)(P.row_end - h0) * cols) return;
+    const int r = h0 + (int)(t / cols);
     const int c = (int)(t % cols);
     const long long p = (long long)r * cols + c;
-    uint32_t* out = wts + p * nul;
+    const int centre = centre_flat(P), fw = centre >> 5, nf = nul - fw;
+    uint32_t* out = F + t * nf;   // word w of the mask at out[w - fw]
     const bool pv = valid[p] != 0;
     uint32_t A[NP];
 #pragma unroll
     for (int i = 0; i < NP; ++i) A[i] = (pv && i < n) ? (keys[i * npx + p] << 1) : 0xFFFFFFFEu;
-    int cnt = 0, word = 0, flat = 0;
+    int cnt = 0, word = fw, flat = centre;
     uint32_t cur = 0u;
-    for (int dy = -Ny; dy <= Ny; ++dy) {
+    for (int dy = 0; dy <= Ny; ++dy) {
         const int rr = r + dy;
-        for (int dx = -Nx; dx <= Nx; ++dx, ++flat) {
+        for (int dx = (dy == 0 ? 0 : -Nx); dx <= Nx; ++dx, ++flat) {
             if ((flat >> 5) != word) {
-                out[word] = cur;
+                out[word - fw] = cur;
                 cur = 0u;
                 word = flat >> 5;
             }
             const int cc = c + dx;
-            if (!pv || rr < 0 || rr >= rows || cc < 0 || cc >= cols) continue;
+            if (!pv || rr >= rows || cc < 0 || cc >= cols) continue;
             const long long q = (long long)rr * cols + cc;
             if (!valid[q]) continue;
             bool similar = true;
             if (q != p) {
-                // own tag: KS2 0; AD2 1 when the pixel is the test's first sample
-                const bool fwd = dy > 0 || (dy == 0 && dx > 0);
-                const uint32_t tp = (AD2 && fwd) ? 1u : 0u, tq = tp ^ 1u;
+                // own tag: KS2 0; AD2 1 (the pixel is the test's first sample)
+                const uint32_t tp = AD2 ? 1u : 0u, tq = tp ^ 1u;
                 uint32_t S[2 * NP];
 #pragma unroll
                 for (int i = 0; i < NP; ++i) {
@@ -261,9 +274,9 @@ __global__ __launch_bounds__(256) void nmap_net(const NmapPlan P, uint32_t* __re
             }
         }
     }
-    out[word] = cur;
-    for (int w = word + 1; w < nul; ++w) out[w] = 0u;
-    count[p] = cnt;
+    out[word - fw] = cur;
+    for (int w = word + 1; w < nul; ++w) out[w - fw] = 0u;
+    fcnt[t] = cnt;
 }
 
 // n > 64: one thread per output pixel walks the sorted arrays in memory.
@@ -271,44 +284,44 @@ __global__ __launch_bounds__(256) void nmap_net(const NmapPlan P, uint32_t* __re
 // q = p + (dy, dx), dy > 0 or dy = 0 < dx (nmap.cpp:402-457); the other half
 // of a pixel's window is therefore tested with the neighbour first.
 template <bool AD2>
-__global__ __launch_bounds__(256) void nmap_sweep(const NmapPlan P, uint32_t* __restrict__ wts,
-                                                  int32_t* __restrict__ count) {
+__global__ __launch_bounds__(256) void nmap_sweep(const NmapPlan P, uint32_t* __restrict__ F,
+                                                  int32_t* __restrict__ fcnt) {
     const float* __restrict__ amp_sorted = P.d_amp;
     const uint8_t* __restrict__ valid = P.d_valid;
     const int n = P.n, rows = P.rows, cols = P.cols, Nx = P.Nx, Ny = P.Ny, nul = P.nul;
     const long long npx = (long long)rows * cols;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long nout = (long long)(P.row_end - P.row_begin) * cols;
-    if (t >= nout) return;
-    const int r = P.row_begin + (int)(t / cols);
+    const int h0 = fwd_row0(P);
+    if (t >= (long long)(P.row_end - h0) * cols) return;
+    const int r = h0 + (int)(t / cols);
     const int c = (int)(t % cols);
     const long long p = (long long)r * cols + c;
-    uint32_t* out = wts + p * nul;
+    const int centre = centre_flat(P), fw = centre >> 5, nf = nul - fw;
+    uint32_t* out = F + t * nf;   // word w of the mask at out[w - fw]
     const bool pv = valid[p] != 0;
     const float* a = amp_sorted + p;
-    // Window positions are visited in bit order (flat = 0, 1, 2, ...), so
-    // the word being filled only ever advances by one; each finished word is
-    // stored once.
-    int cnt = 0, word = 0, flat = 0;
+    // Window positions are visited in bit order (flat = centre, centre + 1,
+    // ...), so the word being filled only ever advances by one; each
+    // finished word is stored once.
+    int cnt = 0, word = fw, flat = centre;
     uint32_t cur = 0u;
-    for (int dy = -Ny; dy <= Ny; ++dy) {
+    for (int dy = 0; dy <= Ny; ++dy) {
         const int rr = r + dy;
-        for (int dx = -Nx; dx <= Nx; ++dx, ++flat) {
+        for (int dx = (dy == 0 ? 0 : -Nx); dx <= Nx; ++dx, ++flat) {
             if ((flat >> 5) != word) {
-                out[word] = cur;
+                out[word - fw] = cur;
                 cur = 0u;
                 word = flat >> 5;
             }
             const int cc = c + dx;
-            if (!pv || rr < 0 || rr >= rows || cc < 0 || cc >= cols) continue;
+            if (!pv || rr >= rows || cc < 0 || cc >= cols) continue;
             const long long q = (long long)rr * cols + cc;
             if (!valid[q]) continue;
             bool similar = true;
             if (q != p) {
                 const float* b = amp_sorted + q;
-                if constexpr (AD2) {
-                    const bool fwd = dy > 0 || (dy == 0 && dx > 0);
-                    const double S = ad2_inner(fwd ? a : b, fwd ? b : a, n, npx, P.d_ad2);
+                if constexpr (AD2) {   // the pixel is the test's first sample
+                    const double S = ad2_inner(a, b, n, npx, P.d_ad2);
                     similar = ad2_standardise(S, n, P.ad2_sigma) <= P.ad2_max;
                 } else {
                     similar = ks_similar(ks_count(a, b, n, npx), P);
@@ -320,10 +333,60 @@ __global__ __launch_bounds__(256) void nmap_sweep(const NmapPlan P, uint32_t* __
             }
         }
     }
-    out[word] = cur;
-    for (int w = word + 1; w < nul; ++w) out[w] = 0u;
-    count[p] = cnt;
+    out[word - fw] = cur;
+    for (int w = word + 1; w < nul; ++w) out[w - fw] = 0u;
+    fcnt[t] = cnt;
 }
+
+// Full masks of the output rows.  Backward position flat (< centre) of
+// pixel p holds the forward bit W-1-flat of q = p + (dy, dx), the pair's
+// one test; q lies in rows [fwd_row0, row_end) whenever it is in the image.
+__global__ __launch_bounds__(256) void nmap_mirror(const NmapPlan P, const uint32_t* __restrict__ F,
+                                                   const int32_t* __restrict__ fcnt, uint32_t* __restrict__ wts,
+                                                   int32_t* __restrict__ count) {
+    const int cols = P.cols, Nx = P.Nx, Ny = P.Ny, nul = P.nul;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)(P.row_end - P.row_begin) * cols) return;
+    const int r = P.row_begin + (int)(t / cols);
+    const int c = (int)(t % cols);
+    const int h0 = fwd_row0(P);
+    const int centre = centre_flat(P), fw = centre >> 5, nf = nul - fw;
+    const int last = (2 * Ny + 1) * (2 * Nx + 1) - 1;
+    const long long fp = (long long)(r - h0) * cols + c;
+    uint32_t* out = wts + ((long long)r * cols + c) * nul;
+    int cnt = fcnt[fp];
+    int word = 0, flat = 0;
+    uint32_t cur = 0u;
+    for (int dy = -Ny; dy <= 0; ++dy) {
+        const int rr = r + dy;
+        for (int dx = -Nx; dx <= (dy == 0 ? -1 : Nx); ++dx, ++flat) {
+            if ((flat >> 5) != word) {
+                out[word] = cur;
+                cur = 0u;
+                word = flat >> 5;
+            }
+            const int cc = c + dx;
+            if (rr < 0 || cc < 0 || cc >= cols) continue;
+            const int mf = last - flat;
+            const uint32_t qw = F[((long long)(rr - h0) * cols + cc) * nf + (mf >> 5) - fw];
+            if ((qw >> (mf & 31)) & 1u) {
+                cur |= (1u << (flat & 31));
+                ++cnt;
+            }
+        }
+    }
+    // flat == centre: the rest comes from the pixel's own forward words
+    if (fw != word) {
+        out[word] = cur;
+        cur = 0u;
+    }
+    const uint32_t* mine = F + fp * nf;
+    out[fw] = cur | mine[0];
+    for (int i = 1; i < nf; ++i) out[fw + i] = mine[i];
+    count[(long long)r * cols + c] = cnt;
+}
+
+int nmap_fwd_words(int Nx, int Ny) { return shp_words(Nx, Ny) - ((Ny * (2 * Nx + 1) + Nx) >> 5); }
 
 template <int NP>
 static void launch_prep(const NmapPlan& P, const float2* slc, const uint8_t* mask, hipStream_t s) {
@@ -334,10 +397,9 @@ static void launch_prep(const NmapPlan& P, const float2* slc, const uint8_t* mas
 }
 
 template <int NP>
-static void launch_net(bool ad2, dim3 g, dim3 blk, const NmapPlan& P, uint32_t* wts, int32_t* count,
-                       hipStream_t s) {
-    if (ad2) hipLaunchKernelGGL((nmap_net<NP, true>), g, blk, 0, s, P, wts, count);
-    else hipLaunchKernelGGL((nmap_net<NP, false>), g, blk, 0, s, P, wts, count);
+static void launch_net(bool ad2, dim3 g, dim3 blk, const NmapPlan& P, hipStream_t s) {
+    if (ad2) hipLaunchKernelGGL((nmap_net<NP, true>), g, blk, 0, s, P, P.d_fwd, P.d_fcnt);
+    else hipLaunchKernelGGL((nmap_net<NP, false>), g, blk, 0, s, P, P.d_fwd, P.d_fcnt);
 }
 
 int nmap_launch(const NmapPlan& P, const float2* slc, const uint8_t* mask, uint32_t* wts,
@@ -353,20 +415,24 @@ int nmap_launch(const NmapPlan& P, const float2* slc, const uint8_t* mask, uint3
     const long long nout = (long long)(P.row_end - P.row_begin) * P.cols;
     if (nout > 0) {
         const int threads = 256;
-        const unsigned blocks = (unsigned)((nout + threads - 1) / threads);
+        // forward halves of the output rows and of the Ny rows above them
+        const int h0 = P.row_begin - P.Ny > 0 ? P.row_begin - P.Ny : 0;
+        const long long nfwd = (long long)(P.row_end - h0) * P.cols;
         StageScope sc(STAGE_NMAP_KS, s);
         if (P.method == 1) {
             const int nt = (2 * P.n - 1) * (P.n + 1);
             hipLaunchKernelGGL(ad2_terms, dim3((nt + 255) / 256), dim3(256), 0, s, P.d_ad2, P.n);
         }
         const bool ad2 = P.method == 1;
-        const dim3 g(blocks), blk(threads);
-        if (P.n <= 8) launch_net<8>(ad2, g, blk, P, wts, count, s);
-        else if (P.n <= 16) launch_net<16>(ad2, g, blk, P, wts, count, s);
-        else if (P.n <= 32) launch_net<32>(ad2, g, blk, P, wts, count, s);
-        else if (P.n <= 64) launch_net<64>(ad2, g, blk, P, wts, count, s);
-        else if (ad2) hipLaunchKernelGGL(nmap_sweep<true>, g, blk, 0, s, P, wts, count);
-        else hipLaunchKernelGGL(nmap_sweep<false>, g, blk, 0, s, P, wts, count);
+        const dim3 g((unsigned)((nfwd + threads - 1) / threads)), blk(threads);
+        if (P.n <= 8) launch_net<8>(ad2, g, blk, P, s);
+        else if (P.n <= 16) launch_net<16>(ad2, g, blk, P, s);
+        else if (P.n <= 32) launch_net<32>(ad2, g, blk, P, s);
+        else if (P.n <= 64) launch_net<64>(ad2, g, blk, P, s);
+        else if (ad2) hipLaunchKernelGGL(nmap_sweep<true>, g, blk, 0, s, P, P.d_fwd, P.d_fcnt);
+        else hipLaunchKernelGGL(nmap_sweep<false>, g, blk, 0, s, P, P.d_fwd, P.d_fcnt);
+        hipLaunchKernelGGL(nmap_mirror, dim3((unsigned)((nout + threads - 1) / threads)), dim3(threads), 0, s,
+                           P, (const uint32_t*)P.d_fwd, (const int32_t*)P.d_fcnt, wts, count);
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

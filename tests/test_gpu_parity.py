@@ -77,6 +77,29 @@ def test_nmap_bit_exact(n, method):
     assert np.array_equal(cg, cr)
 
 
+@pytest.mark.parametrize("method", ["KS2", "AD2"])
+@pytest.mark.parametrize("n", [30, 72])
+def test_nmap_row_range(n, method):
+    """A call over rows [row_begin, row_end) gives those rows of the
+    full-tile map (their backward neighbours above row_begin included) and
+    leaves every other row of the output untouched."""
+    rows, cols, Nx, Ny = 24, 26, 4, 3
+    slc = synthetic.stack(n, rows, cols, seed=300 + n, patches=6)
+    wr, cr = oracle.nmap(slc, Nx, Ny, 0.05, method=method)
+    d = to_dev(slc, torch)
+    for rb, re in ((0, 5), (7, 16), (20, 24), (2, 3), (10, 10)):
+        print(f"case nmap rows [{rb}, {re}) n={n} {method}", flush=True)
+        wg, cg = tile.nmap(d, Nx, Ny, 0.05, method=method, row_begin=rb, row_end=re)
+        torch.cuda.synchronize()
+        wg = wg.cpu().numpy().view(np.uint32)
+        cg = cg.cpu().numpy()
+        assert np.array_equal(wg[rb:re], wr[rb:re])
+        assert np.array_equal(cg[rb:re], cr[rb:re])
+        outside = np.ones(rows, bool)
+        outside[rb:re] = False
+        assert not wg[outside].any() and not cg[outside].any()
+
+
 def _full_rank_gap(gap, rows, cols, Nx, Ny, n, precision):
     """For the float-Gamma precisions: gap with pixels whose (full-mask)
     window holds fewer than n samples marked ill-posed.  Their sample

@@ -147,9 +147,13 @@ def test_server_request_bytes_and_create_without_device():
     words = L.fringe_mask_words(7, 5)
     npx = 64 * 80
     # the server-computed mask and count are kept with the outputs, and its
-    # nmap workspace holds N float amplitudes and a validity byte per pixel
+    # nmap workspace holds N float amplitudes and a validity byte per pixel,
+    # then the forward half of each mask (the centre's word on: position
+    # 5 * 15 + 7 = 82 is in word 2, so words 2..5) and a forward count
+    fwd_words = words - (5 * 15 + 7) // 32
     extra = (npx * words * 4 + 255) // 256 * 256 + (npx * 4 + 255) // 256 * 256
-    extra += ((npx * 30 * 4 + 255) // 256 * 256 + (npx + 255) // 256 * 256 + 255) // 256 * 256
+    extra += ((npx * 30 * 4 + 255) // 256 * 256 + (npx + 255) // 256 * 256
+              + (npx * fwd_words * 4 + 255) // 256 * 256 + (npx * 4 + 255) // 256 * 256 + 255) // 256 * 256
     assert with_nmap - given == extra
     outputs = sum((b + 255) // 256 * 256 for b in (npx * 30 * 8, npx * 4, npx * 8))
     assert given >= outputs + L.fringe_link_workspace_bytes(ctypes.byref(t))

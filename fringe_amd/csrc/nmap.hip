@@ -255,15 +255,19 @@ __global__ __launch_bounds__(256) void nmap_net(const NmapPlan P, uint32_t* __re
                     }
                     similar = ad2_standardise(Ssum, n, P.ad2_sigma) <= P.ad2_max;
                 } else {
-                    // d_j = 2 (#tag-1 in the first j+1) - (j+1)
+                    // d_j = 2 (#tag-1 in the first j+1) - (j+1), taken where
+                    // the amplitude changes (the end of a run of ties).
+                    // Padding keys sort above every amplitude and differ from
+                    // one another only in the tag, so no end falls among them,
+                    // and at the last amplitude d = 0: j needs no bound.
                     int nb = 0, hi = 0, lo = 0;
 #pragma unroll
                     for (int j = 0; j < 2 * NP - 1; ++j) {
                         nb += (int)(S[j] & 1u);
-                        const int d = 2 * nb - (j + 1);
-                        const bool end = ((S[j] ^ S[j + 1]) > 1u) && (j < 2 * n);
-                        hi = end ? max(hi, d) : hi;
-                        lo = end ? min(lo, d) : lo;
+                        const bool end = (S[j] ^ S[j + 1]) > 1u;
+                        const int x = end ? 2 * nb - (j + 1) : 0;
+                        hi = max(hi, x);
+                        lo = min(lo, x);
                     }
                     similar = ks_similar(max(hi, -lo), P);
                 }

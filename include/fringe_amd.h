@@ -149,7 +149,10 @@ int fringe_coherence_run(const fringe_link_tile* t, const void* d_slc, const uin
  * d_mask (optional, uint8 [row][col]) marks usable pixels.  h_amp_const holds
  * the per-band amplitudeConstant metadata (<= 0 or NULL: missing), normalised
  * by band 1 as nmap.cpp:204-233 does.  Output rows [row_begin, row_end) of
- * d_wts[row][col][nulong] and d_count[row][col] (int32) are written. */
+ * d_wts[row][col][nulong] and d_count[row][col] (int32) are written, and no
+ * other rows of them; every row of d_slc within Ny of the output rows is
+ * read.  Each pixel pair is tested once (the reference sweep's order), in a
+ * pass that writes the workspace, and a second pass assembles the masks. */
 typedef struct fringe_nmap_tile {
     int rows, cols, nbands;
     int Nx, Ny;

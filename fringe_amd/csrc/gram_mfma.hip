@@ -70,14 +70,31 @@ __global__ void date_absmax_clear(uint32_t* dmax, int n) {
 __global__ __launch_bounds__(256) void date_absmax(const float2* __restrict__ slc, long long npx,
                                                    uint32_t* __restrict__ dmax) {
     const int t = blockIdx.y;
+    const float2* __restrict__ band = slc + (long long)t * npx;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     float m = 0.0f;
-    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npx;
-         p += (long long)gridDim.x * blockDim.x) {
-        const float2 z = slc[(long long)t * npx + p];
+    // four independent loads in flight per thread (the maximum does not
+    // depend on the order, and fmaxf drops NaN in any order)
+    for (; p + 3 * stride < npx; p += 4 * stride) {
+        float2 z[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) z[k] = band[p + k * stride];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) m = fmaxf(m, fmaxf(fabsf(z[k].x), fabsf(z[k].y)));
+    }
+    for (; p < npx; p += stride) {
+        const float2 z = band[p];
         m = fmaxf(m, fmaxf(fabsf(z.x), fabsf(z.y)));
     }
     for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-    if ((threadIdx.x & 63) == 0) atomicMax(dmax + t, __float_as_uint(m));
+    // one atomic per workgroup: every date's word shares a cache line, and
+    // atomics on one line serialise
+    __shared__ float wm[4];
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicMax(dmax + t, __float_as_uint(fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]))));
 }
 // Power-of-two exponent k with max * 2^k in [2^14, 2^15); 0 when the date is
 // all zero or not finite.
@@ -573,7 +590,7 @@ int pack_launch(const float2* slc, int rows, int cols, int n, bool f16, uint32_t
     const unsigned blocks = (unsigned)((npx + 63) / 64);
     if (f16) {
         hipLaunchKernelGGL(date_absmax_clear, dim3((n + 63) / 64), dim3(64), 0, s, dmax, n);
-        const unsigned bx = (unsigned)std::min<long long>((npx + 255) / 256, 1024);
+        const unsigned bx = (unsigned)std::min<long long>((npx + 255) / 256, 64);   // 64 workgroups per date
         hipLaunchKernelGGL(date_absmax, dim3(bx, n), dim3(256), 0, s, slc, npx, dmax);
         hipLaunchKernelGGL(pack_hilo<true>, dim3(blocks), dim3(256), (size_t)64 * 2 * n * 4, s, slc, n,
                            npx, (const uint32_t*)dmax, pk);

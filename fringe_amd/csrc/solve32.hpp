@@ -35,6 +35,7 @@ template <typename T, int NR> struct LdsR {
     T D[NR], E[NR], E2[NR];
     T B[NR];          // inverse-iteration right-hand side
     T FP[NR], FQ[NR], FS[NR], FX[NR];   // eliminated rows
+    T FF[NR], FK[NR];  // multiplier of elimination step i, 1 where its pivot was kept
     double TR[NR];    // sweep: published pivot row
     double PV;        // sweep: pivot
     double XT[NR / 8][NR];   // sweep: one block column on its way to the row layout
@@ -375,45 +376,62 @@ __device__ __forceinline__ T bisect(LdsR<T, NR>& L, int n, int k, int r, T& tnor
 // ---- eigenvector of tridiag(D, E) for lam: 2 rounds of inverse iteration --
 // Every lane of the half runs the same elimination chain on broadcast
 // operands; lane 0 records the eliminated rows for the back substitution,
-// and lane i keeps component i of the result.
+// and lane i keeps component i of the result.  The elimination of
+// T - lam I (pivot choices, multipliers, eliminated rows) is the same in
+// both rounds, so the second round only carries the new right-hand side
+// through the recorded steps.
 template <typename T, int NR>
 __device__ __forceinline__ T tri_eigvec(LdsR<T, NR>& L, int n, int r, T lam, T tnorm) {
     const T pert = max(Lim<T>::eps() * tnorm, Lim<T>::tiny());
     L.B[r] = T(0.5) + T((r * 37 + 11) & 63) * T(1.0 / 64.0);
     wsync();
     T xme = T(0);
+    T cd = L.D[0] - lam;   // last pivot of the elimination, kept for round 2
     for (int round = 0; round < 2; ++round) {
-        T cd = L.D[0] - lam;
-        T cu = (n > 1) ? L.E[0] : T(0);
         T cx_ = L.B[0];
-        for (int i = 0; i < n - 1; ++i) {
-            const T e = L.E[i];
-            const T dn = L.D[i + 1] - lam;
-            const T un = (i + 1 < n - 1) ? L.E[i + 1] : T(0);
-            const T bn = L.B[i + 1];
-            const bool keep = fabs(cd) >= fabs(e);
-            T pv, qv, sv, xv, ncd, ncu, ncx;
-            if (keep) {
-                const T cdp = (fabs(cd) < pert) ? (cd < T(0) ? -pert : pert) : cd;
-                const T f = e * fast_rcp(cdp);
-                pv = cdp; qv = cu; sv = T(0); xv = cx_;
-                ncd = dn - f * cu; ncu = un; ncx = bn - f * cx_;
-            } else {
-                const T f = cd * fast_rcp(e);
-                pv = e; qv = dn; sv = un; xv = bn;
-                ncd = cu - f * dn; ncu = -f * un; ncx = cx_ - f * bn;
+        if (round == 0) {
+            T cu = (n > 1) ? L.E[0] : T(0);
+            for (int i = 0; i < n - 1; ++i) {
+                const T e = L.E[i];
+                const T dn = L.D[i + 1] - lam;
+                const T un = (i + 1 < n - 1) ? L.E[i + 1] : T(0);
+                const T bn = L.B[i + 1];
+                const bool keep = fabs(cd) >= fabs(e);
+                T pv, qv, sv, xv, ncd, ncu, ncx, f;
+                if (keep) {
+                    const T cdp = (fabs(cd) < pert) ? (cd < T(0) ? -pert : pert) : cd;
+                    f = e * fast_rcp(cdp);
+                    pv = cdp; qv = cu; sv = T(0); xv = cx_;
+                    ncd = dn - f * cu; ncu = un; ncx = bn - f * cx_;
+                } else {
+                    f = cd * fast_rcp(e);
+                    pv = e; qv = dn; sv = un; xv = bn;
+                    ncd = cu - f * dn; ncu = -f * un; ncx = cx_ - f * bn;
+                }
+                if (r == 0) {
+                    L.FP[i] = pv;
+                    L.FQ[i] = qv;
+                    L.FS[i] = sv;
+                    L.FX[i] = xv;
+                    L.FF[i] = f;
+                    L.FK[i] = keep ? T(1) : T(0);
+                }
+                cd = ncd;
+                cu = ncu;
+                cx_ = ncx;
             }
-            if (r == 0) {
-                L.FP[i] = pv;
-                L.FQ[i] = qv;
-                L.FS[i] = sv;
-                L.FX[i] = xv;
+            if (fabs(cd) < pert) cd = (cd < T(0)) ? -pert : pert;
+        } else {
+            for (int i = 0; i < n - 1; ++i) {
+                const T bn = L.B[i + 1];
+                const T f = L.FF[i];
+                const bool keep = L.FK[i] != T(0);
+                const T xv = keep ? cx_ : bn;
+                const T ncx = keep ? bn - f * cx_ : cx_ - f * bn;
+                if (r == 0) L.FX[i] = xv;
+                cx_ = ncx;
             }
-            cd = ncd;
-            cu = ncu;
-            cx_ = ncx;
         }
-        if (fabs(cd) < pert) cd = (cd < T(0)) ? -pert : pert;
         wsync();
         T x1 = cx_ * fast_rcp(cd);   // x_{n-1}
         T x2 = T(0);                 // x_{i+2}

@@ -74,19 +74,22 @@ __global__ __launch_bounds__(256) void date_absmax(const float2* __restrict__ sl
     const long long stride = (long long)gridDim.x * blockDim.x;
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     float m = 0.0f;
-    // four independent loads in flight per thread (the maximum does not
-    // depend on the order, and fmaxf drops NaN in any order)
+    // Largest finite |Re|, |Im|: an infinite or NaN sample makes only the
+    // coherence of the windows that hold it non-finite (as in FP64), and must
+    // not take the scale of the whole date with it.  Four independent loads
+    // in flight per thread; the maximum does not depend on the order.
+    auto fin_abs = [](float2 z) {
+        const float a = fmaxf(fabsf(z.x), fabsf(z.y));   // NaN parts dropped
+        return (a < INFINITY && z.x == z.x && z.y == z.y) ? a : 0.0f;
+    };
     for (; p + 3 * stride < npx; p += 4 * stride) {
         float2 z[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) z[k] = band[p + k * stride];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) m = fmaxf(m, fmaxf(fabsf(z[k].x), fabsf(z[k].y)));
+        for (int k = 0; k < 4; ++k) m = fmaxf(m, fin_abs(z[k]));
     }
-    for (; p < npx; p += stride) {
-        const float2 z = band[p];
-        m = fmaxf(m, fmaxf(fabsf(z.x), fabsf(z.y)));
-    }
+    for (; p < npx; p += stride) m = fmaxf(m, fin_abs(band[p]));
     for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
     // one atomic per workgroup: every date's word shares a cache line, and
     // atomics on one line serialise

@@ -213,6 +213,38 @@ def test_link_failure_codes_and_untouched(precision):
     assert np.all(g2[1] == 0) and np.all(r2["tcorr"] == 0)
 
 
+@pytest.mark.parametrize("module,method", [("phase_link", "MLE"), ("evd", "EVD")])
+def test_mixed_scale_ignores_nonfinite_samples(module, method):
+    """An infinite and a NaN sample in a stack of amplitudes far beyond the
+    fp16 range: the mixed path's per-date scale comes from the finite
+    samples, so exactly the pixels whose window holds a bad sample fail, with
+    the plain FP64 path's code, and every other pixel agrees with FP64."""
+    n, rows, cols, Nx, Ny = 10, 20, 24, 2, 1
+    print(f"case non-finite samples {module}/{method}", flush=True)
+    slc = (synthetic.stack(n, rows, cols, seed=130, patches=4) * np.float32(1.0e6)).astype(np.complex64)
+    bad_at = ((2, 4, 5), (7, 14, 17))   # (date, row, col)
+    slc[2, 4, 5] = complex(np.inf, 0.0)
+    slc[7, 14, 17] = complex(np.nan, 1.0)
+    wts = synthetic.full_mask(rows, cols, Nx, Ny)
+    out = {}
+    for precision in ("fp64", "mixed"):
+        ph, tc, co = tile.link(to_dev(slc, torch), wts_dev(wts, torch), Nx, Ny, module=module, method=method,
+                               min_neighbors=2, precision=precision)
+        torch.cuda.synchronize()
+        out[precision] = (ph.cpu().numpy(), tc.cpu().numpy())
+    bad = np.zeros((rows, cols), bool)
+    for _, r0, c0 in bad_at:
+        bad[max(r0 - Ny, 0):r0 + Ny + 1, max(c0 - Nx, 0):c0 + Nx + 1] = True
+    (p64, t64), (p16, t16) = out["fp64"], out["mixed"]
+    assert np.array_equal(t64 < 0, bad) and np.array_equal(t16 < 0, bad)
+    assert np.array_equal(t64[bad], t16[bad])
+    good = ~bad
+    dphi = np.abs(np.angle(p16[:, good] * np.conj(p64[:, good]))).max(axis=0)
+    # near-degenerate pixels aside, the two paths agree to the usual bounds
+    assert np.mean(dphi <= PHASE_TOL) >= 0.9
+    assert np.mean(np.abs(t16[good] - t64[good]) <= TCORR_TOL) >= 0.9
+
+
 def test_mixed_precision_tcorr_and_phase_at_cfg1_scale():
     # N=10, 64x64 crop of the 256^2 EVD workload (all-ones mask, 11x11 window)
     n, rows, cols, Nx, Ny = 10, 64, 64, 5, 5

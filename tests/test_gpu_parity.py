@@ -183,6 +183,28 @@ def test_link_generic_solver_n72(precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("module,method", [("phase_link", "MLE"), ("evd", "EVD"), ("evd", "MLE")])
+@pytest.mark.parametrize("n", [72, 96])
+def test_link_generic_solver_modes(n, module, method, precision):
+    # every estimator of the LDS-resident solver above 64 dates; a 15 x 11
+    # window holds 165 samples in the interior (6 x 6 pixels), fewer at the
+    # edges, where the sample coherence of 96 dates can be rank deficient
+    rows, cols, Nx, Ny = 16, 20, 7, 5
+    print(f"case generic n={n} {module}/{method} {precision}", flush=True)
+    slc = synthetic.stack(n, rows, cols, seed=400 + n)
+    wts = synthetic.full_mask(rows, cols, Nx, Ny)
+    g, r = _run_both(slc, wts, Nx, Ny, module, method, precision)
+    st = compare_link(g, r, _full_rank_gap(r["gap"], rows, cols, Nx, Ny, n, precision),
+                      vmin=None if precision == "fp64" else r["vmin"])
+    print(st)
+    assert st["untouched_equal"]
+    if precision == "fp64":
+        assert st["codes_equal"]
+    assert st["n_well"] >= (1 if (module, method) == ("evd", "MLE") else 36)
+    assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_link_stbas_and_reference_index(precision):
     n, rows, cols, Nx, Ny = 12, 14, 16, 3, 2
     print(f"case stbas {precision}", flush=True)

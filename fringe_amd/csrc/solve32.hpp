@@ -386,7 +386,7 @@ __device__ __forceinline__ T tri_eigvec(LdsR<T, NR>& L, int n, int r, T lam, T t
     L.B[r] = T(0.5) + T((r * 37 + 11) & 63) * T(1.0 / 64.0);
     wsync();
     T xme = T(0);
-    T cd = L.D[0] - lam;   // last pivot of the elimination, kept for round 2
+    T cd = L.D[0] - lam;   // ends as the last pivot, reused by the second pass
     for (int round = 0; round < 2; ++round) {
         T cx_ = L.B[0];
         if (round == 0) {

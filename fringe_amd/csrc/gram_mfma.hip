@@ -350,8 +350,9 @@ __global__ __launch_bounds__(64 * WAVES) void gram_mfma(MfmaArgs a) {
 // Workgroups are numbered so that each XCD (which owns an L2) takes a
 // contiguous run of tiles and the halos neighbouring tiles share stay in it.
 // ---------------------------------------------------------------------------
-template <bool F16>
-__global__ __launch_bounds__(256) void gram_mfma_lds(MfmaArgs a) {
+template <bool F16, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void gram_mfma_lds(MfmaArgs a) {
+    constexpr int NT = 64 * WAVES;
     typedef typename Frag<F16>::type frag_t;
     extern __shared__ __align__(16) char smem[];
     const int n = a.n, Nx = a.Nx, Ny = a.Ny;
@@ -379,12 +380,12 @@ __global__ __launch_bounds__(256) void gram_mfma_lds(MfmaArgs a) {
     {
         const int total = (NPOS + 1) * 32;
         const uint2* src = (const uint2*)a.pk;
-        for (int e0 = threadIdx.x; e0 < total; e0 += 8 * 256) {
+        for (int e0 = threadIdx.x; e0 < total; e0 += 8 * NT) {
             uint2 v[8];
             bool in[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                const int e = e0 + u * 256;
+                const int e = e0 + u * NT;
                 const int pos = e >> 5, d = e & 31;
                 const int rr = rbase + pos / WC, cc = cbase + pos % WC;
                 in[u] = e < total && d < n && pos < NPOS && rr >= 0 && rr < a.rows && cc >= 0 && cc < a.cols;
@@ -394,7 +395,7 @@ __global__ __launch_bounds__(256) void gram_mfma_lds(MfmaArgs a) {
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                const int e = e0 + u * 256;
+                const int e = e0 + u * NT;
                 if (e < total) rec[e] = in[u] ? v[u] : make_uint2(0u, 0u);
             }
         }
@@ -425,9 +426,9 @@ __global__ __launch_bounds__(256) void gram_mfma_lds(MfmaArgs a) {
         return a.wts[px * a.nul + (lane < a.nul ? lane : 0)];
     };
     wbuf[lane] = mask_word(wave);
-    for (int pid = wave; pid < TH * TW; pid += 4) {
+    for (int pid = wave; pid < TH * TW; pid += WAVES) {
         const uint32_t wword = wbuf[lane];
-        const uint32_t wnext = mask_word(pid + 4);
+        const uint32_t wnext = mask_word(pid + WAVES);
         const int r = R0 + pid / TW, c = C0 + pid % TW;
         if (r >= row_end || r >= a.rows || c >= a.cols) {   // wave-uniform
             wbuf[lane] = wnext;
@@ -551,10 +552,10 @@ __global__ __launch_bounds__(256) void gram_mfma_lds(MfmaArgs a) {
     }
 }
 
-static size_t gram_lds_bytes(int Nx, int Ny, int TH, int TW, int list_cap) {
+static size_t gram_lds_bytes(int Nx, int Ny, int TH, int TW, int list_cap, int waves) {
     // records are staged padded to 32 dates
     const size_t stage = (size_t)((TH + 2 * Ny) * (TW + 2 * Nx) + 1) * 32 * 8;
-    return stage + (size_t)4 * (list_cap + 32 + 64) * 4;
+    return stage + (size_t)waves * (list_cap + 32 + 64) * 4;
 }
 
 static int gram_list_cap(int Nx, int Ny) {
@@ -562,22 +563,30 @@ static int gram_list_cap(int Nx, int Ny) {
     return ((Wn + 15) / 16) * 16 + 16;
 }
 
-// largest TH x TW tile (TH <= 4, TW <= 8) whose stage keeps two workgroups
-// on a CU; returns the stage's LDS bytes
-static size_t gram_stage_tile(int Nx, int Ny, int& TH, int& TW) {
-    const size_t budget = 78 * 1024;
+// Tile of the staged kernel.  First choice: a 4 x 16 tile for eight waves,
+// one workgroup per CU, which re-stages ~6.6 halo records per pixel at a
+// 15 x 11 window (a 3 x 8 tile, ~11.9).  Otherwise the largest TH x TW tile
+// (TH <= 4, TW <= 8) for four waves whose stage keeps two workgroups on a
+// CU.  Returns the stage's LDS bytes.
+static size_t gram_stage_tile(int Nx, int Ny, int& TH, int& TW, int& waves) {
     const int cap = gram_list_cap(Nx, Ny);
+    waves = 8;
+    TH = 4;
+    TW = 16;
+    if (gram_lds_bytes(Nx, Ny, TH, TW, cap, waves) <= 150 * 1024) return gram_lds_bytes(Nx, Ny, TH, TW, cap, waves);
+    const size_t budget = 78 * 1024;
+    waves = 4;
     TH = 4;
     TW = 8;
-    while (TH > 1 && gram_lds_bytes(Nx, Ny, TH, TW, cap) > budget) --TH;
-    while (TW > 1 && gram_lds_bytes(Nx, Ny, TH, TW, cap) > budget) TW >>= 1;
-    return gram_lds_bytes(Nx, Ny, TH, TW, cap);
+    while (TH > 1 && gram_lds_bytes(Nx, Ny, TH, TW, cap, waves) > budget) --TH;
+    while (TW > 1 && gram_lds_bytes(Nx, Ny, TH, TW, cap, waves) > budget) TW >>= 1;
+    return gram_lds_bytes(Nx, Ny, TH, TW, cap, waves);
 }
 
 int gram_stage_rows(int n, int Nx, int Ny) {
     if (n > 32) return 1;
-    int TH, TW;
-    return gram_stage_tile(Nx, Ny, TH, TW) <= 160 * 1024 ? TH : 1;
+    int TH, TW, waves;
+    return gram_stage_tile(Nx, Ny, TH, TW, waves) <= 160 * 1024 ? TH : 1;
 }
 
 // packed records followed by the per-date maxima (256-byte aligned)
@@ -634,14 +643,20 @@ int gram_mfma_launch(const LinkPlan& P, const uint32_t* pk, const uint32_t* wts,
     a.list_cap = gram_list_cap(P.Nx, P.Ny);
     const bool f16 = (P.precision != FRINGE_PREC_MIXED_BF16);
     if (P.n <= 32) {
-        int TH, TW;
-        const size_t bytes = gram_stage_tile(P.Nx, P.Ny, TH, TW);
+        int TH, TW, waves;
+        const size_t bytes = gram_stage_tile(P.Nx, P.Ny, TH, TW, waves);
         if (bytes <= 160 * 1024) {
             a.TH = TH;
             a.TW = TW;
             const dim3 grid((unsigned)((a.cols + TW - 1) / TW), (unsigned)((a.chunk_rows + TH - 1) / TH));
-            if (f16) hipLaunchKernelGGL(gram_mfma_lds<true>, grid, dim3(256), bytes, s, a);
-            else hipLaunchKernelGGL(gram_mfma_lds<false>, grid, dim3(256), bytes, s, a);
+            const dim3 blk(64 * waves);
+            if (waves == 8) {
+                if (f16) hipLaunchKernelGGL((gram_mfma_lds<true, 8>), grid, blk, bytes, s, a);
+                else hipLaunchKernelGGL((gram_mfma_lds<false, 8>), grid, blk, bytes, s, a);
+            } else {
+                if (f16) hipLaunchKernelGGL((gram_mfma_lds<true, 4>), grid, blk, bytes, s, a);
+                else hipLaunchKernelGGL((gram_mfma_lds<false, 4>), grid, blk, bytes, s, a);
+            }
         } else if (f16) {
             launch_mfma<1, 4, true>(a, s);
         } else {

@@ -563,13 +563,18 @@ static int gram_list_cap(int Nx, int Ny) {
     return ((Wn + 15) / 16) * 16 + 16;
 }
 
-// Tile of the staged kernel.  First choice: a 4 x 16 tile for eight waves,
-// one workgroup per CU, which re-stages ~6.6 halo records per pixel at a
-// 15 x 11 window (a 3 x 8 tile, ~11.9).  Otherwise the largest TH x TW tile
+// Tile of the staged kernel.  First choice: an 8 x 16 tile for twelve
+// waves (three per SIMD), one workgroup per CU, which stages ~4.2 records
+// per pixel at a 15 x 11 window; then a 4 x 16 tile for eight waves (~6.6;
+// a 3 x 8 tile stages ~11.9).  Otherwise the largest TH x TW tile
 // (TH <= 4, TW <= 8) for four waves whose stage keeps two workgroups on a
 // CU.  Returns the stage's LDS bytes.
 static size_t gram_stage_tile(int Nx, int Ny, int& TH, int& TW, int& waves) {
     const int cap = gram_list_cap(Nx, Ny);
+    waves = 12;
+    TH = 8;
+    TW = 16;
+    if (gram_lds_bytes(Nx, Ny, TH, TW, cap, waves) <= 156 * 1024) return gram_lds_bytes(Nx, Ny, TH, TW, cap, waves);
     waves = 8;
     TH = 4;
     TW = 16;
@@ -650,7 +655,10 @@ int gram_mfma_launch(const LinkPlan& P, const uint32_t* pk, const uint32_t* wts,
             a.TW = TW;
             const dim3 grid((unsigned)((a.cols + TW - 1) / TW), (unsigned)((a.chunk_rows + TH - 1) / TH));
             const dim3 blk(64 * waves);
-            if (waves == 8) {
+            if (waves == 12) {
+                if (f16) hipLaunchKernelGGL((gram_mfma_lds<true, 12>), grid, blk, bytes, s, a);
+                else hipLaunchKernelGGL((gram_mfma_lds<false, 12>), grid, blk, bytes, s, a);
+            } else if (waves == 8) {
                 if (f16) hipLaunchKernelGGL((gram_mfma_lds<true, 8>), grid, blk, bytes, s, a);
                 else hipLaunchKernelGGL((gram_mfma_lds<false, 8>), grid, blk, bytes, s, a);
             } else {

@@ -281,6 +281,25 @@ def test_mixed_precision_tcorr_and_phase_at_cfg1_scale():
         assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL
 
 
+@pytest.mark.parametrize("Nx,Ny", [(3, 2), (10, 5), (12, 7)])
+def test_staged_gram_tile_variants(Nx, Ny):
+    # the staged fp16 hi/lo Gram picks its tile by window size: 8 x 16 pixels
+    # on twelve waves (7 x 5 window), 4 x 16 on eight (21 x 11), one pixel on
+    # four (25 x 15); each against the FP64 Gram
+    n, rows, cols = 30, 20, 40
+    print(f"case staged Gram window {2 * Nx + 1}x{2 * Ny + 1}", flush=True)
+    slc = synthetic.stack(n, rows, cols, seed=500 + Nx, patches=6)
+    wts, _ = oracle.nmap(slc, Nx, Ny, 0.05)
+    g64, np64 = tile.coherence(to_dev(slc, torch), wts_dev(wts, torch), Nx, Ny, precision="fp64", min_neighbors=2)
+    g16, np16 = tile.coherence(to_dev(slc, torch), wts_dev(wts, torch), Nx, Ny, precision="mixed", min_neighbors=2)
+    torch.cuda.synchronize()
+    g64, np64, g16, np16 = g64.cpu().numpy(), np64.cpu().numpy(), g16.cpu().numpy(), np16.cpu().numpy()
+    assert np.array_equal(np64, np16)
+    linked = np64 >= 0
+    assert linked.sum() > 0.5 * rows * cols
+    assert np.abs(g16[linked] - g64[linked]).max() < 2e-5
+
+
 @pytest.mark.parametrize("n", [10, 30, 48])
 def test_coherence_matrices(n):
     # fringe_coherence_run: FP64 Gram against a numpy FP64 brute force of

@@ -453,6 +453,39 @@ def cfg4(args):
     print(json.dumps(out))
 
 
+def vs_oracle(oracle, res, band, wb, rr, Nx, Ny, min_nb, threads, gpu):
+    """Phase, temporal-coherence and code agreement of the GPU outputs with
+    the oracle's on the oracle's row band.  A pixel off by more than 1e-3 rad
+    is counted as ill-posed when its relative eigen-gap (recomputed by the
+    oracle for the rows that hold such pixels) is below 1e-3."""
+    import torch
+    r0, r1, a0 = rr
+    ph_o = res["phase"][:, r0 - a0:r1 - a0]
+    tc_o = res["tcorr"][r0 - a0:r1 - a0]
+    out = {"reference": "oracle (FP64 LAPACK restatement of phase_link.cpp:479-666)",
+           "rows": [r0, r1], "pixels": int(tc_o.size)}
+    gaps = {}
+    for name, (ph, tcg) in gpu.items():
+        ph_g = ph[:, r0:r1].cpu().numpy()
+        tc_g = tcg[r0:r1].cpu().numpy()
+        ok = (tc_o > 0) & (tc_g > 0)
+        err = np.abs(np.angle(ph_g * np.conj(ph_o))).max(axis=0)
+        bad = ok & (err > 1e-3)
+        for row in np.unique(np.nonzero(bad)[0]):
+            if row not in gaps:
+                g = oracle.link(band, wb, Nx, Ny, module="phase_link", method="MLE", min_neighbors=min_nb,
+                                rows_range=(r0 - a0 + row, r0 - a0 + row + 1), want_gap=True, threads=threads)
+                gaps[row] = g["gap"][r0 - a0 + row]
+        n_well_bad = sum(int((gaps[row][np.nonzero(bad[row])[0]] >= 1e-3).sum())
+                         for row in np.unique(np.nonzero(bad)[0]))
+        out[name] = {"phase_err_max_rad": float(err[ok].max()) if ok.any() else 0.0,
+                     "phase_err_p99_rad": float(np.percentile(err[ok], 99)) if ok.any() else 0.0,
+                     "n_phase_err_gt_1e-3": int(bad.sum()), "n_gt_1e-3_with_gap_ge_1e-3": n_well_bad,
+                     "tcorr_err_max": float(np.abs(tc_g - tc_o)[ok].max()) if ok.any() else 0.0,
+                     "codes_equal": bool(np.array_equal(np.where(tc_o <= 0, tc_o, 1), np.where(tc_g <= 0, tc_g, 1)))}
+    return out
+
+
 def parse():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -466,6 +499,8 @@ def parse():
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--cpu-rows", type=int, default=512, help="output rows of the CPU oracle sample (about 10 s on 16 cores)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--device-gen", action="store_true",
+                    help="generate the tile with torch's generator on the GPU instead of numpy PCG64 on the host")
     ap.add_argument("--seed", type=int, default=102)
     ap.add_argument("--copy-probe", action="store_true", help="run only the copy kernel used to calibrate PMC bytes")
     ap.add_argument("--serve", action="store_true", help="serving workload: Poisson stream of mixed tiles")
@@ -514,7 +549,13 @@ def main():
     n, rows, cols, Nx, Ny = args.n, args.rows, args.cols, args.nx, args.ny
     dev = torch.device("cuda:0")
     torch.cuda.manual_seed(args.seed)
-    slc = synthetic.stack_torch(n, rows, cols, args.seed, device=dev, patches=32)
+    if args.device_gen:
+        slc = synthetic.stack_torch(n, rows, cols, args.seed, device=dev, patches=32)
+        data = "synthetic (seeded torch generator on the GPU)"
+    else:
+        # numpy PCG64(seed): the same tile can be rebuilt on any host
+        slc = torch.from_numpy(synthetic.stack(n, rows, cols, args.seed, patches=32)).to(dev)
+        data = f"synthetic (numpy PCG64 seed {args.seed}, fringe_amd.synthetic.stack, copied to the GPU)"
     wts, cnt = tile.nmap(slc, Nx, Ny, 0.05)
     torch.cuda.synchronize()
     L.fringe_timing_reset()
@@ -597,6 +638,7 @@ def main():
 
     # ---- CPU baseline: the oracle on a bounded row band -------------------
     cpu = None
+    acc_oracle = None
     if not args.no_cpu:
         try:
             sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
@@ -615,6 +657,8 @@ def main():
             cpu = {"value": lk / dt, "unit": "pixels/s", "cores": threads, "kind": "port",
                    "sample": f"{r1 - r0} output rows x {cols} cols of the same tile ({lk} pixels linked, {dt:.1f} s), "
                              "oracle/fringe_oracle.cpp with FP64 LAPACK, OpenMP"}
+            acc_oracle = vs_oracle(oracle, res, band, wb, (r0, r1, a0), Nx, Ny, min_nb, threads,
+                                   {"mixed": (ph_mixed, tc), "fp64": (fp64.phase, fp64.tcorr)})
         except Exception as e:  # the baseline is reported, never the product
             cpu = {"error": str(e)}
 
@@ -623,13 +667,13 @@ def main():
         "value": value, "unit": "pixels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "mixed: f16 hi/lo MFMA Gram (f32 acc), f32 eigensolver, f64 |Gamma| inverse",
-        "data": "synthetic (seeded, generated on the GPU)",
+        "data": data,
         "config": {"workload": f"phase_link EMI, N={n}, {rows}x{cols} px, window {2*Nx+1}x{2*Ny+1}, "
                                f"nmap KS2 mask (alpha=0.05) on gamma-patch amplitudes, minNeighbors={min_nb}",
                    "pixels_linked": linked, "shp_fill_mean": fill},
         "fp64_value": value64, "fp64_ms_per_step": ms64, "speedup_vs_fp64": value / value64,
         "p50_tile_ms": float(np.percentile(lat, 50)), "p99_tile_ms": float(np.percentile(lat, 99)),
-        "accuracy_vs_fp64": acc,
+        "accuracy_vs_fp64": acc, "accuracy_vs_oracle": acc_oracle,
         "stages_ms_per_step": {k: v[0] for k, v in per.items()},
         "nmap_ms": {k: v[0] for k, v in nm_per.items()},
         "stages_fp64_ms_per_step": {k: v[0] / max(args.fp64_steps, 1) for k, v in stages64.items() if v[1]},

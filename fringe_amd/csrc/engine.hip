@@ -196,9 +196,8 @@ static int link_chunk_pixels(const fringe_link_tile* t) {
     return (int)(rows * t->cols);
 }
 
-static size_t link_layout(const fringe_link_tile* t, size_t* off_gamma, size_t* off_npix,
+static size_t link_layout(const fringe_link_tile* t, int chunk, size_t* off_gamma, size_t* off_npix,
                           size_t* off_pack) {
-    const int chunk = link_chunk_pixels(t);
     const int n = t->nbands;
     const size_t e = link_uses_mfma(n, t->precision) ? 8 : 16;
     size_t b = 0;
@@ -211,30 +210,33 @@ static size_t link_layout(const fringe_link_tile* t, size_t* off_gamma, size_t* 
     return b;
 }
 
-}  // namespace fringe
-
-using namespace fringe;
-
-extern "C" {
-
-const char* fringe_version(void) { return "fringe_amd 0.1 (gfx950)"; }
-const char* fringe_last_error(void) { return g_last_error.c_str(); }
-int fringe_mask_words(int Nx, int Ny) { return shp_words(Nx, Ny); }
-
-size_t fringe_link_workspace_bytes(const fringe_link_tile* t) {
-    if (link_check(t) != 0) return 0;
+size_t link_workspace_for_chunk(const fringe_link_tile* t, int chunk) {
     size_t a, b, c;
-    return link_layout(t, &a, &b, &c);
+    return link_layout(t, chunk, &a, &b, &c);
 }
 
-int fringe_link_run(const fringe_link_tile* t, const void* d_slc, const uint32_t* d_wts,
-                    void* d_phase, float* d_tcorr, void* d_comp, void* d_work, size_t work_bytes,
-                    void* stream) {
-    const int st = link_check(t);
-    if (st != 0) return st;
-    if (!d_slc || !d_wts || !d_phase || !d_tcorr || !d_comp || !d_work) return FRINGE_ERR_ARGUMENT;
+// Largest chunk (whole rows, whole tiles of the staged Gram when possible)
+// whose workspace fits in `bytes`; 0 when not even one row fits.
+int link_chunk_for_bytes(const fringe_link_tile* t, size_t bytes) {
+    const long long out_rows = t->row_end - t->row_begin;
+    if (out_rows <= 0) return t->cols;
+    const int th = link_uses_mfma(t->nbands, t->precision) ? gram_stage_rows(t->nbands, t->Nx, t->Ny) : 1;
+    long long lo = 0, hi = out_rows;   // lo rows fit (0 = nothing yet)
+    while (lo < hi) {
+        const long long mid = (lo + hi + 1) / 2;
+        if (link_workspace_for_chunk(t, (int)(mid * t->cols)) <= bytes) lo = mid;
+        else hi = mid - 1;
+    }
+    if (lo > th && lo < out_rows) lo -= lo % th;
+    return (int)(lo * t->cols);
+}
+
+// Body of fringe_link_run with an explicit chunk (pixels per Gram/solve
+// round, a multiple of cols); the workspace must hold link_layout(chunk).
+int link_run_chunked(const fringe_link_tile* t, const void* d_slc, const uint32_t* d_wts, void* d_phase,
+                     float* d_tcorr, void* d_comp, void* d_work, size_t work_bytes, int chunk, void* stream) {
     size_t og, on, op;
-    const size_t need = link_layout(t, &og, &on, &op);
+    const size_t need = link_layout(t, chunk, &og, &on, &op);
     if (work_bytes < need) return FRINGE_ERR_WORKSPACE;
     if (t->row_end == t->row_begin) return 0;
     LinkPlan P{};
@@ -254,7 +256,7 @@ int fringe_link_run(const fringe_link_tile* t, const void* d_slc, const uint32_t
     P.row_begin = t->row_begin;
     P.row_end = t->row_end;
     P.npx = (long long)t->rows * t->cols;
-    P.chunk = link_chunk_pixels(t);
+    P.chunk = chunk;
     P.d_gamma = (char*)d_work + og;
     P.d_npix = (int*)((char*)d_work + on);
     P.d_pack = (uint32_t*)((char*)d_work + op);
@@ -264,6 +266,31 @@ int fringe_link_run(const fringe_link_tile* t, const void* d_slc, const uint32_t
         return FRINGE_ERR_DEVICE;
     }
     return 0;
+}
+
+}  // namespace fringe
+
+using namespace fringe;
+
+extern "C" {
+
+const char* fringe_version(void) { return "fringe_amd 0.1 (gfx950)"; }
+const char* fringe_last_error(void) { return g_last_error.c_str(); }
+int fringe_mask_words(int Nx, int Ny) { return shp_words(Nx, Ny); }
+
+size_t fringe_link_workspace_bytes(const fringe_link_tile* t) {
+    if (link_check(t) != 0) return 0;
+    return link_workspace_for_chunk(t, link_chunk_pixels(t));
+}
+
+int fringe_link_run(const fringe_link_tile* t, const void* d_slc, const uint32_t* d_wts,
+                    void* d_phase, float* d_tcorr, void* d_comp, void* d_work, size_t work_bytes,
+                    void* stream) {
+    const int st = link_check(t);
+    if (st != 0) return st;
+    if (!d_slc || !d_wts || !d_phase || !d_tcorr || !d_comp || !d_work) return FRINGE_ERR_ARGUMENT;
+    return link_run_chunked(t, d_slc, d_wts, d_phase, d_tcorr, d_comp, d_work, work_bytes, link_chunk_pixels(t),
+                            stream);
 }
 
 size_t fringe_coherence_workspace_bytes(const fringe_link_tile* t) {

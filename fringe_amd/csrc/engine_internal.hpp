@@ -13,7 +13,7 @@ namespace fringe {
 
 constexpr int kMaxBandsNmap = 128;   // nmap sorts at most this many dates
 constexpr int kMaxBandsLink = 128;   // phase-linking solver limit
-constexpr int kMaxMaskWords = 16;    // window up to 512 positions
+constexpr int kMaxMaskWords = 64;    // window up to 2048 positions (the reference sequential default 59 x 19 is 1121)
 
 // Decision table of the integer KS statistic, one bit per k = D*n.
 struct KsTable {
@@ -64,6 +64,14 @@ int calamp_launch(const float2* z, const uint8_t* mask, long long n, double2* pa
 
 // 0, or the code fringe_link_run would return for these options.
 int link_check(const fringe_link_tile* t);
+// Workspace bytes of fringe_link_run when each Gram/solve round covers
+// `chunk` pixels (a multiple of cols), and the largest such chunk that fits
+// in `bytes` (0 when not one row fits).
+size_t link_workspace_for_chunk(const fringe_link_tile* t, int chunk);
+int link_chunk_for_bytes(const fringe_link_tile* t, size_t bytes);
+// fringe_link_run with an explicit chunk (arguments already checked).
+int link_run_chunked(const fringe_link_tile* t, const void* d_slc, const uint32_t* d_wts, void* d_phase,
+                     float* d_tcorr, void* d_comp, void* d_work, size_t work_bytes, int chunk, void* stream);
 void set_last_error(const std::string& what);
 
 int coherence_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, void* gamma, int* npix,

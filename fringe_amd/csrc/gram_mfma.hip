@@ -140,6 +140,10 @@ __global__ __launch_bounds__(256) void pack_hilo(const float2* __restrict__ slc,
         for (int e = threadIdx.x; e < CS; e += blockDim.x) pk[npx * CS + e] = 0u;
 }
 
+// The staged kernel reads a pixel's mask words from lanes 0..15, two words
+// per 64-position pass: windows of up to 512 positions.
+constexpr int kStageWords = 16;
+
 struct MfmaArgs {
     const uint32_t* pk;   // packed records, [rows*cols + 1][n][re, im]
     const uint32_t* wts;
@@ -405,7 +409,7 @@ __global__ __launch_bounds__(64 * WAVES) void gram_mfma_lds(MfmaArgs a) {
     const int Wx = 2 * Nx + 1, Wn = (2 * Ny + 1) * Wx;
     const int m = lane & 31, hf = lane >> 5;
     // window offsets of this lane in every 64-wide pass (pixel-independent)
-    constexpr int kPass = (16 * 32 + 63) / 64;
+    constexpr int kPass = (kStageWords * 32 + 63) / 64;
     int offy[kPass], offx[kPass];
 #pragma unroll
     for (int it = 0; it < kPass; ++it) {
@@ -585,13 +589,29 @@ static size_t gram_stage_tile(int Nx, int Ny, int& TH, int& TW, int& waves) {
     TW = 8;
     while (TH > 1 && gram_lds_bytes(Nx, Ny, TH, TW, cap, waves) > budget) --TH;
     while (TW > 1 && gram_lds_bytes(Nx, Ny, TH, TW, cap, waves) > budget) TW >>= 1;
-    return gram_lds_bytes(Nx, Ny, TH, TW, cap, waves);
+    if (TH * TW >= 4) return gram_lds_bytes(Nx, Ny, TH, TW, cap, waves);
+    // a stage for fewer than four pixels leaves waves idle: take the largest
+    // tile of at least four pixels that fits one workgroup per CU, or none
+    // (the unstaged kernel then gathers from L2)
+    for (TH = 4; TH >= 1; --TH)
+        for (TW = 8; TW >= 1; TW >>= 1)
+            if (TH * TW >= 4 && gram_lds_bytes(Nx, Ny, TH, TW, cap, waves) <= 156 * 1024)
+                return gram_lds_bytes(Nx, Ny, TH, TW, cap, waves);
+    return (size_t)1 << 40;
+}
+
+// The staged kernel takes windows of up to kStageWords mask words; larger
+// windows go to the unstaged kernel, which walks any number of words.
+static bool gram_staged(int n, int Nx, int Ny, int& TH, int& TW, int& waves, size_t& bytes) {
+    if (n > 32 || shp_words(Nx, Ny) > kStageWords) return false;
+    bytes = gram_stage_tile(Nx, Ny, TH, TW, waves);
+    return bytes <= 160 * 1024;
 }
 
 int gram_stage_rows(int n, int Nx, int Ny) {
-    if (n > 32) return 1;
     int TH, TW, waves;
-    return gram_stage_tile(Nx, Ny, TH, TW, waves) <= 160 * 1024 ? TH : 1;
+    size_t bytes;
+    return gram_staged(n, Nx, Ny, TH, TW, waves, bytes) ? TH : 1;
 }
 
 // packed records followed by the per-date maxima (256-byte aligned)
@@ -649,8 +669,8 @@ int gram_mfma_launch(const LinkPlan& P, const uint32_t* pk, const uint32_t* wts,
     const bool f16 = (P.precision != FRINGE_PREC_MIXED_BF16);
     if (P.n <= 32) {
         int TH, TW, waves;
-        const size_t bytes = gram_stage_tile(P.Nx, P.Ny, TH, TW, waves);
-        if (bytes <= 160 * 1024) {
+        size_t bytes = 0;
+        if (gram_staged(P.n, P.Nx, P.Ny, TH, TW, waves, bytes)) {
             a.TH = TH;
             a.TW = TW;
             const dim3 grid((unsigned)((a.cols + TW - 1) / TW), (unsigned)((a.chunk_rows + TH - 1) / TH));

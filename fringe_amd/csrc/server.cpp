@@ -264,6 +264,20 @@ void dispatcher(fringe_server* S) {
             lk.lock();
             continue;
         }
+        // stopping with nothing running: a queued request that could not be
+        // admitted now never will be (the arena space it waits for is held
+        // by outputs nobody releases any more); fail it so the loop ends
+        if (S->stop && S->running == 0 && !S->queue.empty()) {
+            const auto now = Clock::now();
+            for (Request* q : S->queue) {
+                q->code = FRINGE_ERR_WORKSPACE;
+                q->state = FRINGE_REQ_DONE;
+                q->t_start = q->t_done = now;
+            }
+            S->queue.clear();
+            S->done.notify_all();
+            continue;
+        }
         S->wake.wait(lk);
     }
 }

@@ -91,7 +91,8 @@ class TileServer:
 
     def fetch(self, rid):
         """Copy a done request's outputs into new tensors:
-        (phase, tcorr, comp, wts or None, count or None)."""
+        (phase, tcorr, comp, wts or None, count or None).  Returns after the
+        copies have finished, so the request may be released at once."""
         ptrs = [ctypes.c_void_p() for _ in range(5)]
         code = self.L.fringe_server_result(self.h, rid, *[ctypes.byref(p) for p in ptrs])
         if code:
@@ -114,6 +115,10 @@ class TileServer:
                                      ctypes.c_void_p(s.cuda_stream))
                 if e:
                     raise RuntimeError(f"hipMemcpyAsync failed ({e})")
+        # the copies must have read the arena before release() can hand the
+        # region to another request, whose kernels run on a server stream
+        # that is not ordered with this one
+        s.synchronize()
         return tuple(out)
 
     def release(self, rid):

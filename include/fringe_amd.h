@@ -275,7 +275,10 @@ int fringe_server_result(fringe_server* s, long long id, void** d_phase, float**
 int fringe_server_times(fringe_server* s, long long id, double* queued_ms, double* total_ms);
 /* Free a done request's outputs. */
 int fringe_server_release(fringe_server* s, long long id);
-/* Run every submitted request to completion, then free the arena and streams. */
+/* Run every submitted request that can still be admitted to completion, then
+ * free the arena and streams.  Queued requests that cannot be admitted once
+ * nothing else runs (their space is held by outputs never released) end with
+ * FRINGE_ERR_WORKSPACE instead of blocking. */
 int fringe_server_destroy(fringe_server* s);
 
 /* ---- stage timing (no counterpart in the reference; for profiling) ------

@@ -666,17 +666,22 @@ extern "C" int oracle_link(const float* slc, const uint32_t* wts, int rows, int 
             if (gap && sel_matrix) {
                 // Diagnostic: neighbour gap of the selected eigenvalue over the
                 // spectral spread, from the full FP64 spectrum.
+                // (eigenvalues 1, 2, n-1 and n are all it needs)
                 std::vector<zd> m = *sel_matrix;
                 EigWork all;
                 all.prepare(n);
-                std::vector<double> lam(n);
-                for (int k = 1; k <= n; ++k) {
+                auto eig = [&](int k) {
                     std::vector<zd> mm = m;
                     all.eigen_with_index(mm.data(), k, false);
-                    lam[k - 1] = all.w[0];
+                    return all.w[0];
+                };
+                if (n < 2) {
+                    gap[pp] = 1.0;
+                    continue;
                 }
-                double spread = std::max(lam[n - 1] - lam[0], 1e-300);
-                double g = (sel_index == 1) ? lam[1] - lam[0] : lam[n - 1] - lam[n - 2];
+                const double l1 = eig(1), ln = eig(n);
+                const double spread = std::max(ln - l1, 1e-300);
+                const double g = (sel_index == 1) ? eig(2) - l1 : ln - eig(n - 1);
                 gap[pp] = g / spread;
             }
         }

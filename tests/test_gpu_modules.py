@@ -152,6 +152,64 @@ def test_sequential_against_oracle_replay(tmp_path):
     assert wrapped_err(ph, ref["phase"])[:, ok].max() <= 5 * PHASE_TOL
 
 
+def test_sequential_uneven_ministacks_120_dates(tmp_path):
+    """120 dates split into mini-stacks of lengths drawn from 8..20 (the
+    uneven-length extension of sequential.py:193-240), every mini-stack and
+    the datum connection replayed through the oracle."""
+    import datetime
+    from fringe_amd.sequential import run_sequential
+    n, rows, cols, Nx, Ny = 120, 16, 20, 5, 4
+    rng = np.random.default_rng(104)
+    sizes = []
+    while sum(sizes) < n:
+        sizes.append(int(rng.integers(8, 21)))
+    slc = synthetic.stack(n, rows, cols, seed=104)
+    t0 = datetime.date(2022, 1, 1)
+    dates = []
+    for t in range(n):
+        d = (t0 + datetime.timedelta(12 * t)).strftime("%Y%m%d")
+        os.makedirs(tmp_path / "SLC" / d)
+        raster.write_envi(str(tmp_path / "SLC" / d / (d + ".slc")), slc[t])
+        dates.append(d)
+    wts = synthetic.full_mask(rows, cols, Nx, Ny)
+    wf = str(tmp_path / "wts")
+    raster.write_envi(wf, wts, meta={"HALFWINDOWX": Nx, "HALFWINDOWY": Ny})
+    print(f"case: sequential 120 dates, mini-stacks {sizes}", flush=True)
+    res = run_sequential(str(tmp_path / "SLC"), wf, str(tmp_path / "seq"), halfWindowX=Nx, halfWindowY=Ny,
+                         minNeighbors=5, verbose=False, mini_stack_sizes=sizes)
+    assert all(rc == 0 for _, rc in res) and len(res) == len(sizes) + 1
+    comps, ends = [], []
+    s0 = 0
+    for k, size in enumerate(sizes, start=1):
+        s1 = min(n, s0 + size)
+        stack = np.concatenate([np.array(comps).reshape(-1, rows, cols), slc[s0:s1]]) if comps else slc[s0:s1]
+        assert stack.shape[0] == (k - 1) + (s1 - s0)
+        ref = oracle.link(stack.astype(np.complex64), wts, Nx, Ny, module="evd", method="MLE",
+                          mini_stack_count=k, want_gap=True)
+        folder = str(tmp_path / "seq" / "miniStacks" / f"{dates[s0]}_{dates[s1 - 1]}" / "EVD")
+        band_dates = ends + dates[s0:s1]
+        ph = np.stack([raster.read(os.path.join(folder, d + ".slc")) for d in band_dates])
+        tc = raster.read(os.path.join(folder, "tcorr.bin"), dtype=np.float32)
+        comp = raster.read(str(tmp_path / "seq" / "compressedSlc" / dates[s1 - 1] / (dates[s1 - 1] + ".slc")))
+        ok = (ref["tcorr"] > 0) & (ref["gap"] >= 1e-3)
+        assert ok.sum() > 0.9 * rows * cols
+        assert np.array_equal(ref["tcorr"] > 0, tc > 0)
+        assert wrapped_err(ph, ref["phase"])[:, ok].max() <= PHASE_TOL
+        assert np.abs(tc - ref["tcorr"])[ok].max() <= TCORR_TOL
+        rel = np.abs(comp - ref["comp"])[ok] / np.maximum(np.abs(ref["comp"][ok]), 1e-6)
+        assert rel.max() < 1e-3
+        comps.append(ref["comp"])
+        ends.append(dates[s1 - 1])
+        s0 = s1
+    dc = str(tmp_path / "seq" / "Datum_connection" / "EVD")
+    ref = oracle.link(np.array(comps).astype(np.complex64), wts, Nx, Ny, module="evd", method="MLE",
+                      want_gap=True)
+    ph = np.stack([raster.read(os.path.join(dc, d + ".slc")) for d in ends])
+    ok = (ref["tcorr"] > 0) & (ref["gap"] >= 1e-3)
+    assert ok.sum() > 0.9 * rows * cols
+    assert wrapped_err(ph, ref["phase"])[:, ok].max() <= 5 * PHASE_TOL
+
+
 def test_datum_adjustment_wrapped_and_unwrapped(tmp_path):
     # adjustMiniStacks.py after a sequential run: wrapped phases multiplied by
     # the mini-stack's datum phase as GDAL's "mul" pixel function does it (in

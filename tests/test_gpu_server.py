@@ -12,7 +12,11 @@ if not gpu_available():
 
 import torch  # noqa: E402
 
+import numpy as np  # noqa: E402
+
+import oracle  # noqa: E402
 from fringe_amd import synthetic, tile  # noqa: E402
+from helpers import PHASE_TOL, TCORR_TOL, compare_link  # noqa: E402
 from fringe_amd.server import TileServer  # noqa: E402
 
 DEV = torch.device("cuda:0")
@@ -43,14 +47,15 @@ def _inputs():
 
 
 def _fetch_all(srv, ids):
+    """Wait, fetch and release each request at once (other requests may be
+    queued behind it and take its arena space), then read the copies."""
     res = []
     for rid in ids:
         assert srv.wait(rid) == 0
         r = srv.fetch(rid)
-        torch.cuda.synchronize()
-        res.append([x.cpu() if x is not None else None for x in r])
         srv.release(rid)
-    return res
+        res.append(r)
+    return [[x.cpu() if x is not None else None for x in r] for r in res]
 
 
 def test_results_equal_alone_and_in_the_mix():
@@ -136,3 +141,65 @@ def test_per_request_errors():
         q, t = srv.times(rid)
         assert 0 <= q <= t
         srv.release(rid)
+
+
+def test_destroy_ends_requests_that_can_never_be_admitted():
+    print("case server destroy with unreleased outputs", flush=True)
+    n, rows, cols, Nx, Ny = 10, 64, 64, 3, 2
+    slc = synthetic.stack_torch(n, rows, cols, 9, device=DEV)
+    p = tile.link_params(rows, cols, n, Nx, Ny, precision="mixed")
+    with TileServer(1 << 30, streams=2) as probe:
+        need = probe.request_bytes(p, with_mask=False)
+    # room for one request; once it is done its outputs alone leave too
+    # little for another
+    srv = TileServer(need + 4096, streams=2)
+    first = srv.submit(slc, p, prob=0.05)
+    assert srv.wait(first) == 0
+    # the first request's outputs stay in the arena (never released), so the
+    # second cannot be admitted; closing the server must still return
+    second = srv.submit(slc, p, prob=0.05)
+    st, _ = srv.poll(second)
+    assert st == 0
+    srv.close()
+
+
+def test_large_n_requests_against_the_oracle():
+    """Requests of the serving mix at 48 and 96 dates, 21 x 11 and 13 x 11
+    windows, mixed-precision EMI: each result equals the oracle's to the
+    phase and coherence bounds, and is the same alone and in the mix."""
+    cases = [(48, 24, 32, 10, 5, True), (96, 20, 24, 6, 5, False), (30, 40, 48, 7, 5, True)]
+    inp = []
+    for i, (n, rows, cols, Nx, Ny, srv_nmap) in enumerate(cases):
+        slc = synthetic.stack(n, rows, cols, seed=900 + i, patches=0 if n == 96 else 8)
+        wts = None if srv_nmap else synthetic.full_mask(rows, cols, Nx, Ny)
+        p = tile.link_params(rows, cols, n, Nx, Ny, module="phase_link", method="MLE", min_neighbors=5,
+                             precision="mixed")
+        inp.append((slc, wts, p))
+    d = [(torch.from_numpy(slc).to(DEV), None if w is None else torch.from_numpy(w.view(np.int32)).to(DEV), p)
+         for slc, w, p in inp]
+    print("case server n=48/96/30 against the oracle", flush=True)
+    with TileServer(8 << 30, streams=4) as srv:
+        alone = []
+        for slc, wts, p in d:
+            alone += _fetch_all(srv, [srv.submit(slc, p, wts=wts)])
+        mixed = _fetch_all(srv, [srv.submit(slc, p, wts=wts) for slc, wts, p in d])
+    for a, m in zip(alone, mixed):
+        for x, y in zip(a, m):
+            if x is not None:
+                assert torch.equal(x, y)
+    for (slc, wts, p), res, (n, rows, cols, Nx, Ny, srv_nmap) in zip(inp, alone, cases):
+        if srv_nmap:
+            w_ref, c_ref = oracle.nmap(slc, Nx, Ny, 0.05)
+            assert np.array_equal(res[3].numpy().view(np.uint32), w_ref)
+            wts = w_ref
+        ref = oracle.link(slc, wts, Nx, Ny, module="phase_link", method="MLE", min_neighbors=5, want_gap=True)
+        gap = ref["gap"].copy()
+        rr, cc = np.arange(rows)[:, None], np.arange(cols)[None, :]
+        k = ((np.minimum(rr + Ny, rows - 1) - np.maximum(rr - Ny, 0) + 1)
+             * (np.minimum(cc + Nx, cols - 1) - np.maximum(cc - Nx, 0) + 1))
+        gap[k < n] = -1.0      # windows with fewer samples than dates: rank-deficient Gamma
+        st = compare_link((res[0].numpy(), res[1].numpy(), res[2].numpy()), ref, gap, vmin=ref["vmin"])
+        print(f"  n={n}: {st}")
+        assert st["untouched_equal"]
+        assert st["n_well"] >= 0.25 * rows * cols
+        assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL

@@ -172,6 +172,16 @@ def test_nmap_oracle_matches_brute_force():
     assert c1[4, 5] == 0 and not w1[4, 5].any()
 
 
+def test_nmap_oracle_matches_brute_force_at_59x19():
+    # the reference sequential default window (sequential.py -x 29 -y 9):
+    # 1121 positions in 36 words, wider and taller than the image
+    slc = synthetic.stack(10, 10, 12, seed=13, patches=4)
+    w1, c1 = oracle.nmap(slc, 29, 9, 0.05)
+    w2, c2 = _nmap_brute(slc, 29, 9, 0.05)
+    assert w1.shape[-1] == 36
+    assert np.array_equal(w1, w2) and np.array_equal(c1, c2)
+
+
 def test_nmap_amplitude_constants_and_mask():
     slc = synthetic.stack(8, 8, 8, seed=12, patches=4)
     mask = np.ones((8, 8), np.uint8)

@@ -170,6 +170,9 @@ int link_check(const fringe_link_tile* t) {
         return FRINGE_ERR_UNSUPPORTED;
     // LDS footprint of the generic solver used above 64 dates (one pixel per
     // two-wave workgroup): the pixel's matrices plus the kernel's static LDS
+    if (link_certifies(t->nbands, t->precision) &&
+        resolve_lds_bytes(t->nbands, t->Nx, t->Ny) > (size_t)device_lds_limit())
+        return FRINGE_ERR_UNSUPPORTED;
     if (t->nbands > 64) {
         const size_t need = (t->precision == FRINGE_PREC_FP64 ? solve_group_bytes<double>(t->nbands)
                                                               : solve_group_bytes<float>(t->nbands)) +
@@ -196,24 +199,29 @@ static int link_chunk_pixels(const fringe_link_tile* t) {
     return (int)(rows * t->cols);
 }
 
-static size_t link_layout(const fringe_link_tile* t, int chunk, size_t* off_gamma, size_t* off_npix,
-                          size_t* off_pack) {
+struct LinkLayout {
+    size_t gamma, npix, pack, cert, total;
+};
+
+static LinkLayout link_layout(const fringe_link_tile* t, int chunk) {
     const int n = t->nbands;
     const size_t e = link_uses_mfma(n, t->precision) ? 8 : 16;
+    LinkLayout L{};
     size_t b = 0;
-    *off_gamma = b;
+    L.cert = b;   // first, so its place does not depend on the chunk
+    if (link_certifies(n, t->precision))   // the count, then the list of the tile's output pixels
+        b += 256 + align256((size_t)(t->row_end - t->row_begin) * t->cols * sizeof(int));
+    L.gamma = b;
     b += align256((size_t)chunk * n * n * e);
-    *off_npix = b;
+    L.npix = b;
     b += align256((size_t)chunk * sizeof(int));
-    *off_pack = b;
+    L.pack = b;
     if (link_uses_mfma(n, t->precision)) b += align256(mfma_pack_bytes(t->rows, t->cols, n));
-    return b;
+    L.total = b;
+    return L;
 }
 
-size_t link_workspace_for_chunk(const fringe_link_tile* t, int chunk) {
-    size_t a, b, c;
-    return link_layout(t, chunk, &a, &b, &c);
-}
+size_t link_workspace_for_chunk(const fringe_link_tile* t, int chunk) { return link_layout(t, chunk).total; }
 
 // Largest chunk (whole rows, whole tiles of the staged Gram when possible)
 // whose workspace fits in `bytes`; 0 when not even one row fits.
@@ -235,9 +243,8 @@ int link_chunk_for_bytes(const fringe_link_tile* t, size_t bytes) {
 // round, a multiple of cols); the workspace must hold link_layout(chunk).
 int link_run_chunked(const fringe_link_tile* t, const void* d_slc, const uint32_t* d_wts, void* d_phase,
                      float* d_tcorr, void* d_comp, void* d_work, size_t work_bytes, int chunk, void* stream) {
-    size_t og, on, op;
-    const size_t need = link_layout(t, chunk, &og, &on, &op);
-    if (work_bytes < need) return FRINGE_ERR_WORKSPACE;
+    const LinkLayout LL = link_layout(t, chunk);
+    if (work_bytes < LL.total) return FRINGE_ERR_WORKSPACE;
     if (t->row_end == t->row_begin) return 0;
     LinkPlan P{};
     P.rows = t->rows;
@@ -257,9 +264,14 @@ int link_run_chunked(const fringe_link_tile* t, const void* d_slc, const uint32_
     P.row_end = t->row_end;
     P.npx = (long long)t->rows * t->cols;
     P.chunk = chunk;
-    P.d_gamma = (char*)d_work + og;
-    P.d_npix = (int*)((char*)d_work + on);
-    P.d_pack = (uint32_t*)((char*)d_work + op);
+    P.d_gamma = (char*)d_work + LL.gamma;
+    P.d_npix = (int*)((char*)d_work + LL.npix);
+    P.d_pack = (uint32_t*)((char*)d_work + LL.pack);
+    if (link_certifies(P.n, P.precision)) {
+        P.d_cert_count = (int*)((char*)d_work + LL.cert);
+        P.d_cert_list = (int*)((char*)d_work + LL.cert + 256);
+        P.cert_cap = (int)((long long)(t->row_end - t->row_begin) * t->cols);
+    }
     hipStream_t s = (hipStream_t)stream;
     if (link_launch(P, (const float2*)d_slc, d_wts, (float2*)d_phase, d_tcorr, (float2*)d_comp, s) != 0) {
         set_last_error("fringe_link_run", hipGetLastError());
@@ -291,6 +303,15 @@ int fringe_link_run(const fringe_link_tile* t, const void* d_slc, const uint32_t
     if (!d_slc || !d_wts || !d_phase || !d_tcorr || !d_comp || !d_work) return FRINGE_ERR_ARGUMENT;
     return link_run_chunked(t, d_slc, d_wts, d_phase, d_tcorr, d_comp, d_work, work_bytes, link_chunk_pixels(t),
                             stream);
+}
+
+long long fringe_link_resolved_pixels(const fringe_link_tile* t, const void* d_work, size_t work_bytes) {
+    if (link_check(t) != 0 || !d_work) return -1;
+    if (!link_certifies(t->nbands, t->precision)) return 0;
+    if (work_bytes < 256) return -1;
+    int c = 0;   // the count sits at the start of the workspace
+    if (hipMemcpy(&c, d_work, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    return c;
 }
 
 size_t fringe_coherence_workspace_bytes(const fringe_link_tile* t) {

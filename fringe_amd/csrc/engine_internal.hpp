@@ -49,7 +49,20 @@ struct LinkPlan {
     void* d_gamma;      // [chunk][n][n] complex (double for FP64, float for mixed)
     int* d_npix;        // [chunk]
     uint32_t* d_pack;   // mixed precision: packed (hi|lo) records, [rows*cols+1][2n]
+    // mixed precision: pixels whose FP32 result is not certified, re-solved
+    // in FP64 at the end of the tile (null when not used)
+    int* d_cert_list;   // [cert_cap]
+    int* d_cert_count;  // [1]
+    int cert_cap;       // output pixels of the tile
 };
+
+// The FP64 re-solve of uncertified pixels (n <= 64): dynamic LDS per lane
+// group (its coherence plus the FP64 Gram's scratch), and the whole
+// workgroup's LDS including the solver's static part.
+size_t resolve_group_bytes(int n, int Nx, int Ny);
+size_t resolve_lds_bytes(int n, int Nx, int Ny);
+// Whether a tile of this precision and date count certifies its FP32 results.
+bool link_certifies(int n, int precision);
 
 // SHP-masked neighbourhood mean (despeck.cpp:300-433); mode 0 amplitude,
 // 1 interferogram, 2 interferogram with coherence.  c1, c2: rows*cols

@@ -22,150 +22,20 @@
 namespace fringe {
 
 // ---------------------------------------------------------------------------
-// FP64 Gram: one group of G lanes per pixel, lane g owns rows g + G*q (q < R)
-// and accumulates them against a chunk of CB columns.  The masked,
-// in-image window positions are compacted into a list first, so the work is
-// proportional to popcount(mask), not to the window.
+// FP64 Gram over a chunk of output pixels, one group of G lanes per pixel
+// (the per-pixel body is gram_f64_pixel in gram64.hpp).
 // ---------------------------------------------------------------------------
-struct GramArgs {
-    const float2* slc;
-    const uint32_t* wts;
-    int rows, cols, n, Nx, Ny, nul, min_nb;
-    long long npx;
-    long long first;   // linear pixel index (row*cols + col) of slot 0
-    int slots;
-    size_t group_bytes;
-    void* gamma;       // [slots][n][n] cx<double>
-    int* npix;         // [slots]
-};
-
-template <int G>
-__device__ inline int build_position_list(const uint32_t* w, bool cen, int r0, int c0, int rows,
-                                          int cols, int Nx, int Ny, int* list) {
-    const int g = threadIdx.x % G;
-    const int Wx = 2 * Nx + 1, Wn = (2 * Ny + 1) * Wx;
-    int cnt = 0;
-    for (int b0 = 0; b0 < Wn; b0 += G) {
-        const int f = b0 + g;
-        bool take = false;
-        int q = 0;
-        if (cen && f < Wn) {
-            const int rr = r0 + f / Wx - Ny, cc = c0 + f % Wx - Nx;
-            if (rr >= 0 && rr < rows && cc >= 0 && cc < cols && shp_get(w, f)) {
-                take = true;
-                q = rr * cols + cc;
-            }
-        }
-        const uint64_t bal = group_ballot<G>(take);
-        if (take) {
-            const uint64_t below = (g == 0) ? 0ull : (bal & ((~0ull) >> (64 - g)));
-            list[cnt + __popcll(below)] = q;
-        }
-        cnt += __popcll(bal);
-    }
-    return cnt;
-}
-
 template <int G, int R, int CB>
 __global__ __launch_bounds__(64) void gram_f64(GramArgs a) {
     constexpr int GPW = 64 / G;
     const int grp = threadIdx.x / G, g = threadIdx.x % G;
     const int slot = blockIdx.x * GPW + grp;
-    const int n = a.n;
-    const int Wn = (2 * a.Ny + 1) * (2 * a.Nx + 1);
-    const int NC = (n + CB - 1) / CB;
     extern __shared__ __align__(16) char smem[];
-    char* base = smem + grp * a.group_bytes;
-    int* list = (int*)base;
-    cx<double>* zb = (cx<double>*)(base + ((Wn * 4 + 15) & ~15));
-    double* dg = (double*)(zb + NC * CB);
-
     const bool inslot = slot < a.slots;
     const long long pix = a.first + (inslot ? slot : 0);
-    const int r0 = (int)(pix / a.cols), c0 = (int)(pix % a.cols);
-    const uint32_t* w = a.wts + pix * a.nul;
-    const bool cen = inslot && shp_get(w, shp_flat(0, 0, a.Nx, a.Ny));
-    const int cnt = build_position_list<G>(w, cen, r0, c0, a.rows, a.cols, a.Nx, a.Ny, list);
-    const bool go = cen && cnt >= a.min_nb;
-    const int K = go ? cnt : 0;
-    int Kw = K;
-    if (GPW > 1) Kw = max(Kw, __shfl_xor(Kw, 32));
-    for (int c = g; c < NC * CB; c += G) zb[c] = cx<double>{0.0, 0.0};
-    __syncthreads();
-
-    cx<double>* gam = (cx<double>*)a.gamma + (size_t)(inslot ? slot : 0) * n * n;
-    for (int cb = 0; cb < NC; ++cb) {
-        cx<double> acc[R][CB];
-        double d[R];
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            d[q] = 0.0;
-#pragma unroll
-            for (int c = 0; c < CB; ++c) acc[q][c] = cx<double>{0.0, 0.0};
-        }
-        for (int k = 0; k < Kw; ++k) {
-            const int pos = (k < K) ? list[k] : -1;
-            cx<double> z[R];
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-                const int r = g + G * q;
-                z[q] = cx<double>{0.0, 0.0};
-                if (pos >= 0 && r < n) {
-                    const float2 v = a.slc[(long long)r * a.npx + pos];
-                    z[q] = cx<double>{(double)v.x, (double)v.y};
-                }
-                if (r < n) zb[r] = z[q];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-#pragma unroll
-                for (int c = 0; c < CB; ++c) cfma_c(acc[q][c], z[q], zb[cb * CB + c]);
-                if (cb == 0) d[q] += cabs2(z[q]);
-            }
-            __syncthreads();
-        }
-        if (cb == 0) {
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-                const int r = g + G * q;
-                if (r < n) dg[r] = d[q];
-            }
-            __syncthreads();
-        }
-        if (go) {
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-                const int r = g + G * q;
-                if (r >= n) continue;
-#pragma unroll
-                for (int c = 0; c < CB; ++c) {
-                    const int col = cb * CB + c;
-                    if (col >= n || col < r) continue;
-                    if (col == r) {
-                        gam[(size_t)r * n + r] = cx<double>{1.0, 0.0};
-                    } else {
-                        // Gamma_ij = sum z_i conj(z_j) / sqrt(D_i D_j) (evd.cpp:573-579)
-                        const double s = 1.0 / sqrt(dg[r] * dg[col]);
-                        const cx<double> v = cscale(acc[q][c], s);
-                        gam[(size_t)r * n + col] = v;
-                        gam[(size_t)col * n + r] = cconj(v);
-                    }
-                }
-            }
-        }
-        __syncthreads();
-    }
-    if (inslot && g == 0) a.npix[slot] = go ? cnt : -1;
-}
-
-static size_t gram_f64_group_bytes(int n, int Nx, int Ny, int CB) {
-    const int Wn = (2 * Ny + 1) * (2 * Nx + 1);
-    const int NC = (n + CB - 1) / CB;
-    size_t b = ((size_t)Wn * 4 + 15) & ~(size_t)15;
-    b += (size_t)NC * CB * sizeof(cx<double>);
-    b += (size_t)n * sizeof(double);
-    return (b + 15) & ~(size_t)15;
+    cx<double>* gam = (cx<double>*)a.gamma + (size_t)(inslot ? slot : 0) * a.n * a.n;
+    const int np = gram_f64_pixel<G, R, CB>(a, inslot, pix, gam, smem + grp * a.group_bytes);
+    if (inslot && g == 0) a.npix[slot] = np;
 }
 
 template <int G, int R, int CB>
@@ -197,30 +67,89 @@ size_t solve_static_lds(int precision) {
     return hipFuncGetAttributes(&fa, k) == hipSuccess ? fa.sharedSizeBytes : 256;
 }
 
+// n <= 64: register-resident solver, two pixels per wave (n <= 32) or one
 template <typename T, typename Tg>
-static void dispatch_solve(const SolveArgs& a, hipStream_t s) {
+static void dispatch_reg(const SolveArgs& a, hipStream_t s) {
     if (a.n <= 32) {
         const unsigned blocks = (unsigned)((a.slots + 1) / 2);
         if (a.module == FRINGE_MODULE_PHASE_LINK)
-            hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_EMI, 32>), dim3(blocks), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_EMI, 32, false>), dim3(blocks), dim3(64), 0, s, a);
         else if (a.method == FRINGE_METHOD_MLE)
-            hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_MLE, 32>), dim3(blocks), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_MLE, 32, false>), dim3(blocks), dim3(64), 0, s, a);
         else
-            hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_EVD, 32>), dim3(blocks), dim3(64), 0, s, a);
-    }
-    else if (a.n <= 64) {
+            hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_EVD, 32, false>), dim3(blocks), dim3(64), 0, s, a);
+    } else {
         const unsigned blocks = (unsigned)a.slots;
         if (a.module == FRINGE_MODULE_PHASE_LINK)
-            hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_EMI, 64>), dim3(blocks), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_EMI, 64, false>), dim3(blocks), dim3(64), 0, s, a);
         else if (a.method == FRINGE_METHOD_MLE)
-            hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_MLE, 64>), dim3(blocks), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_MLE, 64, false>), dim3(blocks), dim3(64), 0, s, a);
         else
-            hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_EVD, 64>), dim3(blocks), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_EVD, 64, false>), dim3(blocks), dim3(64), 0, s, a);
     }
-    else launch_solve<T, Tg, 128, 1>(a, s);   // 64 < n <= 128: two waves per pixel
+}
+
+// Workgroups of the FP64 re-solve; each loops over its share of the list.
+constexpr int kResolveBlocks = 1024;
+
+// FP64 re-solve (Gram and estimator chain of the plain FP64 path) of the
+// pixels the FP32 solver of this tile listed as not certified (n <= 64).
+static void launch_resolve(const LinkPlan& P, const uint32_t* wts, SolveArgs a, hipStream_t s) {
+    GramArgs g{};
+    g.slc = a.slc;
+    g.wts = wts;
+    g.rows = P.rows;
+    g.cols = P.cols;
+    g.n = P.n;
+    g.Nx = P.Nx;
+    g.Ny = P.Ny;
+    g.nul = P.nul;
+    g.min_nb = P.min_nb;
+    g.npx = P.npx;
+    a.g64 = g;
+    a.list = P.d_cert_list;
+    a.list_count = P.d_cert_count;
+    a.list_cap = P.cert_cap;
+    a.cert_list = nullptr;
+    a.cert_count = nullptr;
+    a.cert_cap = 0;
+    a.group_bytes = resolve_group_bytes(P.n, P.Nx, P.Ny);
+    const int gpw = P.n <= 32 ? 2 : 1;
+    const size_t lds = a.group_bytes * gpw;
+    if (P.n <= 32) {
+        if (a.module == FRINGE_MODULE_PHASE_LINK)
+            hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_EMI, 32, true>), dim3(kResolveBlocks), dim3(64), lds, s, a);
+        else if (a.method == FRINGE_METHOD_MLE)
+            hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_MLE, 32, true>), dim3(kResolveBlocks), dim3(64), lds, s, a);
+        else
+            hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_EVD, 32, true>), dim3(kResolveBlocks), dim3(64), lds, s, a);
+    } else {
+        if (a.module == FRINGE_MODULE_PHASE_LINK)
+            hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_EMI, 64, true>), dim3(kResolveBlocks), dim3(64), lds, s, a);
+        else if (a.method == FRINGE_METHOD_MLE)
+            hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_MLE, 64, true>), dim3(kResolveBlocks), dim3(64), lds, s, a);
+        else
+            hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_EVD, 64, true>), dim3(kResolveBlocks), dim3(64), lds, s, a);
+    }
+}
+
+size_t resolve_group_bytes(int n, int Nx, int Ny) {
+    return (((size_t)n * n * sizeof(cx<double>) + 255) & ~(size_t)255) + gram_f64_group_bytes(n, Nx, Ny, 32);
+}
+
+size_t resolve_lds_bytes(int n, int Nx, int Ny) {
+    const int gpw = n <= 32 ? 2 : 1;
+    const size_t stat = n <= 32 ? sizeof(LdsR<double, 32>) * 2 : sizeof(LdsR<double, 64>);
+    return resolve_group_bytes(n, Nx, Ny) * gpw + stat;
+}
+
+__global__ void zero_i32(int* p, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0;
 }
 
 bool link_uses_mfma(int n, int precision) { return precision != FRINGE_PREC_FP64 && n <= 64; }
+bool link_certifies(int n, int precision) { return precision != FRINGE_PREC_FP64 && n <= 64; }
 
 size_t link_gamma_bytes_per_pixel(int n, int precision) {
     const size_t e = link_uses_mfma(n, precision) ? 8 : 16;
@@ -270,6 +199,9 @@ int link_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, float
     const long long total = (long long)(P.row_end - P.row_begin) * P.cols;
     const bool fp64 = (P.precision == FRINGE_PREC_FP64);
     const bool mfma = link_uses_mfma(P.n, P.precision);
+    if (P.d_cert_count) {
+        hipLaunchKernelGGL(zero_i32, dim3(1), dim3(64), 0, s, P.d_cert_count, 1);
+    }
     if (mfma) {
         StageScope sc(STAGE_PACK, s);
         if (pack_launch(slc, P.rows, P.cols, P.n, P.precision != FRINGE_PREC_MIXED_BF16, P.d_pack, s) != 0)
@@ -304,6 +236,9 @@ int link_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, float
         }
         }
         SolveArgs a{};
+        a.cert_list = P.d_cert_list;
+        a.cert_count = P.d_cert_count;
+        a.cert_cap = P.cert_cap;
         a.gamma = P.d_gamma;
         a.npix = P.d_npix;
         a.slc = slc;
@@ -320,10 +255,30 @@ int link_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, float
         a.comp = comp;
         {
             StageScope sc(STAGE_SOLVE, s);
-            if (fp64) dispatch_solve<double, double>(a, s);
-            else if (mfma) dispatch_solve<float, float>(a, s);
-            else dispatch_solve<float, double>(a, s);
+            if (P.n <= 64) {
+                if (fp64) dispatch_reg<double, double>(a, s);
+                else dispatch_reg<float, float>(a, s);
+            } else {
+                if (fp64) launch_solve<double, double, 128, 1>(a, s);   // 64 < n <= 128: two waves per pixel
+                else launch_solve<float, double, 128, 1>(a, s);
+            }
         }
+        if (hipGetLastError() != hipSuccess) return -1;
+    }
+    if (P.d_cert_count) {
+        StageScope sc(STAGE_RESOLVE, s);
+        SolveArgs a{};
+        a.slc = slc;
+        a.npx = P.npx;
+        a.n = P.n;
+        a.module = P.module;
+        a.method = P.method;
+        a.ref = P.ref;
+        a.bw = P.bw;
+        a.phase = phase;
+        a.tcorr = tcorr;
+        a.comp = comp;
+        launch_resolve(P, wts, a, s);
         if (hipGetLastError() != hipSuccess) return -1;
     }
     return 0;

@@ -10,6 +10,7 @@
 #pragma once
 #include "device_common.hpp"
 #include "fringe_amd.h"
+#include "gram64.hpp"
 
 namespace fringe {
 
@@ -25,7 +26,34 @@ struct SolveArgs {
     float2* phase;       // [band][row][col]
     float* tcorr;        // [row][col]
     float2* comp;        // [row][col]
+    // FP32 solver: linear pixel indices whose result it cannot certify to the
+    // phase tolerance are appended here (cert_count entries, at most cert_cap)
+    int* cert_list;
+    int* cert_count;
+    int cert_cap;
+    // FP64 re-solve of a pixel list: entry e is pixel list[e], e < *list_count
+    // (at most list_cap); the coherence is recomputed by the FP64 Gram (g64)
+    const int* list;
+    const int* list_count;
+    int list_cap;
+    GramArgs g64;
 };
+
+// Certification of an FP32 eigenpair.  The FP32 chain's backward error is
+// C eps32 |H| for a small C, so the selected unit eigenvector v moves by
+// about C eps32 |H| / gap, and the phase of its smallest component by that
+// over |v_t|.  Against the FP64 oracle, pixels with small gaps and small
+// components showed C between 0.3 and 2.4 (|H| taken as the spectral spread;
+// |T| below bounds it from above).  A result counts as certified when the
+// estimate with C = 5 stays below 5e-4 rad, half the phase tolerance: no
+// eigenvalue of T may lie within 1e4 eps32 |T| / min|v_t| of the selected
+// one.  Decisions against the reference's 1e-6 eigenvalue thresholds are
+// certified 256 eps32 |T| away from them, and positive definiteness of
+// |Gamma| with its smallest sweep pivot above 1e-4 (|Gamma| has unit
+// diagonal, so its pivots lie in (0, 1] when it is definite).
+constexpr float kCertScale = 1.0e4f;
+constexpr float kCertThrMargin = 256.0f;
+constexpr double kCertPivotMin = 1.0e-4;
 
 template <typename T> struct Lim;
 template <> struct Lim<float> {

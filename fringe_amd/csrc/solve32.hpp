@@ -168,10 +168,12 @@ __device__ __forceinline__ bool abs_gamma_block(double (&M)[Blk<NR>::BR][8], con
 // blocks hold -A^-1.  Returns whether every pivot was positive (A is
 // positive definite).
 template <typename T, int NR>
-__device__ __forceinline__ bool sweep_block(LdsR<T, NR>& L, double (&M)[Blk<NR>::BR][8], int n, int r) {
+__device__ __forceinline__ bool sweep_block(LdsR<T, NR>& L, double (&M)[Blk<NR>::BR][8], int n, int r,
+                                            double& pmin) {
     constexpr int BR = Blk<NR>::BR, NCB = Blk<NR>::NCB;
     const int a = r / NCB, b = r % NCB;
     bool ok = true;
+    pmin = 1.0;
     for (int q = 0; 8 * q < n; ++q) {
 #pragma unroll
         for (int kj = 0; kj < 8; ++kj) {
@@ -192,6 +194,7 @@ __device__ __forceinline__ bool sweep_block(LdsR<T, NR>& L, double (&M)[Blk<NR>:
             wsync();
             const double p = L.PV;
             ok = ok && (p > 0.0);
+            pmin = fmin(pmin, p);
             const double piv = fast_rcp(p);
             double pc[8], g[BR];
 #pragma unroll
@@ -325,7 +328,7 @@ __device__ __forceinline__ void tridiag(LdsR<T, NR>& L, cx<T> (&h)[NR], int n, i
 
 // ---- eigenvalue k (0-based ascending) of tridiag(D, E) --------------------
 template <typename T, int NR>
-__device__ __forceinline__ T bisect(LdsR<T, NR>& L, int n, int k, int r, T& tnorm) {
+__device__ __forceinline__ T bisect(LdsR<T, NR>& L, int n, int k, int r, T& tnorm, T& pm) {
     T lo = T(3.0e38), hi = T(-3.0e38), em2 = T(0), tn = T(0);
     if (r < n) {
         const T a = (r > 0) ? fabs(L.E[r - 1]) : T(0);
@@ -345,6 +348,7 @@ __device__ __forceinline__ T bisect(LdsR<T, NR>& L, int n, int k, int r, T& tnor
     wsync();
     tnorm = tn;
     const T pivmin = Lim<T>::tiny() * max(T(1), em2);
+    pm = pivmin;
     const T pad = T(4) * Lim<T>::eps() * tn + T(2) * pivmin;
     lo -= pad;
     hi += pad;
@@ -371,6 +375,40 @@ __device__ __forceinline__ T bisect(LdsR<T, NR>& L, int n, int k, int r, T& tnor
         hi = nhi;
     }
     return T(0.5) * (lo + hi);
+}
+
+// ---- certification of an FP32 eigenpair (criteria in link_solve.hpp) ------
+// Number of eigenvalues of tridiag(D, E) below sig (the Sturm count of
+// bisect, evaluated by every lane alike).
+template <typename T, int NR>
+__device__ __forceinline__ int sturm_below(const LdsR<T, NR>& L, int n, T sig, T pivmin) {
+    T q = L.D[0] - sig;
+    if (fabs(q) < pivmin) q = -pivmin;
+    int cnt = q < T(0);
+    for (int i = 1; i < n; ++i) {
+        q = (L.D[i] - sig) - L.E2[i - 1] * fast_rcp(q);
+        if (fabs(q) < pivmin) q = -pivmin;
+        cnt += q < T(0);
+    }
+    return cnt;
+}
+
+// True when the eigenvector y (lane r holds component r) of eigenvalue lam
+// of tridiag(D, E) (norm bound tn) is not certified: another eigenvalue lies
+// within kCertScale eps |T| / min_t |v_t| of lam, or v has a zero or
+// non-finite component.  `largest`: lam is the largest eigenvalue, else the
+// smallest.
+template <typename T, int NR>
+__device__ __forceinline__ bool eig_uncertain(const LdsR<T, NR>& L, int n, int r, T lam, T tn, T pivmin,
+                                              bool largest, cx<T> y) {
+    const T a2 = (r < n) ? cabs2(y) : T(0);
+    const T s2 = sumR<NR>(a2);
+    const T m2 = minR<NR>((r < n) ? a2 : T(3.0e38));
+    const T vmin = sqrt(m2 / s2);
+    const T delta = T(kCertScale) * Lim<T>::eps() * tn / vmin;
+    if (!(delta <= tn)) return true;
+    const int c = sturm_below(L, n, largest ? lam - delta : lam + delta, pivmin);
+    return largest ? (c <= n - 2) : (c >= 2);
 }
 
 // ---- eigenvector of tridiag(D, E) for lam: 2 rounds of inverse iteration --
@@ -516,84 +554,99 @@ enum { MODE_EMI = 0, MODE_EVD = 1, MODE_MLE = 2 };
 // MODE_EMI: phase_link (EMI with EVD fallback, phase_link.cpp:540-618)
 // MODE_EVD: evd EVD / STBAS (evd.cpp:689-732)
 // MODE_MLE: evd MLE with its failure codes (evd.cpp:584-688)
-// NR = 32: two pixels per wave (n <= 32); NR = 64: one pixel per wave
-// (32 < n <= 64).  The body is the kernel itself: handing the LDS array to a
-// shared device function instead costs the NR = 32 FP32 kernel 56 VGPRs and
-// half its occupancy.
-template <typename T, typename Tg, int MODE, int NR>
-__global__ __launch_bounds__(64) void solve_reg_kernel(SolveArgs a) {
-    __shared__ LdsR<T, NR> sm[64 / NR];
-    const int grp = threadIdx.x / NR, r = threadIdx.x % NR;
-    LdsR<T, NR>& L = sm[grp];
-    // slots last-first: the Gram of this round has just written them in
-    // tile order, and the most recently written coherence matrices are the
-    // ones still in the last-level cache (measured 1.3 % on the solve)
-    const int slot = (gridDim.x - 1 - blockIdx.x) * (64 / NR) + grp;
+// One pixel's estimator chain on the NR lanes of its group: gam is its
+// coherence (global memory, or LDS in the FP64 re-solve), np its neighbour
+// count (-1: not linked).  CERT: the FP32 result is checked and, when it is
+// not certified, the pixel is appended to a.cert_list for the FP64 re-solve.
+template <typename T, typename Tg, int MODE, int NR, bool CERT>
+__device__ __forceinline__ void solve_reg_pixel(const SolveArgs& a, LdsR<T, NR>& L, int r, bool inslot, int np,
+                                                const cx<Tg>* gam, long long pix) {
     const int n = a.n;
-    const bool inslot = slot < a.slots;
-    const int np = inslot ? a.npix[slot] : -1;
     const bool act = np >= 0;
-    const cx<Tg>* gam = (const cx<Tg>*)a.gamma + (size_t)(inslot ? slot : 0) * n * n;
     const cx<Tg>* grow = gam + (size_t)min(r, n - 1) * n;
-    const long long pix = a.first + (inslot ? slot : 0);
 
     float code = 0.0f;
     bool failed = false;
     cx<T> h[NR];
     cx<T> y = {T(0), T(0)};
-    T tn;
+    T tn, pm;
+    bool unc = false;   // CERT: the FP32 result is not certified
 
     if constexpr (MODE == MODE_MLE) {
         const bool fin = load_rows<T, Tg, NR>(h, grow, n, r, act, -1);
         tridiag<T, NR>(L, h, n, r);
-        const T lmin = bisect<T, NR>(L, n, 0, r, tn);
+        const T lmin = bisect<T, NR>(L, n, 0, r, tn, pm);
         if (!fin) { code = -1.0f; failed = true; }
         else if (!(lmin >= T(1.0e-6))) { code = -2.0f; failed = true; }
+        if constexpr (CERT) unc = !(lmin >= T(1.0e-6) + T(kCertThrMargin) * Lim<T>::eps() * tn);
         double M[Blk<NR>::BR][8], m[NR];
+        double pmin1, pmin2;
         abs_gamma_block<NR, Tg>(M, gam, n, r, act, 1.0e-6);
-        const bool pd_shift = sweep_block<T, NR>(L, M, n, r);
+        const bool pd_shift = sweep_block<T, NR>(L, M, n, r, pmin1);
         if (!failed && !pd_shift) { code = -4.0f; failed = true; }
+        if constexpr (CERT) unc = unc || !(pmin1 >= kCertPivotMin);
         abs_gamma_block<NR, Tg>(M, gam, n, r, act, 0.0);
-        const bool pd = sweep_block<T, NR>(L, M, n, r);
+        const bool pd = sweep_block<T, NR>(L, M, n, r, pmin2);
         if (!failed && !pd) { code = -5.0f; failed = true; }
+        if constexpr (CERT) unc = unc || !(pmin2 >= kCertPivotMin);
         blocks_to_rows<T, NR>(L, M, m, r);
         const bool hfin = hadamard_rows<T, Tg, NR>(h, m, grow, n, r, act);
         tridiag<T, NR>(L, h, n, r);
-        const T lam = bisect<T, NR>(L, n, 0, r, tn);
+        const T lam = bisect<T, NR>(L, n, 0, r, tn, pm);
         const T yr = tri_eigvec<T, NR>(L, n, r, lam, tn);
         y = back_transform<T, NR>(L, h, n, r, cx<T>{yr, T(0)});
         if (!failed && !hfin) { code = -6.0f; failed = true; }
         if (!failed && !(lam >= T(1.0e-6))) { code = -7.0f; failed = true; }
+        if constexpr (CERT) {
+            unc = unc || !(lam >= T(1.0e-6) + T(kCertThrMargin) * Lim<T>::eps() * tn);
+            const bool ue = eig_uncertain<T, NR>(L, n, r, lam, tn, pm, false, y);
+            unc = fin && (unc || (!failed && ue));
+        }
     } else if constexpr (MODE == MODE_EVD) {
         const int band = (a.method == FRINGE_METHOD_STBAS) ? a.bw : -1;
         const bool fin = load_rows<T, Tg, NR>(h, grow, n, r, act, band);
         tridiag<T, NR>(L, h, n, r);
-        const T lam = bisect<T, NR>(L, n, n - 1, r, tn);
+        const T lam = bisect<T, NR>(L, n, n - 1, r, tn, pm);
         const T yr = tri_eigvec<T, NR>(L, n, r, lam, tn);
         y = back_transform<T, NR>(L, h, n, r, cx<T>{yr, T(0)});
         if (!fin) { code = -6.0f; failed = true; }
         else if (!(lam >= T(1.0e-6))) { code = -7.0f; failed = true; }
+        if constexpr (CERT) {
+            const bool ue = eig_uncertain<T, NR>(L, n, r, lam, tn, pm, true, y);
+            unc = fin && (ue || !(lam >= T(1.0e-6) + T(kCertThrMargin) * Lim<T>::eps() * tn));
+        }
     } else {
         double M[Blk<NR>::BR][8], m[NR];
+        double pmin;
         bool fin = abs_gamma_block<NR, Tg>(M, gam, n, r, act, 0.0);
         fin = minR<NR>(fin ? 1.0f : 0.0f) > 0.5f;
-        const bool pd = sweep_block<T, NR>(L, M, n, r);
+        const bool pd = sweep_block<T, NR>(L, M, n, r, pmin);
         blocks_to_rows<T, NR>(L, M, m, r);
         const bool hfin = hadamard_rows<T, Tg, NR>(h, m, grow, n, r, act);
         const bool run_evd = !pd || !hfin;
         tridiag<T, NR>(L, h, n, r);
-        const T lam = bisect<T, NR>(L, n, 0, r, tn);
+        const T lam = bisect<T, NR>(L, n, 0, r, tn, pm);
         const T yr = tri_eigvec<T, NR>(L, n, r, lam, tn);
         y = back_transform<T, NR>(L, h, n, r, cx<T>{yr, T(0)});
+        if constexpr (CERT) {
+            // every lane runs the check (it holds cross-lane reductions)
+            const bool ue = eig_uncertain<T, NR>(L, n, r, lam, tn, pm, false, y);
+            unc = !(pmin >= kCertPivotMin) || (!run_evd && ue);
+        }
         if (__any(act && fin && run_evd)) {
             load_rows<T, Tg, NR>(h, grow, n, r, act, -1);
             tridiag<T, NR>(L, h, n, r);
-            const T le = bisect<T, NR>(L, n, n - 1, r, tn);
+            const T le = bisect<T, NR>(L, n, n - 1, r, tn, pm);
             const T ye = tri_eigvec<T, NR>(L, n, r, le, tn);
             const cx<T> yv = back_transform<T, NR>(L, h, n, r, cx<T>{ye, T(0)});
+            if constexpr (CERT) {
+                const bool ue = eig_uncertain<T, NR>(L, n, r, le, tn, pm, true, yv);
+                if (run_evd) unc = unc || ue;
+            }
             if (run_evd) y = yv;
         }
         if (!fin) { code = -1.0f; failed = true; }
+        if constexpr (CERT) unc = unc && fin;
     }
 
     // ---- referenced phases (evd.cpp:738-749) ----
@@ -677,6 +730,53 @@ __global__ __launch_bounds__(64) void solve_reg_kernel(SolveArgs a) {
         }
         a.tcorr[pix] = tco;
         a.comp[pix] = co;
+        if constexpr (CERT) {
+            if (act && unc) {
+                const int e = atomicAdd(a.cert_count, 1);
+                if (e < a.cert_cap) a.cert_list[e] = (int)pix;
+            }
+        }
+    }
+}
+
+// LIST = false: slot s of the round's coherence buffer is pixel first + s;
+// NR = 32 holds two pixels per wave, NR = 64 one.  The FP32 instances
+// certify their results.  LIST = true (T = Tg = double): the FP64 re-solve
+// of the listed pixels; each group computes its pixel's coherence with the
+// FP64 Gram into LDS and solves it there, looping over the list.
+template <typename T, typename Tg, int MODE, int NR, bool LIST>
+__global__ __launch_bounds__(64) void solve_reg_kernel(SolveArgs a) {
+    __shared__ LdsR<T, NR> sm[64 / NR];
+    const int grp = threadIdx.x / NR, r = threadIdx.x % NR;
+    LdsR<T, NR>& L = sm[grp];
+    if constexpr (!LIST) {
+        // slots last-first: the Gram of this round has just written them in
+        // tile order, and the most recently written coherence matrices are the
+        // ones still in the last-level cache (measured 1.3 % on the solve)
+        const int slot = (gridDim.x - 1 - blockIdx.x) * (64 / NR) + grp;
+        const bool inslot = slot < a.slots;
+        const int np = inslot ? a.npix[slot] : -1;
+        const cx<Tg>* gam = (const cx<Tg>*)a.gamma + (size_t)(inslot ? slot : 0) * a.n * a.n;
+        const long long pix = a.first + (inslot ? slot : 0);
+        solve_reg_pixel<T, Tg, MODE, NR, sizeof(T) == 4>(a, L, r, inslot, np, gam, pix);
+    } else {
+        static_assert(sizeof(T) == 8 && sizeof(Tg) == 8, "the re-solve runs in FP64");
+        constexpr int GPW = 64 / NR;
+        extern __shared__ __align__(16) char dyn[];
+        const int n = a.n;
+        char* base = dyn + grp * a.group_bytes;
+        cx<double>* gl = (cx<double>*)base;
+        char* gscratch = base + (((size_t)n * n * sizeof(cx<double>) + 255) & ~(size_t)255);
+        const int cnt = min(*a.list_count, a.list_cap);
+        for (int b0 = blockIdx.x * GPW; b0 < cnt; b0 += gridDim.x * GPW) {
+            const int e = b0 + grp;
+            const bool valid = e < cnt;
+            const long long pix = valid ? (long long)a.list[e] : 0;
+            const int np = gram_f64_pixel<NR, 1, 32>(a.g64, valid, pix, gl, gscratch);
+            __syncthreads();
+            solve_reg_pixel<T, Tg, MODE, NR, false>(a, L, r, valid, np, (const cx<Tg>*)gl, pix);
+            __syncthreads();
+        }
     }
 }
 

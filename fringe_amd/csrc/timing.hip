@@ -16,9 +16,9 @@ std::mutex g_mu;
 bool g_on = false;
 std::vector<Pending> g_pending;
 std::vector<hipEvent_t> g_pool;
-double g_ms[STAGE_COUNT] = {0, 0, 0, 0, 0};
-long long g_calls[STAGE_COUNT] = {0, 0, 0, 0, 0};
-thread_local hipEvent_t t_open[STAGE_COUNT] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+double g_ms[STAGE_COUNT] = {};
+long long g_calls[STAGE_COUNT] = {};
+thread_local hipEvent_t t_open[STAGE_COUNT] = {};
 
 hipEvent_t take_event() {
     if (!g_pool.empty()) {
@@ -30,7 +30,7 @@ hipEvent_t take_event() {
     if (hipEventCreate(&e) != hipSuccess) return nullptr;
     return e;
 }
-const char* kNames[STAGE_COUNT] = {"pack", "gram", "solve", "nmap_prep", "nmap_ks"};
+const char* kNames[STAGE_COUNT] = {"pack", "gram", "solve", "nmap_prep", "nmap_ks", "resolve"};
 }  // namespace
 
 void timing_begin(int stage, hipStream_t s) {

@@ -7,7 +7,7 @@
 namespace fringe {
 
 enum Stage { STAGE_PACK = 0, STAGE_GRAM = 1, STAGE_SOLVE = 2, STAGE_NMAP_PREP = 3, STAGE_NMAP_KS = 4,
-             STAGE_COUNT = 5 };
+             STAGE_RESOLVE = 5, STAGE_COUNT = 6 };
 
 void timing_begin(int stage, hipStream_t s);
 void timing_end(int stage, hipStream_t s);

@@ -83,6 +83,19 @@ class LinkRunner:
         _check(rc, "fringe_link_run")
         return self.phase, self.tcorr, self.comp
 
+    def resolved_pixels(self):
+        """Linear indices (row * cols + col) of the pixels that the last run
+        recomputed with the FP64 path because their FP32 result was not
+        certified (mixed precisions; empty otherwise).  Synchronises."""
+        torch.cuda.synchronize(self.work.device)
+        cnt = _lib.lib().fringe_link_resolved_pixels(ctypes.byref(self.p), _ptr(self.work), self.work_bytes)
+        if cnt < 0:
+            raise RuntimeError("fringe_link_resolved_pixels failed")
+        if cnt == 0:
+            return np.zeros(0, np.int64)
+        lst = self.work[256:256 + 4 * cnt].view(torch.int32).cpu().numpy()
+        return np.sort(lst.astype(np.int64))
+
 
 def link(slc, wts, Nx, Ny, **kw):
     """Phase-link a device tile.  slc: complex64 [n, rows, cols] on the GPU;

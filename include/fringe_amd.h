@@ -130,9 +130,21 @@ typedef struct fringe_link_tile {
 } fringe_link_tile;
 
 size_t fringe_link_workspace_bytes(const fringe_link_tile* t);
+/* Replaces the per-block pixel loops of evd_process (evd.cpp:512-788) and
+ * phase_link's evd_process (phase_link.cpp:479-666) on a device tile.
+ * Mixed precisions (nbands <= 64) certify every FP32 eigenpair: a pixel whose
+ * eigen-gap, smallest eigenvector component or distance from a 1e-6
+ * eigenvalue threshold / positive-definiteness boundary does not guarantee
+ * 5e-4 rad is recomputed end to end by the FP64 path (FP64 Gram and FP64
+ * solver) in the same call, on the same stream. */
 int fringe_link_run(const fringe_link_tile* t, const void* d_slc, const uint32_t* d_wts,
                     void* d_phase, float* d_tcorr, void* d_comp, void* d_work,
                     size_t work_bytes, void* stream);
+/* Pixels of the last fringe_link_run on this workspace that the FP64
+ * re-solve recomputed (0 when the precision does not certify); the caller
+ * synchronises the run's stream first.  No counterpart in the reference: a
+ * diagnostic of the mixed-precision path.  Returns -1 on bad arguments. */
+long long fringe_link_resolved_pixels(const fringe_link_tile* t, const void* d_work, size_t work_bytes);
 
 /* Coherence matrices alone (the first stage of fringe_link_run,
  * evd.cpp:530-582): for every output pixel, slot = (row - row_begin) * cols

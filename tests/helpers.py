@@ -28,11 +28,13 @@ def wts_dev(wts, torch):
 
 
 def compare_link(gpu, ref, gap=None, phase_tol=PHASE_TOL, tcorr_tol=TCORR_TOL, gap_min=GAP_MIN, vmin=None,
-                 vmin_min=VMIN_MIN):
+                 vmin_min=VMIN_MIN, stack=None, ref_index=0):
     """Return a dict of mismatch statistics between GPU and oracle outputs.
     Pixels count as well posed when processed, with eigen-gap >= gap_min and
     (when vmin is given) eigenvector components no smaller than vmin_min of
-    the RMS component."""
+    the RMS component.  With the stack given, comp_err is the compressed-SLC
+    error over the mean |z| of the dates it sums (evd.cpp:755-762): at most
+    the largest phase error, plus rounding."""
     ph_g, tc_g, co_g = gpu
     ph_r, tc_r, co_r = ref["phase"], ref["tcorr"], ref["comp"]
     ok_r = tc_r > 0            # processed, not failed
@@ -52,4 +54,8 @@ def compare_link(gpu, ref, gap=None, phase_tol=PHASE_TOL, tcorr_tol=TCORR_TOL, g
     # compressed SLC relative error on well-posed pixels
     den = np.maximum(np.abs(co_r), 1e-30)
     out["comp_rel_max"] = float((np.abs(co_g - co_r) / den)[well].max()) if well.any() else 0.0
+    if stack is not None:
+        amp = np.abs(stack[ref_index:]).astype(np.float64).mean(axis=0)
+        e = np.abs(co_g.astype(np.complex128) - co_r) / np.maximum(amp, 1e-30)
+        out["comp_err"] = float(e[well].max()) if well.any() else 0.0
     return out

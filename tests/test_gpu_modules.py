@@ -143,13 +143,14 @@ def test_sequential_against_oracle_replay(tmp_path):
         assert np.abs(tc - ref["tcorr"])[ok].max() <= TCORR_TOL
         rel = np.abs(comp - ref["comp"])[ok] / np.maximum(np.abs(ref["comp"][ok]), 1e-6)
         assert rel.max() < 1e-3
-        comps.append(ref["comp"])   # the next mini-stack sees the oracle's compressed SLC
-    # datum connection
+        # the next mini-stack (and its replay) sees this run's compressed SLC
+        comps.append(comp)
+    # datum connection, replayed on the same compressed SLCs
     dc = str(tmp_path / "seq" / "Datum_connection" / "EVD")
     ref = oracle.link(np.array(comps).astype(np.complex64), wts, Nx, Ny, module="evd", method="MLE")
     ph = np.stack([raster.read(os.path.join(dc, dates[min(n, s0 + L) - 1] + ".slc")) for s0 in range(0, n, L)])
     ok = ref["tcorr"] > 0
-    assert wrapped_err(ph, ref["phase"])[:, ok].max() <= 5 * PHASE_TOL
+    assert wrapped_err(ph, ref["phase"])[:, ok].max() <= PHASE_TOL
 
 
 def test_sequential_uneven_ministacks_120_dates(tmp_path):
@@ -193,12 +194,12 @@ def test_sequential_uneven_ministacks_120_dates(tmp_path):
         comp = raster.read(str(tmp_path / "seq" / "compressedSlc" / dates[s1 - 1] / (dates[s1 - 1] + ".slc")))
         ok = (ref["tcorr"] > 0) & (ref["gap"] >= 1e-3)
         assert ok.sum() > 0.9 * rows * cols
-        assert np.array_equal(ref["tcorr"] > 0, tc > 0)
+        assert np.array_equal(np.where(ref["tcorr"] <= 0, ref["tcorr"], 1), np.where(tc <= 0, tc, 1))
         assert wrapped_err(ph, ref["phase"])[:, ok].max() <= PHASE_TOL
         assert np.abs(tc - ref["tcorr"])[ok].max() <= TCORR_TOL
-        rel = np.abs(comp - ref["comp"])[ok] / np.maximum(np.abs(ref["comp"][ok]), 1e-6)
-        assert rel.max() < 1e-3
-        comps.append(ref["comp"])
+        amp = np.abs(stack[k - 1:]).mean(axis=0)
+        assert (np.abs(comp - ref["comp"]) / amp)[ok].max() <= PHASE_TOL
+        comps.append(comp)   # the next mini-stack (and its replay) sees this run's compressed SLC
         ends.append(dates[s1 - 1])
         s0 = s1
     dc = str(tmp_path / "seq" / "Datum_connection" / "EVD")
@@ -207,7 +208,7 @@ def test_sequential_uneven_ministacks_120_dates(tmp_path):
     ph = np.stack([raster.read(os.path.join(dc, d + ".slc")) for d in ends])
     ok = (ref["tcorr"] > 0) & (ref["gap"] >= 1e-3)
     assert ok.sum() > 0.9 * rows * cols
-    assert wrapped_err(ph, ref["phase"])[:, ok].max() <= 5 * PHASE_TOL
+    assert wrapped_err(ph, ref["phase"])[:, ok].max() <= PHASE_TOL
 
 
 def test_datum_adjustment_wrapped_and_unwrapped(tmp_path):

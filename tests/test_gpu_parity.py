@@ -19,14 +19,19 @@ import oracle  # noqa: E402
 from fringe_amd import synthetic, tile  # noqa: E402
 
 
-def _run_both(slc, wts, Nx, Ny, module, method, precision, min_nb=2, ref=0, bw=-1, want_gap=True):
+def _run_both(slc, wts, Nx, Ny, module, method, precision, min_nb=2, ref=0, bw=-1, want_gap=True,
+              want_resolved=False):
     r = oracle.link(slc, wts, Nx, Ny, module=module, method=method, min_neighbors=min_nb,
                     mini_stack_count=ref + 1, bandwidth=bw, want_gap=want_gap)
-    ph, tc, co = tile.link(to_dev(slc, torch), wts_dev(wts, torch), Nx, Ny, module=module,
-                           method=method, min_neighbors=min_nb, ref_index=ref, bandwidth=bw,
-                           precision=precision)
+    n, rows, cols = slc.shape
+    runner = tile.LinkRunner(tile.link_params(rows, cols, n, Nx, Ny, module=module, method=method,
+                                              min_neighbors=min_nb, ref_index=ref, bandwidth=bw,
+                                              precision=precision))
+    ph, tc, co = runner.run(to_dev(slc, torch), wts_dev(wts, torch))
     torch.cuda.synchronize()
     g = (ph.cpu().numpy(), tc.cpu().numpy(), co.cpu().numpy())
+    if want_resolved:
+        return g, r, runner.resolved_pixels()
     return g, r
 
 
@@ -38,12 +43,13 @@ def test_link_full_mask_small(precision, module, method):
     slc = synthetic.stack(n, rows, cols, seed=101)
     wts = synthetic.full_mask(rows, cols, Nx, Ny)
     g, r = _run_both(slc, wts, Nx, Ny, module, method, precision)
-    st = compare_link(g, r, r["gap"])
+    st = compare_link(g, r, r["gap"], stack=slc)
     print(st)
-    assert st["untouched_equal"]
+    assert st["untouched_equal"] and st["codes_equal"]
     assert st["n_well"] > 0.9 * rows * cols
     assert st["phase_max"] <= PHASE_TOL
     assert st["tcorr_max"] <= TCORR_TOL
+    assert st["comp_err"] <= PHASE_TOL
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
@@ -53,11 +59,12 @@ def test_link_n30_nmap_mask(precision):
     slc = synthetic.stack(n, rows, cols, seed=102, patches=8)
     wts, cnt = oracle.nmap(slc, Nx, Ny, 0.05)
     g, r = _run_both(slc, wts, Nx, Ny, "phase_link", "MLE", precision, min_nb=5)
-    st = compare_link(g, r, r["gap"])
+    st = compare_link(g, r, r["gap"], stack=slc)
     print(st)
-    assert st["untouched_equal"]
+    assert st["untouched_equal"] and st["codes_equal"]
     assert st["phase_max"] <= PHASE_TOL
     assert st["tcorr_max"] <= TCORR_TOL
+    assert st["comp_err"] <= PHASE_TOL
 
 
 @pytest.mark.parametrize("method", ["KS2", "AD2"])
@@ -101,14 +108,14 @@ def test_nmap_row_range(n, method):
 
 
 def _full_rank_gap(gap, rows, cols, Nx, Ny, n, precision):
-    """For the float-Gamma precisions: gap with pixels whose (full-mask)
-    window holds fewer than n samples marked ill-posed.  Their sample
-    coherence is rank deficient: |Gamma| is near singular, so MLE / EMI turn
-    the ~1e-6 error of a float Gamma into phase errors of any size (an FP64
-    solve of the same float Gamma misses as well), and the EVD eigen-gaps
+    """Above 64 dates, for the float-solver precision: gap with pixels whose
+    (full-mask) window holds fewer than n samples marked ill-posed.  Their
+    sample coherence is rank deficient: |Gamma| is near singular, so MLE / EMI
+    turn FP32 rounding into phase errors of any size, and the EVD eigen-gaps
     shrink to where n * 1e-6 / gap exceeds 1e-3 rad.  FP64 keeps every pixel
-    the eigen-gap admits."""
-    if precision == "fp64":
+    the eigen-gap admits, and so does the mixed path up to 64 dates, which
+    re-solves every uncertified pixel in FP64."""
+    if precision == "fp64" or n <= 64:
         return gap
     r, c = np.arange(rows)[:, None], np.arange(cols)[None, :]
     k = ((np.minimum(r + Ny, rows - 1) - np.maximum(r - Ny, 0) + 1)
@@ -127,16 +134,15 @@ def test_link_64row_solver(n, module, method, precision):
     slc = synthetic.stack(n, rows, cols, seed=300 + n)
     wts = synthetic.full_mask(rows, cols, Nx, Ny)
     g, r = _run_both(slc, wts, Nx, Ny, module, method, precision)
-    st = compare_link(g, r, _full_rank_gap(r["gap"], rows, cols, Nx, Ny, n, precision),
-                      vmin=None if precision == "fp64" else r["vmin"])
+    st = compare_link(g, r, r["gap"], stack=slc)
     print(st)
     assert st["untouched_equal"]
-    if precision == "fp64":
-        assert st["codes_equal"]   # evd MLE failure codes as the reference sets them
+    assert st["codes_equal"]   # evd MLE failure codes as the reference sets them
     # at least the 6 x 6 pixels with whole windows; evd MLE's thresholds code
     # most pixels at n = 64
     assert st["n_well"] >= (1 if (module, method) == ("evd", "MLE") else 36)
     assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL
+    assert st["comp_err"] <= PHASE_TOL
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
@@ -146,10 +152,10 @@ def test_link_64row_solver_stbas_reference(precision):
     slc = synthetic.stack(n, rows, cols, seed=364)
     wts = synthetic.full_mask(rows, cols, Nx, Ny)
     g, r = _run_both(slc, wts, Nx, Ny, "evd", "STBAS", precision, bw=10, ref=3)
-    st = compare_link(g, r, _full_rank_gap(r["gap"], rows, cols, Nx, Ny, n, precision),
-                      vmin=None if precision == "fp64" else r["vmin"])
+    st = compare_link(g, r, r["gap"], stack=slc, ref_index=3)
     print(st)
-    assert st["n_well"] >= 36
+    assert st["n_well"] >= 36 and st["codes_equal"]
+    assert st["comp_err"] <= PHASE_TOL
     assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL
     assert np.all(g[0][3][r["tcorr"] > 0] == 1 + 0j)
 
@@ -202,6 +208,37 @@ def test_link_generic_solver_modes(n, module, method, precision):
         assert st["codes_equal"]
     assert st["n_well"] >= (1 if (module, method) == ("evd", "MLE") else 36)
     assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL
+
+
+@pytest.mark.parametrize("n,Nx,Ny,module,method", [(10, 29, 9, "evd", "MLE"), (30, 7, 5, "phase_link", "MLE"),
+                                                  (64, 5, 4, "evd", "MLE"), (64, 5, 4, "evd", "EVD"),
+                                                  (48, 5, 4, "phase_link", "MLE")])
+def test_mixed_resolve_equals_fp64(n, Nx, Ny, module, method):
+    """Pixels whose FP32 eigenpair the mixed path cannot certify are
+    recomputed by the FP64 path (FP64 Gram and solver) in the same call:
+    their outputs equal the plain FP64 run's bit for bit, and every other
+    pixel agrees with the oracle to the tolerances."""
+    rows, cols = 20, 40
+    print(f"case resolve n={n} {2 * Nx + 1}x{2 * Ny + 1} {module}/{method}", flush=True)
+    slc = synthetic.stack(n, rows, cols, seed=700 + n, patches=6)
+    wts, _ = oracle.nmap(slc, Nx, Ny, 0.05)
+    out = {}
+    for precision in ("fp64", "mixed"):
+        g, r, res = _run_both(slc, wts, Nx, Ny, module, method, precision, min_nb=5, want_resolved=True)
+        out[precision] = (g, res)
+    (g64, _), (gmx, res) = out["fp64"], out["mixed"]
+    print("re-solved", res.size)
+    if n >= 48:   # the corner windows hold fewer samples than dates: |Gamma| is near singular there
+        assert res.size > 0
+    rr, cc = np.divmod(res, cols)
+    for k in range(3):
+        a = g64[k][..., rr, cc] if k == 0 else g64[k][rr, cc]
+        b = gmx[k][..., rr, cc] if k == 0 else gmx[k][rr, cc]
+        assert np.array_equal(np.ascontiguousarray(a).view(np.uint32), np.ascontiguousarray(b).view(np.uint32))
+    st = compare_link(gmx, r, r["gap"], stack=slc)
+    print(st)
+    assert st["untouched_equal"] and st["codes_equal"]
+    assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL and st["comp_err"] <= PHASE_TOL
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

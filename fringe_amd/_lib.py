@@ -30,6 +30,7 @@ EXPORTS = [
     "fringe_despeck_options_init", "fringe_despeck_process",
     "fringe_calamp_options_init", "fringe_calamp_process",
     "fringe_coherence_workspace_bytes", "fringe_coherence_run", "fringe_link_resolved_pixels",
+    "fringe_link_run_noalloc",
 ]
 FRINGE_REQ_QUEUED, FRINGE_REQ_RUNNING, FRINGE_REQ_DONE = 0, 1, 2
 
@@ -97,6 +98,7 @@ def lib():
     L.fringe_link_workspace_bytes.argtypes = [ctypes.POINTER(LinkTile)]
     L.fringe_link_workspace_bytes.restype = sz
     L.fringe_link_run.argtypes = [ctypes.POINTER(LinkTile), vp, vp, vp, vp, vp, vp, sz, vp]
+    L.fringe_link_run_noalloc.argtypes = [ctypes.POINTER(LinkTile), vp, vp, vp, vp, vp, vp]
     L.fringe_link_resolved_pixels.argtypes = [ctypes.POINTER(LinkTile), vp, sz]
     L.fringe_link_resolved_pixels.restype = ctypes.c_longlong
     L.fringe_coherence_workspace_bytes.argtypes = [ctypes.POINTER(LinkTile)]

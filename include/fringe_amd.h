@@ -140,6 +140,18 @@ size_t fringe_link_workspace_bytes(const fringe_link_tile* t);
 int fringe_link_run(const fringe_link_tile* t, const void* d_slc, const uint32_t* d_wts,
                     void* d_phase, float* d_tcorr, void* d_comp, void* d_work,
                     size_t work_bytes, void* stream);
+/* fringe_link_run without a workspace and without any device allocation,
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  The workspace is a fixed 256 MiB device array that each GPU
+ * holds for the life of the process; a tile whose workspace cannot fit it
+ * (the packed records of the whole tile plus one row of coherence matrices:
+ * 1024 x 1024 pixels at 30 dates fit, 2048 x 2048 do not) is refused with
+ * FRINGE_ERR_WORKSPACE before any launch.  Calls on one GPU share the
+ * array, so calls that overlap on one GPU (other streams, other threads)
+ * race; outputs equal fringe_link_run's bit for bit. */
+int fringe_link_run_noalloc(const fringe_link_tile* t, const void* d_slc, const uint32_t* d_wts,
+                            void* d_phase, float* d_tcorr, void* d_comp, void* stream);
 /* Pixels of the last fringe_link_run on this workspace that the FP64
  * re-solve recomputed (0 when the precision does not certify); the caller
  * synchronises the run's stream first.  No counterpart in the reference: a

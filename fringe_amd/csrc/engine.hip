@@ -19,7 +19,7 @@ void set_last_error(const char* what, hipError_t e) {
 }
 void set_last_error(const std::string& what) { g_last_error = what; }
 
-static int device_lds_limit() {
+int device_lds_limit() {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 65536;
     static int cached[64] = {0};
@@ -54,7 +54,6 @@ static double kolmogorov_prob(double z) {
     return 0.0;
 }
 
-static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // ---------------------------------------------------------------------------
 // Two-sample Anderson-Darling decision (AD2unique.hpp): sigma_N of Scholz &
@@ -170,14 +169,19 @@ int link_check(const fringe_link_tile* t) {
         return FRINGE_ERR_UNSUPPORTED;
     // LDS footprint of the generic solver used above 64 dates (one pixel per
     // two-wave workgroup): the pixel's matrices plus the kernel's static LDS
-    if (link_certifies(t->nbands, t->precision) &&
-        resolve_lds_bytes(t->nbands, t->Nx, t->Ny) > (size_t)device_lds_limit())
-        return FRINGE_ERR_UNSUPPORTED;
-    if (t->nbands > 64) {
-        const size_t need = (t->precision == FRINGE_PREC_FP64 ? solve_group_bytes<double>(t->nbands)
-                                                              : solve_group_bytes<float>(t->nbands)) +
+    // LDS of the solver kernels: above 64 dates the work matrix moves to
+    // global memory when it does not fit, the rest of a pixel's group must
+    const size_t lds = (size_t)device_lds_limit();
+    if (t->nbands <= 64) {
+        if (link_certifies(t->nbands, t->precision) && resolve_lds_bytes(t->nbands, t->Nx, t->Ny) > lds)
+            return FRINGE_ERR_UNSUPPORTED;
+    } else {
+        const size_t rest = (t->precision == FRINGE_PREC_FP64 ? solve_group_bytes<double>(t->nbands, false)
+                                                              : solve_group_bytes<float>(t->nbands, false)) +
                             solve_static_lds(t->precision);
-        if (need > (size_t)device_lds_limit()) return FRINGE_ERR_UNSUPPORTED;
+        if (rest > lds) return FRINGE_ERR_UNSUPPORTED;
+        if (link_certifies(t->nbands, t->precision) && resolve_generic_lds_bytes(t->nbands, t->Nx, t->Ny) > lds)
+            return FRINGE_ERR_UNSUPPORTED;
     }
     return 0;
 }
@@ -200,7 +204,7 @@ static int link_chunk_pixels(const fringe_link_tile* t) {
 }
 
 struct LinkLayout {
-    size_t gamma, npix, pack, cert, total;
+    size_t gamma, npix, pack, cert, awork, awork_stride, rwork, total;
 };
 
 static LinkLayout link_layout(const fringe_link_tile* t, int chunk) {
@@ -217,6 +221,14 @@ static LinkLayout link_layout(const fringe_link_tile* t, int chunk) {
     b += align256((size_t)chunk * sizeof(int));
     L.pack = b;
     if (link_uses_mfma(n, t->precision)) b += align256(mfma_pack_bytes(t->rows, t->cols, n));
+    L.awork = b;
+    if (n > 64 && !solve_a_in_lds(n, t->precision)) {   // the solver's work matrices, one per slot
+        const size_t el = t->precision == FRINGE_PREC_FP64 ? 16 : 8;
+        L.awork_stride = align256((size_t)n * (n | 1) * el);
+        b += (size_t)chunk * L.awork_stride;
+    }
+    L.rwork = b;
+    if (n > 64 && link_certifies(n, t->precision)) b += (size_t)kResolveGenericBlocks * resolve_generic_area_bytes(n);
     L.total = b;
     return L;
 }
@@ -272,6 +284,11 @@ int link_run_chunked(const fringe_link_tile* t, const void* d_slc, const uint32_
         P.d_cert_list = (int*)((char*)d_work + LL.cert + 256);
         P.cert_cap = (int)((long long)(t->row_end - t->row_begin) * t->cols);
     }
+    if (LL.awork_stride) {
+        P.d_awork = (char*)d_work + LL.awork;
+        P.awork_stride = LL.awork_stride;
+    }
+    if (P.n > 64 && link_certifies(P.n, P.precision)) P.d_rwork = (char*)d_work + LL.rwork;
     hipStream_t s = (hipStream_t)stream;
     if (link_launch(P, (const float2*)d_slc, d_wts, (float2*)d_phase, d_tcorr, (float2*)d_comp, s) != 0) {
         set_last_error("fringe_link_run", hipGetLastError());

@@ -12,7 +12,8 @@
 namespace fringe {
 
 constexpr int kMaxBandsNmap = 128;   // nmap sorts at most this many dates
-constexpr int kMaxBandsLink = 128;   // phase-linking solver limit
+constexpr int kMaxBandsLink = 256;   // phase-linking solver limit
+constexpr int kResolveGenericBlocks = 256;   // workgroups of the FP64 re-solve above 64 dates
 constexpr int kMaxMaskWords = 64;    // window up to 2048 positions (the reference sequential default 59 x 19 is 1121)
 
 // Decision table of the integer KS statistic, one bit per k = D*n.
@@ -54,6 +55,12 @@ struct LinkPlan {
     int* d_cert_list;   // [cert_cap]
     int* d_cert_count;  // [1]
     int cert_cap;       // output pixels of the tile
+    // above 64 dates: the solver's work matrices when they do not fit in LDS
+    // ([chunk] areas of awork_stride bytes, null otherwise), and the FP64
+    // re-solve's per-workgroup areas (mixed precisions)
+    void* d_awork;
+    size_t awork_stride;
+    void* d_rwork;
 };
 
 // The FP64 re-solve of uncertified pixels (n <= 64): dynamic LDS per lane
@@ -63,6 +70,13 @@ size_t resolve_group_bytes(int n, int Nx, int Ny);
 size_t resolve_lds_bytes(int n, int Nx, int Ny);
 // Whether a tile of this precision and date count certifies its FP32 results.
 bool link_certifies(int n, int precision);
+int device_lds_limit();
+// Above 64 dates: whether the solver's work matrix fits in LDS; the FP64
+// re-solve's global area per workgroup and its LDS.
+bool solve_a_in_lds(int n, int precision);
+size_t resolve_generic_area_bytes(int n);
+size_t resolve_generic_lds_bytes(int n, int Nx, int Ny);
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // SHP-masked neighbourhood mean (despeck.cpp:300-433); mode 0 amplitude,
 // 1 interferogram, 2 interferogram with coherence.  c1, c2: rows*cols

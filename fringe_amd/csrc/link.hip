@@ -49,22 +49,46 @@ static void launch_gram_f64(GramArgs a, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // Solver launch
 // ---------------------------------------------------------------------------
-template <typename T, typename Tg, int G, int R>
-static void launch_solve(SolveArgs a, hipStream_t s) {
-    constexpr int GPW = G >= 64 ? 1 : 64 / G;
-    a.group_bytes = solve_group_bytes<T>(a.n);
-    const unsigned blocks = (unsigned)((a.slots + GPW - 1) / GPW);
-    hipLaunchKernelGGL((solve_kernel<T, Tg, G, R>), dim3(blocks), dim3(G > 64 ? G : 64), a.group_bytes * GPW, s,
+// Above 64 dates: one pixel per two-wave workgroup, rows g + 128 q; the
+// work matrix in LDS, or at a.awork when it does not fit there.
+template <typename T, typename Tg, int R>
+static void launch_solve_generic(SolveArgs a, hipStream_t s) {
+    a.group_bytes = solve_group_bytes<T>(a.n, a.awork == nullptr);
+    hipLaunchKernelGGL((solve_kernel<T, Tg, 128, R, false>), dim3((unsigned)a.slots), dim3(128), a.group_bytes, s,
                        a);
 }
 
-// Static LDS of the solver kernel that runs above 64 dates (its reduction
-// scratch); the launch check adds it to the dynamic LDS of a pixel.
+template <typename T, typename Tg>
+static void dispatch_generic(const SolveArgs& a, hipStream_t s) {
+    if (a.n <= 128) launch_solve_generic<T, Tg, 1>(a, s);
+    else launch_solve_generic<T, Tg, 2>(a, s);
+}
+
+// Static LDS of the solver kernels that run above 64 dates (their reduction
+// scratch); the launch checks add it to the dynamic LDS of a pixel.
 size_t solve_static_lds(int precision) {
     hipFuncAttributes fa{};
-    const void* k = precision == FRINGE_PREC_FP64 ? (const void*)solve_kernel<double, double, 128, 1>
-                                                   : (const void*)solve_kernel<float, double, 128, 1>;
+    const void* k = precision == FRINGE_PREC_FP64 ? (const void*)solve_kernel<double, double, 128, 2, false>
+                                                   : (const void*)solve_kernel<float, double, 128, 2, false>;
     return hipFuncGetAttributes(&fa, k) == hipSuccess ? fa.sharedSizeBytes : 256;
+}
+
+// Whether the work matrix of the solver above 64 dates fits in LDS.
+bool solve_a_in_lds(int n, int precision) {
+    const size_t b = precision == FRINGE_PREC_FP64 ? solve_group_bytes<double>(n) : solve_group_bytes<float>(n);
+    return b + solve_static_lds(precision) <= (size_t)device_lds_limit();
+}
+
+// Per-workgroup global area of the FP64 re-solve above 64 dates: the FP64
+// coherence, then the work matrix.
+size_t resolve_generic_area_bytes(int n) {
+    const int S = n | 1;
+    return align256((size_t)n * n * sizeof(cx<double>)) + align256((size_t)n * S * sizeof(cx<double>));
+}
+
+size_t resolve_generic_lds_bytes(int n, int Nx, int Ny) {
+    return solve_group_bytes<double>(n, false) + 2 * gram_f64_group_bytes(n, Nx, Ny, 16) +
+           solve_static_lds(FRINGE_PREC_FP64);
 }
 
 // n <= 64: register-resident solver, two pixels per wave (n <= 32) or one
@@ -133,6 +157,38 @@ static void launch_resolve(const LinkPlan& P, const uint32_t* wts, SolveArgs a, 
     }
 }
 
+// FP64 re-solve above 64 dates: each workgroup loops over its share of the
+// list, with its coherence and work matrix in its global area.
+static void launch_resolve_generic(const LinkPlan& P, const uint32_t* wts, SolveArgs a, hipStream_t s) {
+    GramArgs g{};
+    g.slc = a.slc;
+    g.wts = wts;
+    g.rows = P.rows;
+    g.cols = P.cols;
+    g.n = P.n;
+    g.Nx = P.Nx;
+    g.Ny = P.Ny;
+    g.nul = P.nul;
+    g.min_nb = P.min_nb;
+    g.npx = P.npx;
+    a.g64 = g;
+    a.list = P.d_cert_list;
+    a.list_count = P.d_cert_count;
+    a.list_cap = P.cert_cap;
+    a.cert_list = nullptr;
+    a.cert_count = nullptr;
+    a.cert_cap = 0;
+    a.awork = P.d_rwork;
+    a.awork_stride = resolve_generic_area_bytes(P.n);
+    a.group_bytes = solve_group_bytes<double>(P.n, false);
+    const size_t lds = a.group_bytes + 2 * gram_f64_group_bytes(P.n, P.Nx, P.Ny, 16);
+    const dim3 grid((unsigned)kResolveGenericBlocks);
+    if (P.n <= 128)
+        hipLaunchKernelGGL((solve_kernel<double, double, 128, 1, true>), grid, dim3(128), lds, s, a);
+    else
+        hipLaunchKernelGGL((solve_kernel<double, double, 128, 2, true>), grid, dim3(128), lds, s, a);
+}
+
 size_t resolve_group_bytes(int n, int Nx, int Ny) {
     return (((size_t)n * n * sizeof(cx<double>) + 255) & ~(size_t)255) + gram_f64_group_bytes(n, Nx, Ny, 32);
 }
@@ -149,7 +205,7 @@ __global__ void zero_i32(int* p, int count) {
 }
 
 bool link_uses_mfma(int n, int precision) { return precision != FRINGE_PREC_FP64 && n <= 64; }
-bool link_certifies(int n, int precision) { return precision != FRINGE_PREC_FP64 && n <= 64; }
+bool link_certifies(int n, int precision) { return precision != FRINGE_PREC_FP64; }
 
 size_t link_gamma_bytes_per_pixel(int n, int precision) {
     const size_t e = link_uses_mfma(n, precision) ? 8 : 16;
@@ -189,7 +245,8 @@ int coherence_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, 
     g.npix = npix;
     if (P.n <= 32) launch_gram_f64<32, 1, 32>(g, s);
     else if (P.n <= 64) launch_gram_f64<64, 1, 32>(g, s);
-    else launch_gram_f64<64, 2, 16>(g, s);
+    else if (P.n <= 128) launch_gram_f64<64, 2, 16>(g, s);
+    else launch_gram_f64<64, 4, 16>(g, s);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -230,7 +287,8 @@ int link_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, float
             g.npix = P.d_npix;
             if (P.n <= 32) launch_gram_f64<32, 1, 32>(g, s);
             else if (P.n <= 64) launch_gram_f64<64, 1, 32>(g, s);
-            else launch_gram_f64<64, 2, 16>(g, s);
+            else if (P.n <= 128) launch_gram_f64<64, 2, 16>(g, s);
+            else launch_gram_f64<64, 4, 16>(g, s);
         } else {
             if (gram_mfma_launch(P, P.d_pack, wts, first, slots, s) != 0) return -1;
         }
@@ -259,8 +317,10 @@ int link_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, float
                 if (fp64) dispatch_reg<double, double>(a, s);
                 else dispatch_reg<float, float>(a, s);
             } else {
-                if (fp64) launch_solve<double, double, 128, 1>(a, s);   // 64 < n <= 128: two waves per pixel
-                else launch_solve<float, double, 128, 1>(a, s);
+                a.awork = P.d_awork;
+                a.awork_stride = P.awork_stride;
+                if (fp64) dispatch_generic<double, double>(a, s);
+                else dispatch_generic<float, double>(a, s);
             }
         }
         if (hipGetLastError() != hipSuccess) return -1;
@@ -278,7 +338,8 @@ int link_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, float
         a.phase = phase;
         a.tcorr = tcorr;
         a.comp = comp;
-        launch_resolve(P, wts, a, s);
+        if (P.n <= 64) launch_resolve(P, wts, a, s);
+        else launch_resolve_generic(P, wts, a, s);
         if (hipGetLastError() != hipSuccess) return -1;
     }
     return 0;

@@ -37,6 +37,12 @@ struct SolveArgs {
     const int* list_count;
     int list_cap;
     GramArgs g64;
+    // Above 64 dates, when a pixel's matrices do not fit a workgroup's LDS:
+    // the solver's n x (n|1) work matrix lives in global memory, awork +
+    // awork_stride * (slot, or workgroup in the re-solve); the re-solve also
+    // keeps its FP64 coherence there, at the start of each workgroup's area.
+    void* awork;
+    size_t awork_stride;
 };
 
 // Certification of an FP32 eigenpair.  The FP32 chain's backward error is
@@ -81,20 +87,27 @@ template <typename T> struct SolveLds {
 
 __host__ __device__ inline size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
 
-template <typename T> __host__ __device__ inline size_t solve_group_bytes(int n) {
+// LDS bytes of one pixel's group; a_lds = false leaves the work matrix A
+// out (it is then in global memory).
+template <typename T> __host__ __device__ inline size_t solve_group_bytes(int n, bool a_lds = true) {
     const int S = n | 1;
-    size_t b = align16((size_t)n * S * sizeof(cx<T>));
+    size_t b = a_lds ? align16((size_t)n * S * sizeof(cx<T>)) : 0;
     b += 4 * align16((size_t)n * sizeof(cx<T>));
     b += 8 * align16((size_t)n * sizeof(T));
     b += align16((size_t)n * sizeof(cx<float>));
     return b;
 }
 
-template <typename T> __device__ inline SolveLds<T> solve_carve(char* base, int n) {
+template <typename T> __device__ inline SolveLds<T> solve_carve(char* base, int n, cx<T>* aglob = nullptr) {
     SolveLds<T> L;
     const int S = n | 1;
     char* p = base;
-    L.A = (cx<T>*)p;   p += align16((size_t)n * S * sizeof(cx<T>));
+    if (aglob) {
+        L.A = aglob;
+    } else {
+        L.A = (cx<T>*)p;
+        p += align16((size_t)n * S * sizeof(cx<T>));
+    }
     L.V = (cx<T>*)p;   p += align16((size_t)n * sizeof(cx<T>));
     L.W = (cx<T>*)p;   p += align16((size_t)n * sizeof(cx<T>));
     L.TAU = (cx<T>*)p; p += align16((size_t)n * sizeof(cx<T>));
@@ -167,12 +180,14 @@ __device__ void load_abs_gamma(double* M, int MS, const cx<Tg>* gam, int n, bool
 // j-th Cholesky diagonal, so the matrix is declared not positive definite
 // exactly when a Cholesky factorisation would be (a pivot <= 0 or NaN).
 template <int G, int R>
-__device__ bool sweep_inverse(double* M, int MS, int n) {
+__device__ bool sweep_inverse(double* M, int MS, int n, double& pmin) {
     const int g = threadIdx.x % G;
     bool ok = true;
+    pmin = 1.0;
     for (int j = 0; j < n; ++j) {
         const double p = M[j * MS + j];
         ok = ok && (p > 0.0);
+        pmin = fmin(pmin, p);
         const double piv = 1.0 / p;
         __syncthreads();
 #pragma unroll
@@ -329,7 +344,7 @@ __device__ inline int sturm_count(const T* D, const T* E2, int n, T sigma, T piv
 // k-th smallest eigenvalue (0-based) by (G+1)-section: lane g tests the
 // g-th interior point of the current interval.  tnorm receives max|D|+|E|.
 template <typename T, int G>
-__device__ T select_eigenvalue(SolveLds<T>& L, int n, int k, T& tnorm) {
+__device__ T select_eigenvalue(SolveLds<T>& L, int n, int k, T& tnorm, T& pm) {
     const int g = threadIdx.x % G;
     T lo = T(3.0e38), hi = T(-3.0e38), em2 = T(0), tn = T(0);
     for (int i = g; i < n; i += G) {
@@ -350,6 +365,7 @@ __device__ T select_eigenvalue(SolveLds<T>& L, int n, int k, T& tnorm) {
     __syncthreads();
     tnorm = tn;
     const T pivmin = Lim<T>::tiny() * max(T(1), em2);
+    pm = pivmin;
     const T pad = T(4) * Lim<T>::eps() * tn + T(2) * pivmin;
     lo -= pad;
     hi += pad;
@@ -430,10 +446,9 @@ __device__ void tridiag_inverse_iteration(SolveLds<T>& L, int n, T lam, T tnorm)
 // Eigenvector of A for eigenvalue index k (0-based ascending) after
 // tridiagonalize(): returns the eigenvalue, y holds the lane's rows.
 template <typename T, int G, int R>
-__device__ T extreme_eigenpair(SolveLds<T>& L, int S, int n, int k, cx<T> (&y)[R]) {
+__device__ T extreme_eigenpair(SolveLds<T>& L, int S, int n, int k, cx<T> (&y)[R], T& tnorm, T& pivmin) {
     const int g = threadIdx.x % G;
-    T tnorm;
-    const T lam = select_eigenvalue<T, G>(L, n, k, tnorm);
+    const T lam = select_eigenvalue<T, G>(L, n, k, tnorm, pivmin);
     if (g == 0) tridiag_inverse_iteration<T>(L, n, lam, tnorm);
     __syncthreads();
 #pragma unroll
@@ -473,56 +488,92 @@ template <typename T> __device__ inline cx<T> unit_or(cx<T> u, cx<T> fallback, b
     return ok ? cscale(u, T(1) / a) : fallback;
 }
 
-// ---------------------------------------------------------------------------
-template <typename T, typename Tg, int G, int R>
-__global__ __launch_bounds__(G > 64 ? G : 64) void solve_kernel(SolveArgs a) {
-    constexpr int GPW = G >= 64 ? 1 : 64 / G;
-    const int grp = threadIdx.x / G, g = threadIdx.x % G;
-    const int slot = blockIdx.x * GPW + grp;
+// Certification of the eigenpair (criteria at kCertScale): true when the
+// unit eigenvector y (lane rows) of eigenvalue lam of tridiag(D, E), norm
+// bound tn, has another eigenvalue within kCertScale eps |T| / min|v_t|, or
+// a zero or non-finite component.  Every lane of the group takes part.
+template <typename T, int G, int R>
+__device__ bool eig_uncertain_g(const SolveLds<T>& L, int n, T lam, T tn, T pivmin, bool largest,
+                                const cx<T> (&y)[R]) {
+    const int g = threadIdx.x % G;
+    T s2 = T(0), m2 = T(3.0e38);
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int r = g + G * q;
+        if (r < n) {
+            const T a2 = cabs2(y[q]);
+            s2 += a2;
+            m2 = min(m2, a2);
+        }
+    }
+    s2 = gsum<G>(s2);
+    m2 = gmin<G>(m2);
+    const T vmin = sqrt(m2 / s2);
+    const T delta = T(kCertScale) * Lim<T>::eps() * tn / vmin;
+    if (!(delta <= tn)) return true;
+    const int c = sturm_count(L.D, L.E2, n, largest ? lam - delta : lam + delta, pivmin);
+    return largest ? (c <= n - 2) : (c >= 2);
+}
+
+// One pixel's estimator chain on the G lanes of its group (n > 64 in this
+// build): gam is its coherence, np its neighbour count (-1: not linked).
+// CERT: the FP32 result is checked and, when it is not certified, the pixel
+// is appended to a.cert_list for the FP64 re-solve.
+template <typename T, typename Tg, int G, int R, bool CERT>
+__device__ __forceinline__ void solve_generic_pixel(const SolveArgs& a, SolveLds<T>& L, bool inslot, int np,
+                                                    const cx<Tg>* gam, long long pix) {
+    const int g = threadIdx.x % G;
     const int n = a.n, S = n | 1;
     const int MS = S * (int)(sizeof(cx<T>) / sizeof(double));
-    extern __shared__ __align__(16) char smem[];
-    SolveLds<T> L = solve_carve<T>(smem + grp * a.group_bytes, n);
     double* M = (double*)L.A;
-
-    const bool inslot = slot < a.slots;
-    const int np = inslot ? a.npix[slot] : -1;
     const bool act = np >= 0;
-    const cx<Tg>* gam = (const cx<Tg>*)a.gamma + (size_t)(inslot ? slot : 0) * n * n;
-    const long long pix = a.first + (inslot ? slot : 0);
 
     float code = 0.0f;
     bool failed = false;
     cx<T> y[R];
+    T tn, pm;
+    bool unc = false;   // CERT: the FP32 result is not certified
 
     if (a.module == FRINGE_MODULE_EVD && a.method == FRINGE_METHOD_MLE) {
         // evd.cpp:584-688: lambda_min(Gamma) (-1/-2), lambda_min(|Gamma|) (-3/-4),
         // inv(|Gamma|) (-5), smallest eigenpair of inv(|Gamma|) o Gamma (-6/-7).
         const bool fin = load_gamma<T, Tg, G, R>(L.A, S, gam, n, act, -1);
         tridiagonalize<T, G, R>(L, S, n);
-        T tn;
-        const T lmin = select_eigenvalue<T, G>(L, n, 0, tn);
+        const T lmin = select_eigenvalue<T, G>(L, n, 0, tn, pm);
         if (!fin) { code = -1.0f; failed = true; }
         else if (!(lmin >= T(1.0e-6))) { code = -2.0f; failed = true; }
+        if constexpr (CERT) unc = !(lmin >= T(1.0e-6) + T(kCertThrMargin) * Lim<T>::eps() * tn);
+        double pmin;
         load_abs_gamma<Tg, G, R>(M, MS, gam, n, act, 1.0e-6);
-        const bool pd_shift = sweep_inverse<G, R>(M, MS, n);
+        const bool pd_shift = sweep_inverse<G, R>(M, MS, n, pmin);
         if (!failed && !pd_shift) { code = -4.0f; failed = true; }
+        if constexpr (CERT) unc = unc || !(pmin >= kCertPivotMin);
         load_abs_gamma<Tg, G, R>(M, MS, gam, n, act, 0.0);
-        const bool pd = sweep_inverse<G, R>(M, MS, n);
+        const bool pd = sweep_inverse<G, R>(M, MS, n, pmin);
         if (!failed && !pd) { code = -5.0f; failed = true; }
+        if constexpr (CERT) unc = unc || !(pmin >= kCertPivotMin);
         const bool hfin = form_hadamard<T, Tg, G, R>(L.A, S, M, MS, gam, n, act);
         tridiagonalize<T, G, R>(L, S, n);
-        const T lam = extreme_eigenpair<T, G, R>(L, S, n, 0, y);
+        const T lam = extreme_eigenpair<T, G, R>(L, S, n, 0, y, tn, pm);
         if (!failed && !hfin) { code = -6.0f; failed = true; }
         if (!failed && !(lam >= T(1.0e-6))) { code = -7.0f; failed = true; }
+        if constexpr (CERT) {
+            unc = unc || !(lam >= T(1.0e-6) + T(kCertThrMargin) * Lim<T>::eps() * tn);
+            const bool ue = eig_uncertain_g<T, G, R>(L, n, lam, tn, pm, false, y);
+            unc = fin && (unc || (!failed && ue));
+        }
     } else if (a.module == FRINGE_MODULE_EVD) {
         // EVD / STBAS (evd.cpp:689-732)
         const int band = (a.method == FRINGE_METHOD_STBAS) ? a.bw : -1;
         const bool fin = load_gamma<T, Tg, G, R>(L.A, S, gam, n, act, band);
         tridiagonalize<T, G, R>(L, S, n);
-        const T lam = extreme_eigenpair<T, G, R>(L, S, n, n - 1, y);
+        const T lam = extreme_eigenpair<T, G, R>(L, S, n, n - 1, y, tn, pm);
         if (!fin) { code = -6.0f; failed = true; }
         else if (!(lam >= T(1.0e-6))) { code = -7.0f; failed = true; }
+        if constexpr (CERT) {
+            const bool ue = eig_uncertain_g<T, G, R>(L, n, lam, tn, pm, true, y);
+            unc = fin && (ue || !(lam >= T(1.0e-6) + T(kCertThrMargin) * Lim<T>::eps() * tn));
+        }
     } else {
         // phase_link EMI (phase_link.cpp:540-618)
         bool fin = true;
@@ -537,23 +588,34 @@ __global__ __launch_bounds__(G > 64 ? G : 64) void solve_kernel(SolveArgs a) {
             }
             fin = group_all<G>(fin);
         }
-        const bool pd = sweep_inverse<G, R>(M, MS, n);
+        double pmin;
+        const bool pd = sweep_inverse<G, R>(M, MS, n, pmin);
         const bool hfin = form_hadamard<T, Tg, G, R>(L.A, S, M, MS, gam, n, act);
+        // run_evd is uniform over the group: the barriers below are block-wide
         bool run_evd = !pd || !hfin;
         tridiagonalize<T, G, R>(L, S, n);
-        extreme_eigenpair<T, G, R>(L, S, n, 0, y);
+        const T lam = extreme_eigenpair<T, G, R>(L, S, n, 0, y, tn, pm);
+        if constexpr (CERT) {
+            const bool ue = eig_uncertain_g<T, G, R>(L, n, lam, tn, pm, false, y);
+            unc = !(pmin >= kCertPivotMin) || (!run_evd && ue);
+        }
         if (__any(act && fin && run_evd)) {
             // EVD fallback: largest eigenvector of Gamma
             load_gamma<T, Tg, G, R>(L.A, S, gam, n, act, -1);
             tridiagonalize<T, G, R>(L, S, n);
             cx<T> ye[R];
-            extreme_eigenpair<T, G, R>(L, S, n, n - 1, ye);
+            const T le = extreme_eigenpair<T, G, R>(L, S, n, n - 1, ye, tn, pm);
+            if constexpr (CERT) {
+                const bool ue = eig_uncertain_g<T, G, R>(L, n, le, tn, pm, true, ye);
+                if (run_evd) unc = unc || ue;
+            }
             if (run_evd) {
 #pragma unroll
                 for (int q = 0; q < R; ++q) y[q] = ye[q];
             }
         }
         if (!fin) { code = -1.0f; failed = true; }
+        if constexpr (CERT) unc = unc && fin;
     }
 
     // ---- referenced phases (evd.cpp:738-749) ------------------------------
@@ -643,6 +705,55 @@ __global__ __launch_bounds__(G > 64 ? G : 64) void solve_kernel(SolveArgs a) {
         }
         a.tcorr[pix] = tco;
         a.comp[pix] = co;
+        if constexpr (CERT) {
+            if (act && unc) {
+                const int e = atomicAdd(a.cert_count, 1);
+                if (e < a.cert_cap) a.cert_list[e] = (int)pix;
+            }
+        }
+    }
+}
+
+// Above 64 dates.  LIST = false: one pixel per group, slot s of the round's
+// coherence buffer is pixel first + s; the FP32 instance certifies its
+// results.  LIST = true (T = Tg = double, G = 128): the FP64 re-solve of the
+// listed pixels; both waves of a workgroup compute the listed pixel's FP64
+// coherence (the same values) into the workgroup's global area, then solve
+// it, looping over the list.
+template <typename T, typename Tg, int G, int R, bool LIST>
+__global__ __launch_bounds__(G > 64 ? G : 64) void solve_kernel(SolveArgs a) {
+    constexpr int GPW = G >= 64 ? 1 : 64 / G;
+    const int grp = threadIdx.x / G;
+    const int n = a.n, S = n | 1;
+    extern __shared__ __align__(16) char smem[];
+    if constexpr (!LIST) {
+        const int slot = blockIdx.x * GPW + grp;
+        const bool inslot = slot < a.slots;
+        if (GPW == 1 && !inslot) return;   // the whole workgroup is past the last slot
+        cx<T>* aglob = a.awork ? (cx<T>*)((char*)a.awork + a.awork_stride * (size_t)(inslot ? slot : 0)) : nullptr;
+        SolveLds<T> L = solve_carve<T>(smem + grp * a.group_bytes, n, aglob);
+        const int np = inslot ? a.npix[slot] : -1;
+        const cx<Tg>* gam = (const cx<Tg>*)a.gamma + (size_t)(inslot ? slot : 0) * n * n;
+        const long long pix = a.first + (inslot ? slot : 0);
+        solve_generic_pixel<T, Tg, G, R, sizeof(T) == 4>(a, L, inslot, np, gam, pix);
+    } else {
+        static_assert(sizeof(T) == 8 && sizeof(Tg) == 8 && G == 128, "the re-solve runs in FP64 on two waves");
+        constexpr int RG = R * 2;   // rows per lane of the 64-lane FP64 Gram
+        char* area = (char*)a.awork + a.awork_stride * (size_t)blockIdx.x;
+        cx<double>* gam = (cx<double>*)area;
+        cx<double>* aglob = (cx<double>*)(area + (((size_t)n * n * sizeof(cx<double>) + 255) & ~(size_t)255));
+        SolveLds<T> L = solve_carve<T>(smem, n, aglob);
+        const size_t gbytes = gram_f64_group_bytes(n, a.g64.Nx, a.g64.Ny, 16);
+        char* gscratch = smem + a.group_bytes + (threadIdx.x / 64) * gbytes;
+        const int cnt = min(*a.list_count, a.list_cap);
+        for (int e = blockIdx.x; e < cnt; e += gridDim.x) {
+            const long long pix = a.list[e];
+            const int np = gram_f64_pixel<64, RG, 16>(a.g64, true, pix, gam, gscratch);
+            __threadfence();
+            __syncthreads();
+            solve_generic_pixel<T, Tg, G, R, false>(a, L, true, np, gam, pix);
+            __syncthreads();
+        }
     }
 }
 

@@ -119,32 +119,27 @@ def test_cfg3_band_gram_and_evd(precision):
 
 
 def test_solver_limits_above_64_dates():
-    """The LDS-resident solver above 64 dates: an FP64 case whose pixel
-    matrices do not fit a workgroup's LDS is refused before any launch (code
-    202, no workspace), and 128 dates in mixed precision run and match the
-    oracle."""
+    """Above 64 dates: a date count beyond the solver's limit (256) is refused
+    before any launch (code 202, no workspace); FP64 cases whose work matrix
+    does not fit a workgroup's LDS (100 and 160 dates) run with it in global
+    memory, and 128 dates in mixed precision run; all match the oracle."""
     L = _lib.lib()
-    p = tile.link_params(16, 20, 100, 7, 5, precision="fp64")
+    p = tile.link_params(16, 20, 257, 7, 5, precision="fp64")
     assert L.fringe_link_workspace_bytes(ctypes.byref(p)) == 0
     dummy = ctypes.c_void_p(256)
     rc = L.fringe_link_run(ctypes.byref(p), dummy, dummy, dummy, dummy, dummy, dummy, 1 << 20, None)
     assert rc == 202
-    n, rows, cols, Nx, Ny = 128, 14, 16, 7, 5
-    print(f"case mixed n={n}", flush=True)
-    slc = synthetic.stack(n, rows, cols, seed=1128)
-    wts = synthetic.full_mask(rows, cols, Nx, Ny)
-    r = oracle.link(slc, wts, Nx, Ny, module="phase_link", method="MLE", want_gap=True)
-    ph, tc, co = tile.link(to_dev(slc, torch), wts_dev(wts, torch), Nx, Ny, module="phase_link", method="MLE",
-                           precision="mixed")
-    torch.cuda.synchronize()
-    gap = r["gap"].copy()
-    # only windows holding at least n samples give a full-rank Gamma
-    rr, cc = np.arange(rows)[:, None], np.arange(cols)[None, :]
-    k = ((np.minimum(rr + Ny, rows - 1) - np.maximum(rr - Ny, 0) + 1)
-         * (np.minimum(cc + Nx, cols - 1) - np.maximum(cc - Nx, 0) + 1))
-    gap[k < n] = -1.0
-    st = compare_link((ph.cpu().numpy(), tc.cpu().numpy(), co.cpu().numpy()), r, gap, vmin=r["vmin"])
-    print(st)
-    assert st["untouched_equal"]
-    assert st["n_well"] >= 16
-    assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL
+    for n, rows, cols, Nx, Ny, precision in ((100, 14, 16, 7, 5, "fp64"), (160, 20, 22, 9, 6, "fp64"),
+                                             (128, 14, 16, 7, 5, "mixed")):
+        print(f"case {precision} n={n}", flush=True)
+        slc = synthetic.stack(n, rows, cols, seed=1000 + n)
+        wts = synthetic.full_mask(rows, cols, Nx, Ny)
+        r = oracle.link(slc, wts, Nx, Ny, module="phase_link", method="MLE", want_gap=True)
+        ph, tc, co = tile.link(to_dev(slc, torch), wts_dev(wts, torch), Nx, Ny, module="phase_link", method="MLE",
+                               precision=precision)
+        torch.cuda.synchronize()
+        st = compare_link((ph.cpu().numpy(), tc.cpu().numpy(), co.cpu().numpy()), r, r["gap"], stack=slc)
+        print(st)
+        assert st["untouched_equal"] and st["codes_equal"]
+        assert st["n_well"] >= 16
+        assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL and st["comp_err"] <= PHASE_TOL

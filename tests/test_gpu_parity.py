@@ -107,22 +107,6 @@ def test_nmap_row_range(n, method):
         assert not wg[outside].any() and not cg[outside].any()
 
 
-def _full_rank_gap(gap, rows, cols, Nx, Ny, n, precision):
-    """Above 64 dates, for the float-solver precision: gap with pixels whose
-    (full-mask) window holds fewer than n samples marked ill-posed.  Their
-    sample coherence is rank deficient: |Gamma| is near singular, so MLE / EMI
-    turn FP32 rounding into phase errors of any size, and the EVD eigen-gaps
-    shrink to where n * 1e-6 / gap exceeds 1e-3 rad.  FP64 keeps every pixel
-    the eigen-gap admits, and so does the mixed path up to 64 dates, which
-    re-solves every uncertified pixel in FP64."""
-    if precision == "fp64" or n <= 64:
-        return gap
-    r, c = np.arange(rows)[:, None], np.arange(cols)[None, :]
-    k = ((np.minimum(r + Ny, rows - 1) - np.maximum(r - Ny, 0) + 1)
-         * (np.minimum(c + Nx, cols - 1) - np.maximum(c - Nx, 0) + 1))
-    return np.where(k >= n, gap, -1.0)
-
-
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("module,method", [("phase_link", "MLE"), ("evd", "EVD"), ("evd", "MLE")])
 @pytest.mark.parametrize("n", [33, 48, 64])
@@ -182,10 +166,10 @@ def test_link_generic_solver_n72(precision):
     slc = synthetic.stack(n, rows, cols, seed=372)
     wts = synthetic.full_mask(rows, cols, Nx, Ny)
     g, r = _run_both(slc, wts, Nx, Ny, "phase_link", "MLE", precision)
-    st = compare_link(g, r, _full_rank_gap(r["gap"], rows, cols, Nx, Ny, n, precision))
+    st = compare_link(g, r, r["gap"], stack=slc)
     print(st)
-    assert st["untouched_equal"]
-    assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL
+    assert st["untouched_equal"] and st["codes_equal"]
+    assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL and st["comp_err"] <= PHASE_TOL
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
@@ -200,19 +184,17 @@ def test_link_generic_solver_modes(n, module, method, precision):
     slc = synthetic.stack(n, rows, cols, seed=400 + n)
     wts = synthetic.full_mask(rows, cols, Nx, Ny)
     g, r = _run_both(slc, wts, Nx, Ny, module, method, precision)
-    st = compare_link(g, r, _full_rank_gap(r["gap"], rows, cols, Nx, Ny, n, precision),
-                      vmin=None if precision == "fp64" else r["vmin"])
+    st = compare_link(g, r, r["gap"], stack=slc)
     print(st)
-    assert st["untouched_equal"]
-    if precision == "fp64":
-        assert st["codes_equal"]
+    assert st["untouched_equal"] and st["codes_equal"]
     assert st["n_well"] >= (1 if (module, method) == ("evd", "MLE") else 36)
-    assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL
+    assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL and st["comp_err"] <= PHASE_TOL
 
 
 @pytest.mark.parametrize("n,Nx,Ny,module,method", [(10, 29, 9, "evd", "MLE"), (30, 7, 5, "phase_link", "MLE"),
                                                   (64, 5, 4, "evd", "MLE"), (64, 5, 4, "evd", "EVD"),
-                                                  (48, 5, 4, "phase_link", "MLE")])
+                                                  (48, 5, 4, "phase_link", "MLE"), (96, 7, 5, "phase_link", "MLE"),
+                                                  (80, 6, 5, "evd", "MLE")])
 def test_mixed_resolve_equals_fp64(n, Nx, Ny, module, method):
     """Pixels whose FP32 eigenpair the mixed path cannot certify are
     recomputed by the FP64 path (FP64 Gram and solver) in the same call:

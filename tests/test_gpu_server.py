@@ -193,19 +193,8 @@ def test_large_n_requests_against_the_oracle():
             assert np.array_equal(res[3].numpy().view(np.uint32), w_ref)
             wts = w_ref
         ref = oracle.link(slc, wts, Nx, Ny, module="phase_link", method="MLE", min_neighbors=5, want_gap=True)
-        gap, vmin = ref["gap"].copy(), None
-        if n > 64:
-            # the FP32 solver above 64 dates: windows with fewer samples than
-            # dates give a rank-deficient Gamma
-            rr, cc = np.arange(rows)[:, None], np.arange(cols)[None, :]
-            k = ((np.minimum(rr + Ny, rows - 1) - np.maximum(rr - Ny, 0) + 1)
-                 * (np.minimum(cc + Nx, cols - 1) - np.maximum(cc - Nx, 0) + 1))
-            gap[k < n] = -1.0
-            vmin = ref["vmin"]
-        st = compare_link((res[0].numpy(), res[1].numpy(), res[2].numpy()), ref, gap, vmin=vmin, stack=slc)
+        st = compare_link((res[0].numpy(), res[1].numpy(), res[2].numpy()), ref, ref["gap"], stack=slc)
         print(f"  n={n}: {st}")
-        assert st["untouched_equal"]
+        assert st["untouched_equal"] and st["codes_equal"]
         assert st["n_well"] >= 0.25 * rows * cols
-        assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL
-        if n <= 64:
-            assert st["codes_equal"] and st["comp_err"] <= PHASE_TOL
+        assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL and st["comp_err"] <= PHASE_TOL

@@ -78,85 +78,114 @@ def copy_probe(mib=1024, reps=5):
     print(json.dumps({"copy_probe_bytes": x.numel() * 4, "reps": reps}))
 
 
-def serve(args):
-    """Serving workload: a Poisson stream (mean `rate` requests/s) of EMI
-    tiles with N in {8, 16, 30, 48, 96}, sides 256..2048, half-windows 5x5 to
-    10x5 and random SHP fill 20-100 %, run through the tile server in mixed
-    precision under a device-memory budget; then the same requests one after
-    another on the plain FP64 path."""
+def serve_load(dev, Ns, sides, n_req, rel_rates, seed, budget, n96_side=None, log=None):
+    """Serving under load: a pool of n_req EMI requests (N from Ns, sides from
+    sides, half-windows 5..10 x 5, random SHP fill 20-100 %) through the tile
+    server in mixed precision.  First every request at once (capacity, in
+    requests/s), then Poisson arrivals at each fraction of that capacity in
+    rel_rates; per point, p50/p99 of submit-to-done latency and the achieved
+    pixels/s."""
     import torch
     from fringe_amd import synthetic, tile
     from fringe_amd.server import TileServer
-    dev = torch.device("cuda:0")
-    rng = np.random.default_rng(105)
-    Ns, sides = [8, 16, 30, 48, 96], [256, 512, 1024, 2048]
+    rng = np.random.default_rng(seed)
     reqs, stacks = [], {}
-    for i in range(args.serve_requests):
-        n, side = int(rng.choice(Ns)), int(rng.choice(sides))
+    for i in range(n_req):
+        n = int(rng.choice(Ns))
+        side = int(n96_side) if (n > 64 and n96_side) else int(rng.choice(sides))
         Nx, fill = int(rng.integers(5, 11)), float(rng.uniform(0.2, 1.0))
         if (n, side) not in stacks:
             stacks[(n, side)] = synthetic.stack_torch(n, side, side, 1000 + len(stacks), device=dev)
         wts = synthetic.random_mask_torch(side, side, Nx, 5, fill, 2000 + i, device=dev)
         reqs.append((n, side, Nx, fill, wts))
-    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=len(reqs)))
     torch.cuda.synchronize()
-    free, total = torch.cuda.mem_get_info()
-    budget = int(min(args.budget_gb * 1e9, free - 16e9))
 
-    def params(n, side, Nx, prec):
+    def params(n, side, Nx):
         return tile.link_params(side, side, n, Nx, 5, module="phase_link", method="MLE", min_neighbors=5,
-                                precision=prec)
+                                precision="mixed")
 
-    with TileServer(budget, streams=4) as srv:
-        # warm-up: one small request per N (kernel loading)
-        for n in Ns:
-            key = next(k for k in stacks if k[0] == n) if any(k[0] == n for k in stacks) else None
-            if key is None:
-                continue
-            w = synthetic.random_mask_torch(key[1], key[1], 5, 5, 0.5, 7, device=dev)
-            srv.wait(srv.submit(stacks[key], params(n, key[1], 5, "mixed"), wts=w))
-        t0 = time.perf_counter()
-        ids = []
-        for (n, side, Nx, fill, wts), at in zip(reqs, arrivals):
-            now = time.perf_counter() - t0
-            if at > now:
-                time.sleep(at - now)
-            ids.append(srv.submit(stacks[(n, side)], params(n, side, Nx, "mixed"), wts=wts))
-        lat, linked = [], 0
-        for i, rid in enumerate(ids):
-            assert srv.wait(rid) == 0
-            lat.append(srv.times(rid)[1])
-            print(f"request {i} done: N={reqs[i][0]} side={reqs[i][1]} {lat[-1]:.1f} ms", file=sys.stderr, flush=True)
-        t_end = time.perf_counter() - t0
-        for rid in ids:
-            tc = srv.fetch(rid)[1]
-            linked += int((tc != 0).sum())
-            srv.release(rid)
-    # plain FP64, serial, same requests
-    t1 = time.perf_counter()
-    linked64 = 0
-    for i, (n, side, Nx, fill, wts) in enumerate(reqs):
+    def run(arrivals):
+        with TileServer(budget, streams=4) as srv:
+            t0 = time.perf_counter()
+            ids = []
+            for (n, side, Nx, fill, wts), at in zip(reqs, arrivals):
+                now = time.perf_counter() - t0
+                if at > now:
+                    time.sleep(at - now)
+                ids.append(srv.submit(stacks[(n, side)], params(n, side, Nx), wts=wts))
+            lat = []
+            for rid in ids:
+                assert srv.wait(rid) == 0
+                lat.append(srv.times(rid)[1])
+            span = time.perf_counter() - t0
+            linked = 0
+            for rid in ids:
+                linked += int((srv.fetch(rid)[1] != 0).sum())
+                srv.release(rid)
+        return np.array(lat), span, linked
+
+    run(np.zeros(len(reqs)))   # warm-up: kernel loading, allocator
+    lat, span, linked = run(np.zeros(len(reqs)))
+    cap = len(reqs) / span
+    out = {"requests": len(reqs), "capacity_req_per_s": cap, "saturated_pixels_per_s": linked / span,
+           "saturated_p50_ms": float(np.percentile(lat, 50)), "saturated_p99_ms": float(np.percentile(lat, 99)),
+           "sweep": []}
+    for rel in rel_rates:
+        arr = np.cumsum(np.random.default_rng(seed + int(rel * 1000)).exponential(1.0 / (rel * cap), size=len(reqs)))
+        lat, span, linked = run(arr)
+        pt = {"offered_frac_of_capacity": rel, "offered_req_per_s": rel * cap, "p50_ms": float(np.percentile(lat, 50)),
+              "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()), "pixels_per_s": linked / span}
+        out["sweep"].append(pt)
+        if log:
+            print(f"serving at {rel:.2f} x capacity: p50 {pt['p50_ms']:.1f} ms p99 {pt['p99_ms']:.1f} ms",
+                  file=log, flush=True)
+    out["mix"] = [[n, s, nx, round(f, 2)] for n, s, nx, f, _ in reqs]
+    return out
+
+
+def serve(args):
+    """Serving workload (cfg5): EMI requests with N in {8, 16, 30, 48, 96},
+    sides 256..2048 (N = 96: 512), half-windows 5x5 to 10x5 and random SHP
+    fill 20-100 %, through the tile server in mixed precision under a
+    device-memory budget: capacity, then Poisson arrivals at 0.5, 0.8 and 1.2
+    times it (p50/p99 latency); then the same requests one after another on
+    the plain FP64 path."""
+    import torch
+    from fringe_amd import synthetic, tile
+    dev = torch.device("cuda:0")
+    free, _ = torch.cuda.mem_get_info()
+    budget = int(min(args.budget_gb * 1e9, free - 16e9))
+    load = serve_load(dev, [8, 16, 30, 48, 96], [256, 512, 1024, 2048], args.serve_requests, [0.5, 0.8, 1.2], 105,
+                      budget, n96_side=512, log=sys.stderr)
+    # plain FP64, serial, same requests (the mix is rebuilt from the same seed)
+    rng = np.random.default_rng(105)
+    stacks = {}
+    t64, linked64 = 0.0, 0
+    for i, (n, side, Nx, fill) in enumerate(load["mix"]):
+        if (n, side) not in stacks:
+            stacks[(n, side)] = synthetic.stack_torch(n, side, side, 1000 + len(stacks), device=dev)
+        wts = synthetic.random_mask_torch(side, side, Nx, 5, fill, 2000 + i, device=dev)
+        torch.cuda.synchronize()
         t2 = time.perf_counter()
-        r = tile.LinkRunner(params(n, side, Nx, "fp64"), device=dev)
+        r = tile.LinkRunner(tile.link_params(side, side, n, Nx, 5, module="phase_link", method="MLE",
+                                             min_neighbors=5, precision="fp64"), device=dev)
         r.run(stacks[(n, side)], wts)
         torch.cuda.synchronize()
+        t64 += time.perf_counter() - t2
         linked64 += int((r.tcorr != 0).sum())
         del r
-        print(f"fp64 request {i}: N={n} side={side} {1e3 * (time.perf_counter() - t2):.1f} ms", file=sys.stderr,
-              flush=True)
-    t64 = time.perf_counter() - t1
-    lat = np.array(lat)
+        print(f"fp64 request {i}: N={n} side={side}", file=sys.stderr, flush=True)
+    below = load["sweep"][0]
     out = {
         "metric": "serving: p50/p99 tile latency and aggregate pixels/s (EMI, mixed) vs serial plain FP64",
-        "value": linked / t_end, "unit": "pixels/s", "n_gpus": 1, "higher_is_better": True,
-        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-        "max_ms": float(lat.max()), "requests": len(reqs), "offered_rate_per_s": args.rate,
-        "span_s": t_end, "serial_fp64_pixels_per_s": linked64 / t64, "serial_fp64_s": t64,
-        "speedup_vs_serial_fp64": (linked / t_end) / (linked64 / t64), "budget_bytes": budget,
+        "value": load["saturated_pixels_per_s"], "unit": "pixels/s", "n_gpus": 1, "higher_is_better": True,
+        "p50_ms": below["p50_ms"], "p99_ms": below["p99_ms"], "offered_frac_of_capacity": below["offered_frac_of_capacity"],
+        "serial_fp64_pixels_per_s": linked64 / t64, "serial_fp64_s": t64,
+        "speedup_vs_serial_fp64": load["saturated_pixels_per_s"] / (linked64 / t64), "budget_bytes": budget,
+        "serving": load,
         "data": "synthetic (seeded; random SHP masks of the drawn fill)",
-        "config": {"workload": "Poisson arrivals, N in {8,16,30,48,96}, sides {256,512,1024,2048}, "
-                               "Nx in 5..10, Ny=5, fill U[0.2,1], phase_link EMI minNeighbors=5",
-                   "mix": [[n, s, nx, round(f, 2)] for n, s, nx, f, _ in reqs]},
+        "config": {"workload": "N in {8,16,30,48,96}, sides {256,512,1024,2048} (N=96: 512), Nx in 5..10, Ny=5, "
+                               "fill U[0.2,1], phase_link EMI minNeighbors=5, 4 server streams"},
     }
     print(json.dumps(out))
 
@@ -514,6 +543,9 @@ def parse():
     ap.add_argument("--cfg4", action="store_true",
                     help="sequential mini-stacks of uneven length (120 dates, 1024^2), on device-resident data")
     ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-under-load sample of the headline run")
+    ap.add_argument("--load-requests", type=int, default=40,
+                    help="requests per point of the headline run's serving-under-load sample")
     ap.add_argument("--rate", type=float, default=40.0, help="mean arrivals per second")
     ap.add_argument("--budget-gb", type=float, default=200.0)
     return ap.parse_args()
@@ -663,6 +695,16 @@ def main():
         except Exception as e:  # the baseline is reported, never the product
             cpu = {"error": str(e)}
 
+    # ---- serving under load: p50/p99 below capacity and a short sweep -----
+    load = None
+    if not args.no_serve:
+        free, _ = torch.cuda.mem_get_info()
+        load = serve_load(dev, [8, 16, 30, 48, 96], [256, 512], args.load_requests, [0.5, 0.8, 1.2], 105,
+                          int(min(32e9, free - 8e9)), n96_side=256, log=sys.stderr)
+        load["config"] = ("tile server, 4 streams, mixed-precision EMI requests: N in {8,16,30,48,96}, sides "
+                          "256/512 (N=96: 256), Nx 5..10, Ny 5, fill U[0.2,1]; Poisson arrivals at fractions of "
+                          "the measured capacity")
+
     out = {
         "metric": "pixels/s phase-linked (EMI, N=30) mixed vs FP64 on GPU; p99 tile latency",
         "value": value, "unit": "pixels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -675,6 +717,9 @@ def main():
                    "pixels_linked": linked, "shp_fill_mean": fill},
         "fp64_value": value64, "fp64_ms_per_step": ms64, "speedup_vs_fp64": value / value64,
         "p50_tile_ms": float(np.percentile(lat, 50)), "p99_tile_ms": float(np.percentile(lat, 99)),
+        "p50_tile_ms_under_load": load["sweep"][0]["p50_ms"] if load else None,
+        "p99_tile_ms_under_load": load["sweep"][0]["p99_ms"] if load else None,
+        "serving_under_load": load,
         "resolved_fp64_pixels": n_resolved, "resolved_fp64_frac": n_resolved / max(linked, 1),
         "accuracy_vs_fp64": acc, "accuracy_vs_oracle": acc_oracle,
         "stages_ms_per_step": {k: v[0] for k, v in per.items()},

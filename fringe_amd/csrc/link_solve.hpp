@@ -54,9 +54,10 @@ struct SolveArgs {
 // estimate with C = 5 stays below 5e-4 rad, half the phase tolerance: no
 // eigenvalue of T may lie within 1e4 eps32 |T| / min|v_t| of the selected
 // one.  Decisions against the reference's 1e-6 eigenvalue thresholds are
-// certified 256 eps32 |T| away from them, and positive definiteness of
-// |Gamma| with its smallest sweep pivot above 1e-4 (|Gamma| has unit
-// diagonal, so its pivots lie in (0, 1] when it is definite).
+// certified 256 eps32 |T| away from them, and the positive definiteness of
+// |Gamma| when no sweep pivot up to the first non-positive one lies within
+// 1e-4 of zero (|Gamma| has unit diagonal, so its pivots lie in (0, 1] when
+// it is definite).
 constexpr float kCertScale = 1.0e4f;
 constexpr float kCertThrMargin = 256.0f;
 constexpr double kCertPivotMin = 1.0e-4;
@@ -186,8 +187,8 @@ __device__ bool sweep_inverse(double* M, int MS, int n, double& pmin) {
     pmin = 1.0;
     for (int j = 0; j < n; ++j) {
         const double p = M[j * MS + j];
+        if (ok) pmin = fmin(pmin, fabs(p));   // up to the first pivot that is not positive
         ok = ok && (p > 0.0);
-        pmin = fmin(pmin, p);
         const double piv = 1.0 / p;
         __syncthreads();
 #pragma unroll

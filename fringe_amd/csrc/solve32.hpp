@@ -193,8 +193,8 @@ __device__ __forceinline__ bool sweep_block(LdsR<T, NR>& L, double (&M)[Blk<NR>:
             }
             wsync();
             const double p = L.PV;
+            if (ok) pmin = fmin(pmin, fabs(p));   // up to the first pivot that is not positive
             ok = ok && (p > 0.0);
-            pmin = fmin(pmin, p);
             const double piv = fast_rcp(p);
             double pc[8], g[BR];
 #pragma unroll

@@ -51,6 +51,16 @@ def read_outputs(folder, dates, rows, cols):
     return ph, tc, co
 
 
+def hdr_fields(path):
+    """key -> value of an ENVI header (<path>.hdr, the SUFFIX=ADD name)."""
+    out = {}
+    for line in open(path + ".hdr"):
+        if "=" in line:
+            k, v = line.split("=", 1)
+            out[k.strip().lower()] = v.strip()
+    return out
+
+
 def test_nmap_and_phase_link_modules(tmp_path):
     n, rows, cols, Nx, Ny = 10, 28, 33, 3, 2
     slc = synthetic.stack(n, rows, cols, seed=31, patches=6)
@@ -64,6 +74,14 @@ def test_nmap_and_phase_link_modules(tmp_path):
     wr, cr = oracle.nmap(slc, Nx, Ny, 0.05)
     assert np.array_equal(wts, wr) and np.array_equal(cnt, cr)
     assert raster.metadata(wf, "HALFWINDOWX", "ENVI") == str(Nx)
+    # headers as GDAL's ENVI driver writes them (nmap.cpp:236-253, :541-548):
+    # UInt32 BIP mask with the half-window keys, Int16 count
+    h = hdr_fields(wf)
+    assert (h["samples"], h["lines"], h["bands"]) == (str(cols), str(rows), str(nul))
+    assert (h["data type"], h["interleave"], h["byte order"]) == ("13", "bip", "0")
+    assert (h["halfwindowx"], h["halfwindowy"]) == (str(Nx), str(Ny))
+    h = hdr_fields(cf)
+    assert (h["samples"], h["lines"], h["bands"], h["data type"]) == (str(cols), str(rows), "1", "2")
     print("case: phase_link module, one block and many blocks", flush=True)
     out1, out2 = str(tmp_path / "pl1"), str(tmp_path / "pl2")
     assert pl_cli.main(["-i", base, "-w", wf, "-o", out1, "-x", str(Nx), "-y", str(Ny)]) == 0
@@ -79,6 +97,11 @@ def test_nmap_and_phase_link_modules(tmp_path):
     assert wrapped_err(g1[0], ref["phase"])[:, ok].max() <= PHASE_TOL
     assert np.abs(g1[1] - ref["tcorr"])[ok].max() <= TCORR_TOL
     assert os.path.exists(os.path.join(out1, dates[0] + ".slc.vrt"))
+    # output headers (evd.cpp:266-366): CFloat32 phases and compressed SLC, Float32 tcorr
+    for name, code in ((dates[0] + ".slc", "6"), ("tcorr.bin", "4"), ("compslc.bin", "6")):
+        h = hdr_fields(os.path.join(out1, name))
+        assert (h["samples"], h["lines"], h["bands"], h["data type"]) == (str(cols), str(rows), "1", code)
+        assert h["byte order"] == "0"
 
 
 @pytest.mark.parametrize("method", ["EVD", "MLE"])

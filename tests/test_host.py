@@ -81,6 +81,98 @@ def test_envi_and_vrt_roundtrip(tmp_path):
     assert raster.metadata(s, "Date", "slc", band=2) == "20220113"
 
 
+def test_vrt_variants_stripmap_and_uavsar(tmp_path):
+    """The raw and stack VRTs stripmap2vrt.py (:86-96, :136-152, :163-174) and
+    uavsar2vrt.py (:88-98, :124-140, :163-172) write: lower-case
+    <sourceFilename> in the per-date raw VRT, a bbox SrcRect in the stack,
+    Wavelength/AcquisitionTime metadata, Float64 geometry bands whose raw
+    band carries a SrcRect (ignored for raw bands, as GDAL does: the raster
+    is the top-left corner), and the UAVSAR LLH file, three float32 per pixel
+    read with PixelOffset 12 and ImageOffset 0 / 4 / 8."""
+    import os
+    rng = np.random.default_rng(5)
+    rows, cols = 11, 14
+    ymin, ymax, xmin, xmax = 2, 9, 3, 12
+    xs, ys = xmax - xmin, ymax - ymin
+    bands = []
+    for d in ("20200101", "20200113"):
+        z = (rng.standard_normal((rows, cols)) + 1j * rng.standard_normal((rows, cols))).astype(np.complex64)
+        f = str(tmp_path / (d + ".slc"))
+        z.tofile(f)
+        with open(str(tmp_path / (d + ".vrt")), "w") as fid:
+            fid.write(f'''<VRTDataset rasterXSize="{cols}" rasterYSize="{rows}">
+    <VRTRasterBand dataType="CFloat32" band="1" subClass="VRTRawRasterBand">
+        <sourceFilename>{os.path.abspath(f)}</sourceFilename>
+        <ImageOffset>0</ImageOffset>
+        <PixelOffset>8</PixelOffset>
+        <LineOffset>{8 * cols}</LineOffset>
+        <ByteOrder>LSB</ByteOrder>
+    </VRTRasterBand>
+</VRTDataset>''')
+        bands.append((d, z))
+    sv = str(tmp_path / "slcs_base.vrt")
+    with open(sv, "w") as fid:
+        fid.write(f'<VRTDataset rasterXSize="{xs}" rasterYSize="{ys}">\n')
+        for i, (d, _) in enumerate(bands):
+            fid.write(f'''    <VRTRasterBand dataType="CFloat32" band="{i + 1}">
+        <SimpleSource>
+            <SourceFilename>{tmp_path / (d + ".vrt")}</SourceFilename>
+            <SourceBand>1</SourceBand>
+            <SourceProperties RasterXSize="{cols}" RasterYSize="{rows}" DataType="CFloat32"/>
+            <SrcRect xOff="{xmin}" yOff="{ymin}" xSize="{xs}" ySize="{ys}"/>
+            <DstRect xOff="0" yOff="0" xSize="{xs}" ySize="{ys}"/>
+        </SimpleSource>
+        <Metadata domain="slc">
+            <MDI key="Date">{d}</MDI>
+            <MDI key="Wavelength">0.03</MDI>
+            <MDI key="AcquisitionTime">{d}</MDI>
+        </Metadata>
+    </VRTRasterBand>\n''')
+        fid.write("</VRTDataset>")
+    assert raster.info(sv) == (xs, ys, 2)
+    for i, (d, z) in enumerate(bands):
+        assert np.array_equal(raster.read(sv, i + 1), z[ymin:ymax, xmin:xmax])
+        assert raster.metadata(sv, "Date", "slc", band=i + 1) == d
+        assert raster.metadata(sv, "Wavelength", "slc", band=i + 1) == "0.03"
+    # stripmap geometry: Float64 raw band with a SrcRect inside it
+    lat = rng.standard_normal((rows, cols))
+    lf = str(tmp_path / "lat.rdr")
+    lat.tofile(lf)
+    gv = str(tmp_path / "lat.vrt")
+    with open(gv, "w") as fid:
+        fid.write(f'''<VRTDataset rasterXSize="{xs}" rasterYSize="{ys}">
+    <VRTRasterBand dataType="Float64" band="1" subClass="VRTRawRasterBand">
+        <SourceFilename>{lf}</SourceFilename>
+        <ImageOffset>0</ImageOffset>
+        <PixelOffset>8</PixelOffset>
+        <LineOffset>{8 * cols}</LineOffset>
+        <ByteOrder>LSB</ByteOrder>
+        <SrcRect xOff="{xmin}" yOff="{ymin}" xSize="{xs}" ySize="{ys}"/>
+        <DstRect xOff="0" yOff="0" xSize="{xs}" ySize="{ys}"/>
+    </VRTRasterBand>
+</VRTDataset>''')
+    assert raster.info(gv) == (xs, ys, 1)
+    # (read converted to float32, the widest real type the reader hands out)
+    assert np.array_equal(raster.read(gv, dtype=np.float32), lat[:ys, :xs].astype(np.float32))
+    # UAVSAR LLH: (lat, lon, height) float32 triplets per pixel
+    llh = rng.standard_normal((rows, cols, 3)).astype(np.float32)
+    hf = str(tmp_path / "x.llh")
+    llh.tofile(hf)
+    for k in range(3):
+        v = str(tmp_path / f"llh{k}.vrt")
+        with open(v, "w") as fid:
+            fid.write(f'''<VRTDataset rasterXSize="{cols}" rasterYSize="{rows}">
+    <VRTRasterBand dataType="Float32" band="1" subClass="VRTRawRasterBand">
+        <SourceFilename>{hf}</SourceFilename>
+        <ImageOffset>{4 * k}</ImageOffset>
+        <PixelOffset>12</PixelOffset>
+        <LineOffset>{12 * cols}</LineOffset>
+        <ByteOrder>LSB</ByteOrder>
+    </VRTRasterBand>
+</VRTDataset>''')
+        assert np.array_equal(raster.read(v, dtype=np.float32), llh[:, :, k])
+
+
 def test_multiband_uint32_bip_and_envi_metadata(tmp_path):
     rng = np.random.default_rng(2)
     w = rng.integers(0, 2 ** 32, size=(5, 6, 4), dtype=np.uint64).astype(np.uint32)

@@ -30,7 +30,8 @@ EXPORTS = [
     "fringe_despeck_options_init", "fringe_despeck_process",
     "fringe_calamp_options_init", "fringe_calamp_process",
     "fringe_coherence_workspace_bytes", "fringe_coherence_run", "fringe_link_resolved_pixels",
-    "fringe_link_run_noalloc",
+    "fringe_link_run_noalloc", "fringe_link_batch_workspace_bytes", "fringe_link_batch_run",
+    "fringe_schedule_set", "fringe_schedule_get", "fringe_schedule_clear",
 ]
 FRINGE_REQ_QUEUED, FRINGE_REQ_RUNNING, FRINGE_REQ_DONE = 0, 1, 2
 
@@ -76,6 +77,38 @@ class CalampOptions(ctypes.Structure):
     _fields_ = [("inputDS", ctypes.c_char_p), ("maskDS", ctypes.c_char_p), ("outputDS", ctypes.c_char_p),
                 ("defaultValue", ctypes.c_double), ("applySqrt", ctypes.c_int), ("blocksize", ctypes.c_int),
                 ("memsize", ctypes.c_int), ("verbose", ctypes.c_int)]
+
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [("tile", LinkTile), ("d_slc", ctypes.c_void_p), ("d_wts", ctypes.c_void_p),
+                ("d_phase", ctypes.c_void_p), ("d_tcorr", ctypes.c_void_p), ("d_comp", ctypes.c_void_p)]
+
+
+FRINGE_SCHED_GRAM_VARIANT, FRINGE_SCHED_SOLVE = 0, 1
+GRAM_VARIANTS = {"default": 0, "staged12": 1, "staged8": 2, "staged4": 3, "unstaged": 4}
+SCHEDULE_FILE = os.path.join(_HERE, "schedules.json")
+
+
+def load_schedules(L, path=None):
+    """Install the autotuned launch schedules (fringe_amd/autotune.py writes
+    them) into the library's table.  FRINGE_SCHEDULES=<file> picks another
+    table, FRINGE_SCHEDULES=none keeps the built-in defaults.  Returns the
+    number of entries installed."""
+    import json
+    path = path or os.environ.get("FRINGE_SCHEDULES", SCHEDULE_FILE)
+    if path == "none" or not os.path.exists(path):
+        return 0
+    with open(path) as f:
+        table = json.load(f)
+    k = 0
+    for e in table.get("gram", []):
+        if L.fringe_schedule_set(FRINGE_SCHED_GRAM_VARIANT, e["n"], e["Nx"], e["Ny"], GRAM_VARIANTS[e["variant"]]) == 0:
+            k += 1
+    for e in table.get("solve", []):
+        v = 0 if e["launch"] == "grid" else (int(e["waves_per_cu"]) << 8) | int(e["grab"])
+        if L.fringe_schedule_set(FRINGE_SCHED_SOLVE, e["n"], e["mode"], e["precision"], v) == 0:
+            k += 1
+    return k
 
 
 class ServerRequest(ctypes.Structure):
@@ -144,6 +177,13 @@ def lib():
                      ("fringe_calamp_options_init", CalampOptions), ("fringe_calamp_process", CalampOptions)):
         if hasattr(L, name):
             getattr(L, name).argtypes = [ctypes.POINTER(st)]
+    bi = ctypes.POINTER(BatchItem)
+    L.fringe_link_batch_workspace_bytes.argtypes = [bi, ctypes.c_int]
+    L.fringe_link_batch_workspace_bytes.restype = sz
+    L.fringe_link_batch_run.argtypes = [bi, ctypes.c_int, vp, sz, vp]
+    L.fringe_schedule_set.argtypes = [ctypes.c_int] * 5
+    L.fringe_schedule_get.argtypes = [ctypes.c_int] * 4
+    load_schedules(L)
     _LIB = L
     return L
 

@@ -1,0 +1,139 @@
+"""Autotuner of the launch schedules (run on the GPU):
+
+    python -m fringe_amd.autotune [--out fringe_amd/schedules.json] [--quick]
+
+For every key of the schedule table it times each candidate on a resident
+synthetic tile, with the engine's own stage timers (HIP events around the
+stage's launches), and keeps the fastest:
+
+* Gram variant, per (date class, window): the staged MFMA kernel on 12, 8
+  or 4 waves (8 x 16, 4 x 16 or up to 4 x 8 pixel tiles), or the unstaged one.
+* Solver launch, per (date class, estimator, precision): one wave per unit,
+  or a persistent grid of W waves per compute unit taking G units at a time
+  from the work queue.
+
+The chosen table (with every candidate's time) is written as JSON; the
+library installs it when it is loaded (fringe_amd._lib.load_schedules).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+MODES = {0: ("phase_link", "MLE"), 1: ("evd", "EVD"), 2: ("evd", "MLE")}
+
+
+def _time_stage(L, run, stage, reps):
+    """Median over reps of the summed stage time (ms) of one call of run()."""
+    import torch
+    from fringe_amd import _lib
+    run()   # warm-up (kernel loading, caches)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        L.fringe_timing_reset()
+        L.fringe_timing_enable(1)
+        run()
+        torch.cuda.synchronize()
+        L.fringe_timing_enable(0)
+        ts.append(_lib.stage_times().get(stage, (0.0, 0))[0])
+    return float(np.median(ts))
+
+
+def tune(quick=False, log=sys.stderr):
+    import torch
+    from fringe_amd import _lib, synthetic, tile
+    L = _lib.lib()
+    L.fringe_schedule_clear()   # tune from the built-in defaults
+    dev = torch.device("cuda:0")
+    reps = 2 if quick else 3
+    rows = 96 if quick else 256
+    cols = 2048
+    gram, solve = [], []
+    stacks = {}
+
+    def data(n, Nx, Ny, fill=None):
+        key = (n, Nx, Ny, fill)
+        if key not in stacks:
+            slc = synthetic.stack_torch(n, rows, cols, 4000 + n, device=dev, patches=32)
+            if fill is None:
+                w, _ = tile.nmap(slc, Nx, Ny, 0.05)
+            else:
+                w = synthetic.random_mask_torch(rows, cols, Nx, Ny, fill, 4100 + n, device=dev)
+            stacks[key] = (slc, w)
+        return stacks[key]
+
+    # ---- Gram variants (mixed precision, date class <= 32) ----------------
+    for (Nx, Ny) in ((7, 5), (5, 5), (6, 5), (10, 5), (3, 2)):
+        n = 30
+        slc, w = data(n, Nx, Ny)
+        p = tile.link_params(rows, cols, n, Nx, Ny, min_neighbors=5, precision="mixed")
+        times = {}
+        for name, v in _lib.GRAM_VARIANTS.items():
+            L.fringe_schedule_set(_lib.FRINGE_SCHED_GRAM_VARIANT, n, Nx, Ny, v)
+            r = tile.LinkRunner(p, device=dev)
+            times[name] = _time_stage(L, lambda: r.run(slc, w), "gram", reps)
+            del r
+        L.fringe_schedule_set(_lib.FRINGE_SCHED_GRAM_VARIANT, n, Nx, Ny, 0)
+        best = min(times, key=times.get)
+        gram.append({"n": n, "Nx": Nx, "Ny": Ny, "variant": best, "ms": times})
+        print(f"gram n<=32 window {2 * Nx + 1}x{2 * Ny + 1}: {best} {times}", file=log, flush=True)
+
+    # ---- solver launches ---------------------------------------------------
+    cands = [("grid", 0, 0)] + [("persistent", wv, g) for wv in ((8, 16) if quick else (8, 12, 16, 24))
+                                for g in ((1, 4) if quick else (1, 2, 4, 8))]
+    cases = [(30, 0, "mixed"), (30, 0, "fp64"), (20, 2, "mixed"), (30, 1, "mixed"),
+             (48, 0, "mixed"), (64, 1, "mixed"), (96, 0, "mixed")]
+    if quick:
+        cases = cases[:3]
+    for n, mode, prec in cases:
+        module, method = MODES[mode]
+        Nx, Ny = (10, 5) if n == 64 else (7, 5)
+        slc, w = data(n, Nx, Ny, fill=None if n <= 64 else 0.9)
+        p = tile.link_params(rows if n <= 64 else 16, cols, n, Nx, Ny, module=module, method=method,
+                             min_neighbors=5, precision=prec)
+        if n > 64:
+            slc, w = slc[:, :16].contiguous(), w[:16].contiguous()
+        key = (n, mode, 0 if prec == "fp64" else 1)
+        times = {}
+        for launch, wv, g in cands:
+            L.fringe_schedule_set(_lib.FRINGE_SCHED_SOLVE, *key, 0 if launch == "grid" else (wv << 8) | g)
+            r = tile.LinkRunner(p, device=dev)
+            times[f"{launch}:{wv}:{g}"] = _time_stage(L, lambda: r.run(slc, w), "solve", reps)
+            del r
+        L.fringe_schedule_set(_lib.FRINGE_SCHED_SOLVE, *key, 0)
+        best = min(times, key=times.get)
+        launch, wv, g = best.split(":")
+        solve.append({"n": n, "mode": mode, "precision": key[2], "estimator": f"{module}/{method}",
+                      "launch": launch, "waves_per_cu": int(wv), "grab": int(g), "ms": times})
+        print(f"solve n={n} {module}/{method} {prec}: {best} ({times[best]:.2f} ms, grid {times['grid:0:0']:.2f})",
+              file=log, flush=True)
+    return {"device": torch.cuda.get_device_name(0), "tile": f"{rows} x {cols}",
+            "note": "stage times in ms per call (median); the library installs the chosen entries",
+            "gram": gram, "solve": solve}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--out", default=os.path.join(HERE, "schedules.json"))
+    ap.add_argument("--quick", action="store_true", help="fewer candidates and a smaller tile")
+    a = ap.parse_args(argv)
+    t0 = time.time()
+    table = tune(quick=a.quick)
+    table["tuning_s"] = round(time.time() - t0, 1)
+    with open(a.out, "w") as f:
+        json.dump(table, f, indent=1)
+    print(json.dumps({"written": a.out, "gram": len(table["gram"]), "solve": len(table["solve"])}))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -3,6 +3,7 @@
 // include/fringe_amd.h.
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -204,7 +205,7 @@ static int link_chunk_pixels(const fringe_link_tile* t) {
 }
 
 struct LinkLayout {
-    size_t gamma, npix, pack, cert, awork, awork_stride, rwork, total;
+    size_t gamma, npix, pack, cert, queue, awork, awork_stride, rwork, total;
 };
 
 static LinkLayout link_layout(const fringe_link_tile* t, int chunk) {
@@ -215,6 +216,8 @@ static LinkLayout link_layout(const fringe_link_tile* t, int chunk) {
     L.cert = b;   // first, so its place does not depend on the chunk
     if (link_certifies(n, t->precision))   // the count, then the list of the tile's output pixels
         b += 256 + align256((size_t)(t->row_end - t->row_begin) * t->cols * sizeof(int));
+    L.queue = b;
+    b += 256;
     L.gamma = b;
     b += align256((size_t)chunk * n * n * e);
     L.npix = b;
@@ -289,6 +292,7 @@ int link_run_chunked(const fringe_link_tile* t, const void* d_slc, const uint32_
         P.awork_stride = LL.awork_stride;
     }
     if (P.n > 64 && link_certifies(P.n, P.precision)) P.d_rwork = (char*)d_work + LL.rwork;
+    P.d_queue = (int*)((char*)d_work + LL.queue);
     hipStream_t s = (hipStream_t)stream;
     if (link_launch(P, (const float2*)d_slc, d_wts, (float2*)d_phase, d_tcorr, (float2*)d_comp, s) != 0) {
         set_last_error("fringe_link_run", hipGetLastError());
@@ -329,6 +333,128 @@ long long fringe_link_resolved_pixels(const fringe_link_tile* t, const void* d_w
     int c = 0;   // the count sits at the start of the workspace
     if (hipMemcpy(&c, d_work, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return -1;
     return c;
+}
+
+// ---- batch of independent tiles --------------------------------------------
+// Layout: queue counters; per tile its certification list and packed
+// records; the round buffer (coherence, and work matrices above 64 dates in
+// global memory) of up to ~2 GB; the round's neighbour counts; the FP64
+// re-solve areas above 64 dates.
+struct BatchLayout {
+    std::vector<size_t> cert, pack;
+    std::vector<size_t> astride;
+    size_t queue, round, round_bytes, npix, rwork, total;
+};
+
+static int batch_check(const fringe_link_batch_item* items, int count) {
+    if (!items || count <= 0) return FRINGE_ERR_ARGUMENT;
+    for (int i = 0; i < count; ++i) {
+        const int st = link_check(&items[i].tile);
+        if (st) return st;
+        if (items[i].tile.module != items[0].tile.module || items[i].tile.method != items[0].tile.method ||
+            items[i].tile.precision != items[0].tile.precision)
+            return FRINGE_ERR_UNSUPPORTED;
+    }
+    return 0;
+}
+
+static BatchLayout batch_layout(const fringe_link_batch_item* items, int count) {
+    BatchLayout L;
+    size_t b = 0;
+    L.queue = b;
+    b += 256;
+    size_t need = 0, max_row = 0, max_rwork = 0, pixels = 0;
+    for (int i = 0; i < count; ++i) {
+        const fringe_link_tile* t = &items[i].tile;
+        const int n = t->nbands;
+        const long long out = (long long)(t->row_end - t->row_begin) * t->cols;
+        L.cert.push_back(b);
+        if (link_certifies(n, t->precision)) b += 256 + align256((size_t)out * sizeof(int));
+        L.pack.push_back(b);
+        if (link_uses_mfma(n, t->precision)) b += align256(mfma_pack_bytes(t->rows, t->cols, n));
+        size_t as = 0;
+        if (n > 64 && !solve_a_in_lds(n, t->precision))
+            as = align256((size_t)n * (n | 1) * (t->precision == FRINGE_PREC_FP64 ? 16 : 8));
+        L.astride.push_back(as);
+        const size_t per_px = link_gamma_bytes_per_pixel(n, t->precision) - sizeof(int) + as;
+        need += (size_t)out * per_px + 512;
+        max_row = std::max(max_row, (size_t)t->cols * per_px + 1024);
+        pixels += (size_t)out;
+        if (n > 64 && link_certifies(n, t->precision))
+            max_rwork = std::max(max_rwork, (size_t)kResolveGenericBlocks * resolve_generic_area_bytes(n));
+    }
+    L.round = b;
+    L.round_bytes = std::max(std::min(need, (size_t)2048 << 20), max_row);
+    b += align256(L.round_bytes);
+    L.npix = b;
+    b += align256(pixels * sizeof(int));
+    L.rwork = b;
+    b += max_rwork;
+    L.total = b;
+    return L;
+}
+
+extern "C" size_t fringe_link_batch_workspace_bytes(const fringe_link_batch_item* items, int count) {
+    if (batch_check(items, count) != 0) return 0;
+    return batch_layout(items, count).total;
+}
+
+extern "C" int fringe_link_batch_run(const fringe_link_batch_item* items, int count, void* d_work,
+                                     size_t work_bytes, void* stream) {
+    const int st = batch_check(items, count);
+    if (st) return st;
+    const BatchLayout L = batch_layout(items, count);
+    if (!d_work || work_bytes < L.total) return FRINGE_ERR_WORKSPACE;
+    BatchPlan B;
+    B.module = items[0].tile.module;
+    B.method = items[0].tile.method;
+    B.precision = items[0].tile.precision;
+    char* w = (char*)d_work;
+    B.d_queue = (int*)(w + L.queue);
+    B.d_round = w + L.round;
+    B.round_bytes = L.round_bytes;
+    B.d_npix = (int*)(w + L.npix);
+    B.d_rwork = L.rwork < L.total ? w + L.rwork : nullptr;
+    for (int i = 0; i < count; ++i) {
+        const fringe_link_batch_item& it = items[i];
+        const fringe_link_tile* t = &it.tile;
+        if (!it.d_slc || !it.d_wts || !it.d_phase || !it.d_tcorr || !it.d_comp) return FRINGE_ERR_ARGUMENT;
+        BatchTile T{};
+        LinkPlan& P = T.P;
+        P.rows = t->rows;
+        P.cols = t->cols;
+        P.n = t->nbands;
+        P.Nx = t->Nx;
+        P.Ny = t->Ny;
+        P.nul = shp_words(t->Nx, t->Ny);
+        P.module = t->module;
+        P.method = t->method;
+        P.min_nb = (t->module == FRINGE_MODULE_EVD) ? 2 : t->min_neighbors;
+        P.ref = t->ref_index;
+        P.bw = t->bandwidth;
+        P.precision = t->precision;
+        P.row_begin = t->row_begin;
+        P.row_end = t->row_end;
+        P.npx = (long long)t->rows * t->cols;
+        P.d_pack = (uint32_t*)(w + L.pack[i]);
+        if (link_certifies(P.n, P.precision)) {
+            P.d_cert_count = (int*)(w + L.cert[i]);
+            P.d_cert_list = (int*)(w + L.cert[i] + 256);
+            P.cert_cap = (int)((long long)(t->row_end - t->row_begin) * t->cols);
+        }
+        P.awork_stride = L.astride[i];
+        T.slc = (const float2*)it.d_slc;
+        T.wts = it.d_wts;
+        T.phase = (float2*)it.d_phase;
+        T.tcorr = it.d_tcorr;
+        T.comp = (float2*)it.d_comp;
+        B.tiles.push_back(T);
+    }
+    if (link_batch_launch(B, (hipStream_t)stream) != 0) {
+        set_last_error("fringe_link_batch_run", hipGetLastError());
+        return FRINGE_ERR_DEVICE;
+    }
+    return 0;
 }
 
 size_t fringe_coherence_workspace_bytes(const fringe_link_tile* t) {

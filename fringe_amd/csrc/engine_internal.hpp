@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "fringe_amd.h"
 
@@ -61,7 +62,30 @@ struct LinkPlan {
     void* d_awork;
     size_t awork_stride;
     void* d_rwork;
+    int* d_queue;       // work queue counter of a persistent solve launch
 };
+
+// A batch of independent tiles (fringe_link_batch_run): per tile its plan
+// (pack records, certification list, FP64 re-solve areas) and buffers; the
+// rounds share one coherence buffer and one neighbour-count array.
+struct BatchTile {
+    LinkPlan P;
+    const float2* slc;
+    const uint32_t* wts;
+    float2* phase;
+    float* tcorr;
+    float2* comp;
+};
+struct BatchPlan {
+    std::vector<BatchTile> tiles;
+    int module, method, precision;
+    void* d_round;       // round buffer: coherence (and work) matrices of the segments
+    size_t round_bytes;
+    int* d_npix;         // neighbour counts of a round's slots
+    int* d_queue;        // [3] queue counters, one per date class
+    void* d_rwork;       // FP64 re-solve areas above 64 dates
+};
+int link_batch_launch(const BatchPlan& B, hipStream_t s);
 
 // The FP64 re-solve of uncertified pixels (n <= 64): dynamic LDS per lane
 // group (its coherence plus the FP64 Gram's scratch), and the whole
@@ -77,6 +101,12 @@ bool solve_a_in_lds(int n, int precision);
 size_t resolve_generic_area_bytes(int n);
 size_t resolve_generic_lds_bytes(int n, int Nx, int Ny);
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+// Value of a schedule key (fringe_schedule_set), or fallback when unset.
+int schedule_lookup(int kind, int k0, int k1, int k2, int fallback);
+// Estimator index of the solver schedules: 0 phase_link EMI, 1 evd EVD/STBAS, 2 evd MLE.
+inline int solve_mode_index(int module, int method) {
+    return module == FRINGE_MODULE_PHASE_LINK ? 0 : (method == FRINGE_METHOD_MLE ? 2 : 1);
+}
 
 // SHP-masked neighbourhood mean (despeck.cpp:300-433); mode 0 amplitude,
 // 1 interferogram, 2 interferogram with coherence.  c1, c2: rows*cols

@@ -573,16 +573,23 @@ static int gram_list_cap(int Nx, int Ny) {
 // a 3 x 8 tile stages ~11.9).  Otherwise the largest TH x TW tile
 // (TH <= 4, TW <= 8) for four waves whose stage keeps two workgroups on a
 // CU.  Returns the stage's LDS bytes.
-static size_t gram_stage_tile(int Nx, int Ny, int& TH, int& TW, int& waves) {
+// `only`: a scheduled variant (FRINGE_GRAM_STAGED12 / 8 / 4) tries that tile
+// alone; FRINGE_GRAM_DEFAULT tries them in order.
+static size_t gram_stage_tile(int Nx, int Ny, int& TH, int& TW, int& waves, int only = FRINGE_GRAM_DEFAULT) {
     const int cap = gram_list_cap(Nx, Ny);
     waves = 12;
     TH = 8;
     TW = 16;
-    if (gram_lds_bytes(Nx, Ny, TH, TW, cap, waves) <= 156 * 1024) return gram_lds_bytes(Nx, Ny, TH, TW, cap, waves);
+    if ((only == FRINGE_GRAM_DEFAULT || only == FRINGE_GRAM_STAGED12) &&
+        gram_lds_bytes(Nx, Ny, TH, TW, cap, waves) <= 156 * 1024)
+        return gram_lds_bytes(Nx, Ny, TH, TW, cap, waves);
     waves = 8;
     TH = 4;
     TW = 16;
-    if (gram_lds_bytes(Nx, Ny, TH, TW, cap, waves) <= 150 * 1024) return gram_lds_bytes(Nx, Ny, TH, TW, cap, waves);
+    if ((only == FRINGE_GRAM_DEFAULT || only == FRINGE_GRAM_STAGED8) &&
+        gram_lds_bytes(Nx, Ny, TH, TW, cap, waves) <= 150 * 1024)
+        return gram_lds_bytes(Nx, Ny, TH, TW, cap, waves);
+    if (only != FRINGE_GRAM_DEFAULT && only != FRINGE_GRAM_STAGED4) return (size_t)1 << 40;
     const size_t budget = 78 * 1024;
     waves = 4;
     TH = 4;
@@ -602,8 +609,16 @@ static size_t gram_stage_tile(int Nx, int Ny, int& TH, int& TW, int& waves) {
 
 // The staged kernel takes windows of up to kStageWords mask words; larger
 // windows go to the unstaged kernel, which walks any number of words.
+// A scheduled variant (fringe_schedule_set) that does not fit the window
+// falls back to the default choice.
 static bool gram_staged(int n, int Nx, int Ny, int& TH, int& TW, int& waves, size_t& bytes) {
     if (n > 32 || shp_words(Nx, Ny) > kStageWords) return false;
+    const int v = schedule_lookup(FRINGE_SCHED_GRAM_VARIANT, n, Nx, Ny, FRINGE_GRAM_DEFAULT);
+    if (v == FRINGE_GRAM_UNSTAGED) return false;
+    if (v != FRINGE_GRAM_DEFAULT) {
+        bytes = gram_stage_tile(Nx, Ny, TH, TW, waves, v);
+        if (bytes <= 160 * 1024) return true;
+    }
     bytes = gram_stage_tile(Nx, Ny, TH, TW, waves);
     return bytes <= 160 * 1024;
 }

@@ -13,6 +13,8 @@
 //   * the inverse of the real |Gamma| by a Gauss-Jordan sweep in FP64, whose
 //     pivots are the Cholesky pivots, so "not positive definite" is decided
 //     the way zpotrf decides it.
+#include <algorithm>
+
 #include "device_common.hpp"
 #include "engine_internal.hpp"
 #include "link_solve.hpp"
@@ -250,12 +252,180 @@ int coherence_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, 
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// ---------------------------------------------------------------------------
+// Pieces of a link: the Gram of a segment of whole rows, the solver launch
+// (a grid, or a persistent queue of jobs), the FP64 re-solve of a tile.
+// ---------------------------------------------------------------------------
+static void gram_segment(const LinkPlan& P, const float2* slc, const uint32_t* wts, long long first, int slots,
+                         void* gamma, int* npix, hipStream_t s, int* rc) {
+    if (!link_uses_mfma(P.n, P.precision)) {
+        GramArgs g{};
+        g.slc = slc;
+        g.wts = wts;
+        g.rows = P.rows;
+        g.cols = P.cols;
+        g.n = P.n;
+        g.Nx = P.Nx;
+        g.Ny = P.Ny;
+        g.nul = P.nul;
+        g.min_nb = P.min_nb;
+        g.npx = P.npx;
+        g.first = first;
+        g.slots = slots;
+        g.gamma = gamma;
+        g.npix = npix;
+        if (P.n <= 32) launch_gram_f64<32, 1, 32>(g, s);
+        else if (P.n <= 64) launch_gram_f64<64, 1, 32>(g, s);
+        else if (P.n <= 128) launch_gram_f64<64, 2, 16>(g, s);
+        else launch_gram_f64<64, 4, 16>(g, s);
+    } else {
+        LinkPlan Q = P;
+        Q.d_gamma = gamma;
+        Q.d_npix = npix;
+        if (gram_mfma_launch(Q, P.d_pack, wts, first, slots, s) != 0) *rc = -1;
+    }
+}
+
+static int cu_count() {
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cached[dev]) {
+        hipDeviceProp_t prop;
+        cached[dev] = (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+                          ? prop.multiProcessorCount : 256;
+    }
+    return cached[dev];
+}
+
+// Date class of the solvers: 0 register NR = 32 (n <= 32), 1 register
+// NR = 64 (n <= 64), 2 generic (n > 64).
+static int solve_class(int n) { return n <= 32 ? 0 : (n <= 64 ? 1 : 2); }
+
+static PJob make_job(const LinkPlan& P, const float2* slc, float2* phase, float* tcorr, float2* comp,
+                     long long first, int slots, const void* gamma, const int* npix, void* awork, size_t awork_stride) {
+    PJob J{};
+    J.gamma = gamma;
+    J.npix = npix;
+    J.slc = slc;
+    J.phase = phase;
+    J.tcorr = tcorr;
+    J.comp = comp;
+    J.cert_list = P.d_cert_list;
+    J.cert_count = P.d_cert_count;
+    J.cert_cap = P.cert_cap;
+    J.awork = awork;
+    J.awork_stride = awork_stride;
+    J.npx = P.npx;
+    J.first = first;
+    J.slots = slots;
+    J.n = P.n;
+    J.ref = P.ref;
+    J.bw = P.bw;
+    const bool f64 = P.precision == FRINGE_PREC_FP64;
+    J.group_bytes = (unsigned)(f64 ? solve_group_bytes<double>(P.n, awork == nullptr)
+                                   : solve_group_bytes<float>(P.n, awork == nullptr));
+    return J;
+}
+
+template <typename K> static int occupancy_blocks(K kernel, int threads, size_t lds) {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, threads, lds) != hipSuccess || b < 1) b = 1;
+    return b;
+}
+
+// One persistent launch over the jobs of pa (all of class cls).  sched: 0
+// = default (one full residency of the kernel, grab 4), else
+// (waves_per_cu << 8 | grab).
+template <typename T, typename Tg>
+static void launch_persistent(PersistArgs pa, int cls, int mode, int sched, hipStream_t s) {
+    int grab = 4, waves_cu = 0;
+    if (sched) {
+        waves_cu = sched >> 8;
+        grab = sched & 0xff;
+    }
+    pa.grab = grab;
+    int units = 0;
+    size_t lds = 0;
+    for (int j = 0; j < pa.njobs; ++j) {
+        const int u = cls == 0 ? (pa.job[j].slots + 1) / 2 : pa.job[j].slots;
+        units += u;
+        pa.unit_end[j] = units;
+        lds = std::max<size_t>(lds, pa.job[j].group_bytes);
+    }
+    if (units == 0) return;
+    hipLaunchKernelGGL(zero_i32, dim3(1), dim3(64), 0, s, pa.queue, 1);
+    const int cus = cu_count();
+    auto grid_for = [&](auto kernel, int threads, size_t dyn) {
+        const int per_cu = waves_cu ? std::max(1, waves_cu / (threads / 64)) : occupancy_blocks(kernel, threads, dyn);
+        return dim3((unsigned)std::min<long long>((long long)per_cu * cus, (units + grab - 1) / grab));
+    };
+    // coherence in double above 64 dates (both precisions), in float below in the mixed ones
+    if constexpr (sizeof(T) == 8 || sizeof(Tg) == 8) {
+        if (cls == 2) {
+            int nmax = 0;
+            for (int j = 0; j < pa.njobs; ++j) nmax = std::max(nmax, pa.job[j].n);
+            if (nmax <= 128) {
+                auto k = solve_generic_persistent<T, 1>;
+                hipLaunchKernelGGL(k, grid_for(k, 128, lds), dim3(128), lds, s, pa);
+            } else {
+                auto k = solve_generic_persistent<T, 2>;
+                hipLaunchKernelGGL(k, grid_for(k, 128, lds), dim3(128), lds, s, pa);
+            }
+            return;
+        }
+    }
+    if constexpr (sizeof(T) == sizeof(Tg)) {
+#define FRINGE_PERSIST(MODE_, NR_)                                                   \
+    do {                                                                             \
+        auto k = solve_reg_persistent<T, Tg, MODE_, NR_>;                            \
+        hipLaunchKernelGGL(k, grid_for(k, 64, 0), dim3(64), 0, s, pa);              \
+    } while (0)
+        if (cls == 0) {
+            if (mode == 0) FRINGE_PERSIST(MODE_EMI, 32);
+            else if (mode == 2) FRINGE_PERSIST(MODE_MLE, 32);
+            else FRINGE_PERSIST(MODE_EVD, 32);
+        } else {
+            if (mode == 0) FRINGE_PERSIST(MODE_EMI, 64);
+            else if (mode == 2) FRINGE_PERSIST(MODE_MLE, 64);
+            else FRINGE_PERSIST(MODE_EVD, 64);
+        }
+#undef FRINGE_PERSIST
+    }
+}
+
+static void launch_persistent_any(const PersistArgs& pa, int cls, int mode, int precision, int sched, hipStream_t s) {
+    if (precision == FRINGE_PREC_FP64) launch_persistent<double, double>(pa, cls, mode, sched, s);
+    else if (cls == 2) launch_persistent<float, double>(pa, cls, mode, sched, s);
+    else launch_persistent<float, float>(pa, cls, mode, sched, s);
+}
+
+static void resolve_tile(const LinkPlan& P, const float2* slc, const uint32_t* wts, float2* phase, float* tcorr,
+                         float2* comp, hipStream_t s) {
+    StageScope sc(STAGE_RESOLVE, s);
+    SolveArgs a{};
+    a.slc = slc;
+    a.npx = P.npx;
+    a.n = P.n;
+    a.module = P.module;
+    a.method = P.method;
+    a.ref = P.ref;
+    a.bw = P.bw;
+    a.phase = phase;
+    a.tcorr = tcorr;
+    a.comp = comp;
+    if (P.n <= 64) launch_resolve(P, wts, a, s);
+    else launch_resolve_generic(P, wts, a, s);
+}
+
 int link_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, float2* phase,
                 float* tcorr, float2* comp, hipStream_t s) {
     const long long first_out = (long long)P.row_begin * P.cols;
     const long long total = (long long)(P.row_end - P.row_begin) * P.cols;
     const bool fp64 = (P.precision == FRINGE_PREC_FP64);
     const bool mfma = link_uses_mfma(P.n, P.precision);
+    const int mode = solve_mode_index(P.module, P.method);
+    const int sched = schedule_lookup(FRINGE_SCHED_SOLVE, P.n, mode, fp64 ? 0 : 1, 0);
     if (P.d_cert_count) {
         hipLaunchKernelGGL(zero_i32, dim3(1), dim3(64), 0, s, P.d_cert_count, 1);
     }
@@ -264,55 +434,44 @@ int link_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, float
         if (pack_launch(slc, P.rows, P.cols, P.n, P.precision != FRINGE_PREC_MIXED_BF16, P.d_pack, s) != 0)
             return -1;
     }
+    int rc = 0;
     for (long long done = 0; done < total; done += P.chunk) {
         const int slots = (int)((total - done) < P.chunk ? (total - done) : P.chunk);
         const long long first = first_out + done;
         {
-        StageScope sc(STAGE_GRAM, s);
-        if (!mfma) {
-            GramArgs g{};
-            g.slc = slc;
-            g.wts = wts;
-            g.rows = P.rows;
-            g.cols = P.cols;
-            g.n = P.n;
-            g.Nx = P.Nx;
-            g.Ny = P.Ny;
-            g.nul = P.nul;
-            g.min_nb = P.min_nb;
-            g.npx = P.npx;
-            g.first = first;
-            g.slots = slots;
-            g.gamma = P.d_gamma;
-            g.npix = P.d_npix;
-            if (P.n <= 32) launch_gram_f64<32, 1, 32>(g, s);
-            else if (P.n <= 64) launch_gram_f64<64, 1, 32>(g, s);
-            else if (P.n <= 128) launch_gram_f64<64, 2, 16>(g, s);
-            else launch_gram_f64<64, 4, 16>(g, s);
+            StageScope sc(STAGE_GRAM, s);
+            gram_segment(P, slc, wts, first, slots, P.d_gamma, P.d_npix, s, &rc);
+            if (rc) return -1;
+        }
+        StageScope sc(STAGE_SOLVE, s);
+        if (sched && P.d_queue) {
+            PersistArgs pa{};
+            pa.module = P.module;
+            pa.method = P.method;
+            pa.njobs = 1;
+            pa.queue = P.d_queue;
+            pa.job[0] = make_job(P, slc, phase, tcorr, comp, first, slots, P.d_gamma, P.d_npix, P.d_awork,
+                                 P.awork_stride);
+            launch_persistent_any(pa, solve_class(P.n), mode, P.precision, sched, s);
         } else {
-            if (gram_mfma_launch(P, P.d_pack, wts, first, slots, s) != 0) return -1;
-        }
-        }
-        SolveArgs a{};
-        a.cert_list = P.d_cert_list;
-        a.cert_count = P.d_cert_count;
-        a.cert_cap = P.cert_cap;
-        a.gamma = P.d_gamma;
-        a.npix = P.d_npix;
-        a.slc = slc;
-        a.npx = P.npx;
-        a.first = first;
-        a.slots = slots;
-        a.n = P.n;
-        a.module = P.module;
-        a.method = P.method;
-        a.ref = P.ref;
-        a.bw = P.bw;
-        a.phase = phase;
-        a.tcorr = tcorr;
-        a.comp = comp;
-        {
-            StageScope sc(STAGE_SOLVE, s);
+            SolveArgs a{};
+            a.cert_list = P.d_cert_list;
+            a.cert_count = P.d_cert_count;
+            a.cert_cap = P.cert_cap;
+            a.gamma = P.d_gamma;
+            a.npix = P.d_npix;
+            a.slc = slc;
+            a.npx = P.npx;
+            a.first = first;
+            a.slots = slots;
+            a.n = P.n;
+            a.module = P.module;
+            a.method = P.method;
+            a.ref = P.ref;
+            a.bw = P.bw;
+            a.phase = phase;
+            a.tcorr = tcorr;
+            a.comp = comp;
             if (P.n <= 64) {
                 if (fp64) dispatch_reg<double, double>(a, s);
                 else dispatch_reg<float, float>(a, s);
@@ -326,21 +485,132 @@ int link_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, float
         if (hipGetLastError() != hipSuccess) return -1;
     }
     if (P.d_cert_count) {
-        StageScope sc(STAGE_RESOLVE, s);
-        SolveArgs a{};
-        a.slc = slc;
-        a.npx = P.npx;
-        a.n = P.n;
-        a.module = P.module;
-        a.method = P.method;
-        a.ref = P.ref;
-        a.bw = P.bw;
-        a.phase = phase;
-        a.tcorr = tcorr;
-        a.comp = comp;
-        if (P.n <= 64) launch_resolve(P, wts, a, s);
-        else launch_resolve_generic(P, wts, a, s);
+        resolve_tile(P, slc, wts, phase, tcorr, comp, s);
         if (hipGetLastError() != hipSuccess) return -1;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Batch of independent tiles (fringe_link_batch_run): rounds of coherence
+// segments from several tiles in one buffer, one persistent solve launch
+// per date class and round.
+// ---------------------------------------------------------------------------
+int link_batch_launch(const BatchPlan& B, hipStream_t s) {
+    const int count = (int)B.tiles.size();
+    for (int i = 0; i < count; ++i) {
+        const BatchTile& T = B.tiles[i];
+        if (T.P.d_cert_count) hipLaunchKernelGGL(zero_i32, dim3(1), dim3(64), 0, s, T.P.d_cert_count, 1);
+        if (link_uses_mfma(T.P.n, T.P.precision)) {
+            StageScope sc(STAGE_PACK, s);
+            if (pack_launch(T.slc, T.P.rows, T.P.cols, T.P.n, T.P.precision != FRINGE_PREC_MIXED_BF16, T.P.d_pack,
+                            s) != 0)
+                return -1;
+        }
+    }
+    const int mode = solve_mode_index(B.module, B.method);
+    int item = 0;
+    long long row = count ? B.tiles[0].P.row_begin : 0;
+    int rc = 0;
+    while (item < count) {
+        PersistArgs pa[3] = {};
+        size_t off = 0;      // bytes of the round buffer in use
+        long long soff = 0;  // slots in use
+        bool full = false;
+        while (item < count && !full) {
+            const BatchTile& T = B.tiles[item];
+            const LinkPlan& P = T.P;
+            const int cls = solve_class(P.n);
+            const long long rem = P.row_end - row;
+            if (rem <= 0) {
+                ++item;
+                if (item < count) row = B.tiles[item].P.row_begin;
+                continue;
+            }
+            if (pa[cls].njobs == kMaxJobs) break;
+            // bytes per row: coherence matrices, plus the work matrices above 64
+            // dates when they live in global memory, each block 256-aligned
+            const size_t per_row = (size_t)P.cols * (link_gamma_bytes_per_pixel(P.n, P.precision) - sizeof(int) +
+                                                     P.awork_stride);
+            long long rows = (long long)((B.round_bytes > off + 512 ? B.round_bytes - off - 512 : 0) / per_row);
+            rows = std::min(rows, rem);
+            const int th = link_uses_mfma(P.n, P.precision) ? gram_stage_rows(P.n, P.Nx, P.Ny) : 1;
+            if (rows < rem && rows > th) rows -= rows % th;
+            if (rows <= 0) {
+                full = true;
+                break;
+            }
+            const int slots = (int)(rows * P.cols);
+            const long long first = row * P.cols;
+            char* gam = (char*)B.d_round + off;
+            const size_t gbytes = align256((size_t)slots * P.n * P.n * (link_uses_mfma(P.n, P.precision) ? 8 : 16));
+            void* aw = nullptr;
+            if (P.awork_stride) aw = gam + gbytes;
+            int* npix = B.d_npix + soff;
+            {
+                StageScope sc(STAGE_GRAM, s);
+                gram_segment(P, T.slc, T.wts, first, slots, gam, npix, s, &rc);
+                if (rc) return -1;
+            }
+            pa[cls].job[pa[cls].njobs++] = make_job(P, T.slc, T.phase, T.tcorr, T.comp, first, slots, gam, npix, aw,
+                                                    P.awork_stride);
+            off += gbytes + (size_t)slots * P.awork_stride;
+            soff += slots;
+            row += rows;
+        }
+        StageScope sc(STAGE_SOLVE, s);
+        for (int c = 0; c < 3; ++c) {
+            if (!pa[c].njobs) continue;
+            pa[c].module = B.module;
+            pa[c].method = B.method;
+            pa[c].queue = B.d_queue + c;
+            const int sched = schedule_lookup(FRINGE_SCHED_SOLVE, c == 0 ? 32 : (c == 1 ? 64 : 256), mode,
+                                              B.precision == FRINGE_PREC_FP64 ? 0 : 1, 0);
+            if (pa[c].njobs == 1 && sched == 0) {
+                // a lone job of its class: the launch the schedule prefers for one tile
+                const PJob& J = pa[c].job[0];
+                SolveArgs a{};
+                a.gamma = J.gamma;
+                a.npix = J.npix;
+                a.slc = J.slc;
+                a.npx = J.npx;
+                a.first = J.first;
+                a.slots = J.slots;
+                a.n = J.n;
+                a.module = B.module;
+                a.method = B.method;
+                a.ref = J.ref;
+                a.bw = J.bw;
+                a.phase = J.phase;
+                a.tcorr = J.tcorr;
+                a.comp = J.comp;
+                a.cert_list = J.cert_list;
+                a.cert_count = J.cert_count;
+                a.cert_cap = J.cert_cap;
+                a.awork = J.awork;
+                a.awork_stride = J.awork_stride;
+                const bool f64 = B.precision == FRINGE_PREC_FP64;
+                if (c < 2) {
+                    if (f64) dispatch_reg<double, double>(a, s);
+                    else dispatch_reg<float, float>(a, s);
+                } else {
+                    if (f64) dispatch_generic<double, double>(a, s);
+                    else dispatch_generic<float, double>(a, s);
+                }
+            } else {
+                launch_persistent_any(pa[c], c, mode, B.precision, sched, s);
+            }
+        }
+        if (hipGetLastError() != hipSuccess) return -1;
+    }
+    for (int i = 0; i < count; ++i) {
+        const BatchTile& T = B.tiles[i];
+        if (T.P.d_cert_count && T.P.row_end > T.P.row_begin) {
+            LinkPlan Q = T.P;
+            Q.d_rwork = B.d_rwork;
+            resolve_tile(Q, T.slc, T.wts, T.phase, T.tcorr, T.comp, s);
+            if (hipGetLastError() != hipSuccess) return -1;
+        }
     }
     return 0;
 }

@@ -62,6 +62,36 @@ constexpr float kCertScale = 1.0e4f;
 constexpr float kCertThrMargin = 256.0f;
 constexpr double kCertPivotMin = 1.0e-4;
 
+// ---- persistent solve launch over a queue of jobs -------------------------
+// A job is one round segment of one tile: slots [0, slots) of its coherence
+// buffer are pixels first + s of that tile.  One launch drains the work
+// units of up to kMaxJobs jobs, of different date counts and tiles, from an
+// atomic counter (a unit is one wave's share: two slots for n <= 32, one
+// slot otherwise); units are taken last-first, the matrices the Gram wrote
+// last being the ones still cached.
+constexpr int kMaxJobs = 16;
+struct PJob {
+    const void* gamma;
+    const int* npix;
+    const float2* slc;
+    float2* phase;
+    float* tcorr;
+    float2* comp;
+    int* cert_list;
+    int* cert_count;
+    void* awork;        // above 64 dates: work matrices in global memory (or null)
+    size_t awork_stride;
+    long long npx, first;
+    int slots, n, ref, bw, cert_cap;
+    unsigned group_bytes;   // above 64 dates: dynamic LDS of this job's group
+};
+struct PersistArgs {
+    int module, method, njobs, grab;
+    int* queue;                 // 0 at launch
+    int unit_end[kMaxJobs];     // prefix sums of the jobs' units
+    PJob job[kMaxJobs];
+};
+
 template <typename T> struct Lim;
 template <> struct Lim<float> {
     static __device__ constexpr float eps() { return 5.9604645e-8f; }
@@ -715,6 +745,47 @@ __device__ __forceinline__ void solve_generic_pixel(const SolveArgs& a, SolveLds
     }
 }
 
+// Solver arguments of job J for the shared per-pixel bodies.
+__device__ inline SolveArgs job_args(const PersistArgs& pa, const PJob& J) {
+    SolveArgs a{};
+    a.gamma = J.gamma;
+    a.npix = J.npix;
+    a.slc = J.slc;
+    a.npx = J.npx;
+    a.first = J.first;
+    a.slots = J.slots;
+    a.n = J.n;
+    a.module = pa.module;
+    a.method = pa.method;
+    a.ref = J.ref;
+    a.bw = J.bw;
+    a.group_bytes = J.group_bytes;
+    a.phase = J.phase;
+    a.tcorr = J.tcorr;
+    a.comp = J.comp;
+    a.cert_list = J.cert_list;
+    a.cert_count = J.cert_count;
+    a.cert_cap = J.cert_cap;
+    a.awork = J.awork;
+    a.awork_stride = J.awork_stride;
+    return a;
+}
+
+// Next unit range of this workgroup from the queue: [u0, u1) counted from
+// the end; u0 >= total when the queue is drained.
+__device__ inline int take_units(const PersistArgs& pa, int total, int* s_u) {
+    __syncthreads();
+    if (threadIdx.x == 0) *s_u = atomicAdd(pa.queue, pa.grab);
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(*s_u);   // uniform: keeps the job's fields in scalar registers
+}
+
+__device__ inline int job_of_unit(const PersistArgs& pa, int u) {
+    int j = 0;
+    while (j + 1 < pa.njobs && u >= pa.unit_end[j]) ++j;
+    return __builtin_amdgcn_readfirstlane(j);
+}
+
 // Above 64 dates.  LIST = false: one pixel per group, slot s of the round's
 // coherence buffer is pixel first + s; the FP32 instance certifies its
 // results.  LIST = true (T = Tg = double, G = 128): the FP64 re-solve of the
@@ -753,6 +824,32 @@ __global__ __launch_bounds__(G > 64 ? G : 64) void solve_kernel(SolveArgs a) {
             __threadfence();
             __syncthreads();
             solve_generic_pixel<T, Tg, G, R, false>(a, L, true, np, gam, pix);
+            __syncthreads();
+        }
+    }
+}
+
+// Persistent launch above 64 dates: one pixel per two-wave workgroup per
+// unit; each job's dynamic LDS fits the launch's (the largest of its jobs).
+template <typename T, int R>
+__global__ __launch_bounds__(128) void solve_generic_persistent(PersistArgs pa) {
+    __shared__ int s_u;
+    extern __shared__ __align__(16) char smem[];
+    const int total = pa.unit_end[pa.njobs - 1];
+    for (;;) {
+        const int u0 = take_units(pa, total, &s_u);
+        if (u0 >= total) break;
+        const int u1 = min(u0 + pa.grab, total);
+        for (int v = u0; v < u1; ++v) {
+            const int u = total - 1 - v;
+            const int j = job_of_unit(pa, u);
+            const PJob& J = pa.job[j];
+            const SolveArgs a = job_args(pa, J);
+            const int slot = u - (j ? pa.unit_end[j - 1] : 0);
+            cx<T>* aglob = a.awork ? (cx<T>*)((char*)a.awork + a.awork_stride * (size_t)slot) : nullptr;
+            SolveLds<T> L = solve_carve<T>(smem, a.n, aglob);
+            const cx<double>* gam = (const cx<double>*)a.gamma + (size_t)slot * a.n * a.n;
+            solve_generic_pixel<T, double, 128, R, sizeof(T) == 4>(a, L, true, a.npix[slot], gam, a.first + slot);
             __syncthreads();
         }
     }

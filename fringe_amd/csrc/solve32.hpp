@@ -745,7 +745,8 @@ __device__ __forceinline__ void solve_reg_pixel(const SolveArgs& a, LdsR<T, NR>&
 // of the listed pixels; each group computes its pixel's coherence with the
 // FP64 Gram into LDS and solves it there, looping over the list.
 template <typename T, typename Tg, int MODE, int NR, bool LIST>
-__global__ __launch_bounds__(64) void solve_reg_kernel(SolveArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && NR == 32 ? 4 : 1)))
+void solve_reg_kernel(SolveArgs a) {
     __shared__ LdsR<T, NR> sm[64 / NR];
     const int grp = threadIdx.x / NR, r = threadIdx.x % NR;
     LdsR<T, NR>& L = sm[grp];
@@ -776,6 +777,38 @@ __global__ __launch_bounds__(64) void solve_reg_kernel(SolveArgs a) {
             __syncthreads();
             solve_reg_pixel<T, Tg, MODE, NR, false>(a, L, r, valid, np, (const cx<Tg>*)gl, pix);
             __syncthreads();
+        }
+    }
+}
+
+// Persistent launch up to 64 dates: each wave takes units (two pixels for
+// NR = 32, one for NR = 64) of the queued jobs until the queue is drained.
+template <typename T, typename Tg, int MODE, int NR>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && NR == 32 ? 4 : 1)))
+void solve_reg_persistent(PersistArgs pa) {
+    __shared__ LdsR<T, NR> sm[64 / NR];
+    __shared__ int s_u;
+    const int total = pa.unit_end[pa.njobs - 1];
+    for (;;) {
+        const int u0 = take_units(pa, total, &s_u);
+        if (u0 >= total) break;
+        const int u1 = min(u0 + pa.grab, total);
+        for (int v = u0; v < u1; ++v) {
+            // the lane's row, opaque to the compiler on every pixel: values derived
+            // from it are then computed inside the loop instead of being hoisted
+            // out of it and held in registers across the whole estimator chain
+            int r = threadIdx.x % NR, grp = threadIdx.x / NR;
+            asm volatile("" : "+v"(r), "+v"(grp));
+            LdsR<T, NR>& L = sm[grp];
+            const int u = total - 1 - v;
+            const int j = job_of_unit(pa, u);
+            const PJob& J = pa.job[j];
+            const SolveArgs a = job_args(pa, J);
+            const int slot = (u - (j ? pa.unit_end[j - 1] : 0)) * (64 / NR) + grp;
+            const bool inslot = slot < a.slots;
+            const int np = inslot ? a.npix[slot] : -1;
+            const cx<Tg>* gam = (const cx<Tg>*)a.gamma + (size_t)(inslot ? slot : 0) * a.n * a.n;
+            solve_reg_pixel<T, Tg, MODE, NR, sizeof(T) == 4>(a, L, r, inslot, np, gam, a.first + (inslot ? slot : 0));
         }
     }
 }

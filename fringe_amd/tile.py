@@ -97,6 +97,43 @@ class LinkRunner:
         return np.sort(lst.astype(np.int64))
 
 
+class BatchRunner:
+    """Outputs + workspace for fringe_link_batch_run over a list of
+    (slc, wts, params) tiles of different date counts and windows (one
+    module, method and precision); the solves of each date class run in one
+    persistent launch per round."""
+
+    def __init__(self, tiles, device="cuda"):
+        L = _lib.lib()
+        dev = torch.device(device)
+        self.tiles = tiles
+        self.outs = []
+        arr = (_lib.BatchItem * len(tiles))()
+        for i, (slc, wts, p) in enumerate(tiles):
+            assert slc.is_contiguous() and slc.dtype == torch.complex64
+            assert wts.is_contiguous() and wts.dtype == _DTYPE_U32
+            assert tuple(slc.shape) == (p.nbands, p.rows, p.cols)
+            assert wts.numel() == p.rows * p.cols * mask_words(p.Nx, p.Ny)
+            ph = torch.zeros((p.nbands, p.rows, p.cols), dtype=torch.complex64, device=dev)
+            tc = torch.zeros((p.rows, p.cols), dtype=torch.float32, device=dev)
+            co = torch.zeros((p.rows, p.cols), dtype=torch.complex64, device=dev)
+            self.outs.append((ph, tc, co))
+            arr[i].tile = p
+            arr[i].d_slc, arr[i].d_wts = slc.data_ptr(), wts.data_ptr()
+            arr[i].d_phase, arr[i].d_tcorr, arr[i].d_comp = ph.data_ptr(), tc.data_ptr(), co.data_ptr()
+        self.items = arr
+        self.work_bytes = L.fringe_link_batch_workspace_bytes(arr, len(tiles))
+        if self.work_bytes == 0:
+            raise RuntimeError("invalid or unsupported batch (one module, method and precision per batch)")
+        self.work = torch.empty(self.work_bytes, dtype=torch.uint8, device=dev)
+
+    def run(self, stream=None):
+        rc = _lib.lib().fringe_link_batch_run(self.items, len(self.tiles), _ptr(self.work), self.work_bytes,
+                                              _stream(stream))
+        _check(rc, "fringe_link_batch_run")
+        return self.outs
+
+
 def link(slc, wts, Nx, Ny, **kw):
     """Phase-link a device tile.  slc: complex64 [n, rows, cols] on the GPU;
     wts: int32 [rows, cols, nulong] on the GPU.  Returns (phase, tcorr, comp)."""

@@ -158,6 +158,50 @@ int fringe_link_run_noalloc(const fringe_link_tile* t, const void* d_slc, const 
  * diagnostic of the mixed-precision path.  Returns -1 on bad arguments. */
 long long fringe_link_resolved_pixels(const fringe_link_tile* t, const void* d_work, size_t work_bytes);
 
+/* ---- batched tiles: one persistent solve launch over mixed date counts ---
+ * No counterpart in the reference, whose pixel loops run one stack at a time
+ * (evd.cpp:512-514; sequential.py:193-240 links mini-stacks one after the
+ * other).  fringe_link_batch_run links `count` independent tiles (their own
+ * nbands, window, size, min_neighbors, ref_index, bandwidth; one module,
+ * method and precision for the whole batch) on one stream: the coherence
+ * matrices of several tiles share each round's buffer, and every round's
+ * Hermitian solves of one date class (<= 32, <= 64, > 64 dates) run in ONE
+ * persistent launch that drains a queue of all tiles' pixels, whatever their
+ * date count and neighbour count.  Outputs equal fringe_link_run's per tile
+ * bit for bit.  Returns 202 for a batch that mixes modules, methods or
+ * precisions. */
+typedef struct fringe_link_batch_item {
+    fringe_link_tile tile;
+    const void* d_slc;
+    const uint32_t* d_wts;
+    void* d_phase;
+    float* d_tcorr;
+    void* d_comp;
+} fringe_link_batch_item;
+size_t fringe_link_batch_workspace_bytes(const fringe_link_batch_item* items, int count);
+int fringe_link_batch_run(const fringe_link_batch_item* items, int count, void* d_work, size_t work_bytes,
+                          void* stream);
+
+/* ---- launch schedules (set by fringe_amd/autotune.py's table) ------------
+ * No counterpart in the reference.  FRINGE_SCHED_GRAM_VARIANT, keyed by
+ * (nbands, Nx, Ny) (nbands taken by class: <= 32, <= 64): the mixed path's
+ * Gram kernel, FRINGE_GRAM_*; a variant that does not fit the window falls
+ * back to the default choice.  FRINGE_SCHED_SOLVE, keyed by (nbands class,
+ * estimator: 0 phase_link EMI, 1 evd EVD/STBAS, 2 evd MLE, precision): 0
+ * launches one wave per unit of work, else (waves_per_cu << 8 | grab) runs a
+ * persistent grid of waves_per_cu waves per compute unit that take `grab`
+ * units at a time from a queue.  Unset keys keep the built-in defaults. */
+#define FRINGE_SCHED_GRAM_VARIANT 0
+#define FRINGE_SCHED_SOLVE 1
+#define FRINGE_GRAM_DEFAULT 0
+#define FRINGE_GRAM_STAGED12 1      /* 12 waves, 8 x 16-pixel tiles staged in LDS */
+#define FRINGE_GRAM_STAGED8 2       /* 8 waves, 4 x 16 */
+#define FRINGE_GRAM_STAGED4 3       /* 4 waves, up to 4 x 8 */
+#define FRINGE_GRAM_UNSTAGED 4      /* one wave per pixel, records gathered from L2 */
+int fringe_schedule_set(int kind, int key0, int key1, int key2, int value);
+int fringe_schedule_get(int kind, int key0, int key1, int key2);
+void fringe_schedule_clear(void);
+
 /* Coherence matrices alone (the first stage of fringe_link_run,
  * evd.cpp:530-582): for every output pixel, slot = (row - row_begin) * cols
  * + col, d_gamma[slot][i][j] = Gamma_ij (row-major, Hermitian, unit

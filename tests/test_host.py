@@ -343,3 +343,29 @@ def test_calamp_module_errors_before_gpu_work(tmp_path):
     raster.write_envi(mf, np.ones((rows, cols + 1), np.uint8))
     c.maskDS = mf
     assert c.run() == 104
+
+
+def test_schedule_table_loads():
+    """The committed autotuner table (fringe_amd/schedules.json) names valid
+    keys and values, and loading it installs them in the library's table."""
+    import json
+    from fringe_amd import _lib
+    L = _lib.lib()
+    with open(_lib.SCHEDULE_FILE) as f:
+        t = json.load(f)
+    assert t["gram"] and t["solve"]
+    for e in t["gram"]:
+        assert e["variant"] in _lib.GRAM_VARIANTS and e["variant"] == min(e["ms"], key=e["ms"].get)
+    for e in t["solve"]:
+        assert e["launch"] in ("grid", "persistent") and e["mode"] in (0, 1, 2) and e["precision"] in (0, 1)
+    L.fringe_schedule_clear()
+    assert _lib.load_schedules(L) == len(t["gram"]) + len(t["solve"])
+    for e in t["gram"]:
+        assert L.fringe_schedule_get(_lib.FRINGE_SCHED_GRAM_VARIANT, e["n"], e["Nx"], e["Ny"]) == \
+            _lib.GRAM_VARIANTS[e["variant"]]
+    for e in t["solve"]:
+        want = 0 if e["launch"] == "grid" else (e["waves_per_cu"] << 8) | e["grab"]
+        assert L.fringe_schedule_get(_lib.FRINGE_SCHED_SOLVE, e["n"], e["mode"], e["precision"]) == want
+    assert L.fringe_schedule_set(_lib.FRINGE_SCHED_GRAM_VARIANT, 30, 7, 5, 9) == 201
+    assert L.fringe_schedule_set(7, 30, 7, 5, 0) == 201
+

@@ -709,6 +709,13 @@ def main():
         g_ms, g_calls = per["gram"]
         rl["gram"] = _roof("gram_mfma", "mfma", gram_flops(n, keff) / (g_ms / 1e3) / 1e12, PEAK_F16_MFMA,
                            "TFLOP/s", g_ms, g_calls, "8*K_eff*N^2 flops per linked pixel (hi/lo passes not counted)")
+        try:   # rocprofv3 MFMA counters of this workload (profiles/pmc_mfma_gram_*.md)
+            with open(os.path.join(PROFILE_DIR, "pmc_mfma.json")) as f:
+                pm = json.load(f)
+            rl["gram"]["rocprof_mfma_busy_frac"] = pm["mfma_busy_frac"]
+            rl["gram"]["rocprof_mfma_executed_frac_of_peak"] = pm["frac_of_f16_peak_executed"]
+        except (OSError, ValueError, KeyError):
+            pass
     if "solve" in per:
         s_ms, s_calls = per["solve"]
         rl["solve"] = _roof("solve_reg_kernel", "valu", solve_flops(n) * linked / (s_ms / 1e3) / 1e12, PEAK_F32_VALU,

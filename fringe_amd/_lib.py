@@ -84,7 +84,7 @@ class BatchItem(ctypes.Structure):
                 ("d_phase", ctypes.c_void_p), ("d_tcorr", ctypes.c_void_p), ("d_comp", ctypes.c_void_p)]
 
 
-FRINGE_SCHED_GRAM_VARIANT, FRINGE_SCHED_SOLVE = 0, 1
+FRINGE_SCHED_GRAM_VARIANT, FRINGE_SCHED_SOLVE, FRINGE_SCHED_GENERIC_A = 0, 1, 2
 GRAM_VARIANTS = {"default": 0, "staged12": 1, "staged8": 2, "staged4": 3, "unstaged": 4}
 SCHEDULE_FILE = os.path.join(_HERE, "schedules.json")
 
@@ -107,6 +107,10 @@ def load_schedules(L, path=None):
     for e in table.get("solve", []):
         v = 0 if e["launch"] == "grid" else (int(e["waves_per_cu"]) << 8) | int(e["grab"])
         if L.fringe_schedule_set(FRINGE_SCHED_SOLVE, e["n"], e["mode"], e["precision"], v) == 0:
+            k += 1
+    for e in table.get("generic_a", []):
+        v = 1 if e["work_matrix"] == "global" else 0
+        if L.fringe_schedule_set(FRINGE_SCHED_GENERIC_A, e["n"], e["precision"], 0, v) == 0:
             k += 1
     return k
 

@@ -11,6 +11,8 @@ stage's launches), and keeps the fastest:
 * Solver launch, per (date class, estimator, precision): one wave per unit,
   or a persistent grid of W waves per compute unit taking G units at a time
   from the work queue.
+* Above 64 dates, per (date class, precision): the solver's work matrix in
+  LDS (where it fits) or in global memory (more pixels in flight).
 
 The chosen table (with every candidate's time) is written as JSON; the
 library installs it when it is loaded (fringe_amd._lib.load_schedules).
@@ -116,9 +118,28 @@ def tune(quick=False, log=sys.stderr):
                       "launch": launch, "waves_per_cu": int(wv), "grab": int(g), "ms": times})
         print(f"solve n={n} {module}/{method} {prec}: {best} ({times[best]:.2f} ms, grid {times['grid:0:0']:.2f})",
               file=log, flush=True)
+    # ---- work matrix above 64 dates: LDS (where it fits) or global memory --
+    generic_a = []
+    for n, prec in ((72, "mixed"), (96, "mixed"), (72, "fp64"), (128, "mixed")):
+        if quick and n != 96:
+            continue
+        slc, w = data(n, 7, 5, fill=0.9)
+        slc, w = slc[:, :16].contiguous(), w[:16].contiguous()
+        p = tile.link_params(16, cols, n, 7, 5, min_neighbors=5, precision=prec)
+        pk = 0 if prec == "fp64" else 1
+        times = {}
+        for place, v in (("lds", 0), ("global", 1)):
+            L.fringe_schedule_set(_lib.FRINGE_SCHED_GENERIC_A, n, pk, 0, v)
+            r = tile.LinkRunner(p, device=dev)
+            times[place] = _time_stage(L, lambda: r.run(slc, w), "solve", reps)
+            del r
+        L.fringe_schedule_set(_lib.FRINGE_SCHED_GENERIC_A, n, pk, 0, 0)
+        best = min(times, key=times.get)
+        generic_a.append({"n": n, "precision": pk, "work_matrix": best, "ms": times})
+        print(f"work matrix n={n} {prec}: {best} {times}", file=log, flush=True)
     return {"device": torch.cuda.get_device_name(0), "tile": f"{rows} x {cols}",
             "note": "stage times in ms per call (median); the library installs the chosen entries",
-            "gram": gram, "solve": solve}
+            "gram": gram, "solve": solve, "generic_a": generic_a}
 
 
 def main(argv=None):

@@ -76,7 +76,10 @@ size_t solve_static_lds(int precision) {
 }
 
 // Whether the work matrix of the solver above 64 dates fits in LDS.
+// The schedule FRINGE_SCHED_GENERIC_A may move it to global memory even
+// when it fits: the pixels in flight are then limited by registers, not LDS.
 bool solve_a_in_lds(int n, int precision) {
+    if (schedule_lookup(FRINGE_SCHED_GENERIC_A, n, precision == FRINGE_PREC_FP64 ? 0 : 1, 0, 0) == 1) return false;
     const size_t b = precision == FRINGE_PREC_FP64 ? solve_group_bytes<double>(n) : solve_group_bytes<float>(n);
     return b + solve_static_lds(precision) <= (size_t)device_lds_limit();
 }

@@ -8,6 +8,8 @@
 //   Solver launch, keyed by (date class, estimator, precision): a grid with
 //   one wave per unit, or a persistent grid of `waves per CU` x CUs waves
 //   that take `grab` units at a time from a queue.
+//   Work matrix above 64 dates, keyed by (date class 96 / 128 / 256,
+//   precision): in LDS where it fits (0), or in global memory (1).
 //
 // Unset keys use the built-in defaults (the heuristics the table replaces).
 #include <map>
@@ -26,6 +28,7 @@ int date_class(int n) { return n <= 32 ? 32 : (n <= 64 ? 64 : 256); }
 
 std::tuple<int, int, int, int> key_of(int kind, int k0, int k1, int k2) {
     if (kind == FRINGE_SCHED_GRAM_VARIANT || kind == FRINGE_SCHED_SOLVE) k0 = date_class(k0);
+    if (kind == FRINGE_SCHED_GENERIC_A) k0 = k0 <= 96 ? 96 : (k0 <= 128 ? 128 : 256);
     return std::make_tuple(kind, k0, k1, k2);
 }
 
@@ -49,6 +52,8 @@ int fringe_schedule_set(int kind, int key0, int key1, int key2, int value) {
     } else if (kind == FRINGE_SCHED_SOLVE) {
         const int waves = value >> 8, grab = value & 0xff;
         if (value != 0 && (waves < 1 || waves > 64 || grab < 1)) return FRINGE_ERR_ARGUMENT;
+    } else if (kind == FRINGE_SCHED_GENERIC_A) {
+        if (value != 0 && value != 1) return FRINGE_ERR_ARGUMENT;
     } else {
         return FRINGE_ERR_ARGUMENT;
     }
@@ -61,7 +66,7 @@ int fringe_schedule_set(int kind, int key0, int key1, int key2, int value) {
 }
 
 int fringe_schedule_get(int kind, int key0, int key1, int key2) {
-    if (kind != FRINGE_SCHED_GRAM_VARIANT && kind != FRINGE_SCHED_SOLVE) return -1;
+    if (kind != FRINGE_SCHED_GRAM_VARIANT && kind != FRINGE_SCHED_SOLVE && kind != FRINGE_SCHED_GENERIC_A) return -1;
     return schedule_lookup(kind, key0, key1, key2, kind == FRINGE_SCHED_SOLVE ? 0 : FRINGE_GRAM_DEFAULT);
 }
 

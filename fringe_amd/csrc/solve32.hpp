@@ -745,7 +745,8 @@ __device__ __forceinline__ void solve_reg_pixel(const SolveArgs& a, LdsR<T, NR>&
 // of the listed pixels; each group computes its pixel's coherence with the
 // FP64 Gram into LDS and solves it there, looping over the list.
 template <typename T, typename Tg, int MODE, int NR, bool LIST>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && NR == 32 ? 4 : 1)))
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && NR == 32 && MODE == MODE_MLE ? 4 : 1)))
 void solve_reg_kernel(SolveArgs a) {
     __shared__ LdsR<T, NR> sm[64 / NR];
     const int grp = threadIdx.x / NR, r = threadIdx.x % NR;

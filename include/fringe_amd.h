@@ -193,6 +193,7 @@ int fringe_link_batch_run(const fringe_link_batch_item* items, int count, void* 
  * units at a time from a queue.  Unset keys keep the built-in defaults. */
 #define FRINGE_SCHED_GRAM_VARIANT 0
 #define FRINGE_SCHED_SOLVE 1
+#define FRINGE_SCHED_GENERIC_A 2    /* (nbands class 96/128/256, precision): 1 = solver work matrix in global memory */
 #define FRINGE_GRAM_DEFAULT 0
 #define FRINGE_GRAM_STAGED12 1      /* 12 waves, 8 x 16-pixel tiles staged in LDS */
 #define FRINGE_GRAM_STAGED8 2       /* 8 waves, 4 x 16 */

@@ -131,3 +131,28 @@ def test_gram_variants_agree(Nx, Ny):
         assert np.array_equal(npx, n0), name
         linked = n0 >= 0
         assert np.abs(g[linked] - g0[linked]).max() < 1e-6, name
+
+
+@pytest.mark.parametrize("n,precision", [(80, "mixed"), (96, "mixed"), (72, "fp64")])
+def test_work_matrix_placement_agrees(n, precision):
+    """Above 64 dates the solver's work matrix in LDS or in global memory (the
+    FRINGE_SCHED_GENERIC_A schedule) gives the same outputs bit for bit."""
+    L = _lib.lib()
+    rows, cols, Nx, Ny = 12, 24, 6, 5
+    print(f"case work matrix placement n={n} {precision}", flush=True)
+    slc = synthetic.stack(n, rows, cols, seed=640 + n)
+    ds = to_dev(slc, torch)
+    w = synthetic.random_mask_torch(rows, cols, Nx, Ny, 0.9, 740 + n, device="cuda")
+    p = tile.link_params(rows, cols, n, Nx, Ny, min_neighbors=5, precision=precision)
+    pk = 0 if precision == "fp64" else 1
+    old = L.fringe_schedule_get(_lib.FRINGE_SCHED_GENERIC_A, n, pk, 0)
+    out = []
+    try:
+        for v in (0, 1):
+            assert L.fringe_schedule_set(_lib.FRINGE_SCHED_GENERIC_A, n, pk, 0, v) == 0
+            out.append(tuple(x.clone() for x in tile.LinkRunner(p).run(ds, w)))
+            torch.cuda.synchronize()
+    finally:
+        L.fringe_schedule_set(_lib.FRINGE_SCHED_GENERIC_A, n, pk, 0, old)
+    assert (out[0][1] != 0).sum() > 0 and _same(out[0], out[1])
+

@@ -359,7 +359,10 @@ def test_schedule_table_loads():
     for e in t["solve"]:
         assert e["launch"] in ("grid", "persistent") and e["mode"] in (0, 1, 2) and e["precision"] in (0, 1)
     L.fringe_schedule_clear()
-    assert _lib.load_schedules(L) == len(t["gram"]) + len(t["solve"])
+    assert _lib.load_schedules(L) == len(t["gram"]) + len(t["solve"]) + len(t.get("generic_a", []))
+    for e in t.get("generic_a", []):
+        assert L.fringe_schedule_get(_lib.FRINGE_SCHED_GENERIC_A, e["n"], e["precision"], 0) == \
+            (1 if e["work_matrix"] == "global" else 0)
     for e in t["gram"]:
         assert L.fringe_schedule_get(_lib.FRINGE_SCHED_GRAM_VARIANT, e["n"], e["Nx"], e["Ny"]) == \
             _lib.GRAM_VARIANTS[e["variant"]]

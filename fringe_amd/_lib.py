@@ -86,6 +86,9 @@ class BatchItem(ctypes.Structure):
 
 FRINGE_SCHED_GRAM_VARIANT, FRINGE_SCHED_SOLVE, FRINGE_SCHED_GENERIC_A = 0, 1, 2
 GRAM_VARIANTS = {"default": 0, "staged12": 1, "staged8": 2, "staged4": 3, "unstaged": 4}
+# FRINGE_SCHED_GENERIC_A values: the solver's work matrix above 64 dates in
+# LDS or in global memory, or (mixed precisions, <= 96 dates) rows in registers
+GENERIC_A_PLACES = {"lds": 0, "global": 1, "registers": 2}
 SCHEDULE_FILE = os.path.join(_HERE, "schedules.json")
 
 
@@ -109,7 +112,7 @@ def load_schedules(L, path=None):
         if L.fringe_schedule_set(FRINGE_SCHED_SOLVE, e["n"], e["mode"], e["precision"], v) == 0:
             k += 1
     for e in table.get("generic_a", []):
-        v = 1 if e["work_matrix"] == "global" else 0
+        v = GENERIC_A_PLACES[e["work_matrix"]]
         if L.fringe_schedule_set(FRINGE_SCHED_GENERIC_A, e["n"], e["precision"], 0, v) == 0:
             k += 1
     return k

@@ -12,7 +12,8 @@ stage's launches), and keeps the fastest:
   or a persistent grid of W waves per compute unit taking G units at a time
   from the work queue.
 * Above 64 dates, per (date class, precision): the solver's work matrix in
-  LDS (where it fits) or in global memory (more pixels in flight).
+  LDS (where it fits) or in global memory (more pixels in flight), or, for
+  the mixed precisions up to 96 dates, each row in a lane's registers.
 
 The chosen table (with every candidate's time) is written as JSON; the
 library installs it when it is loaded (fringe_amd._lib.load_schedules).
@@ -96,6 +97,7 @@ def tune(quick=False, log=sys.stderr):
              (48, 0, "mixed"), (64, 1, "mixed"), (96, 0, "mixed")]
     if quick:
         cases = cases[:3]
+    generic_a_default = {pk: L.fringe_schedule_get(_lib.FRINGE_SCHED_GENERIC_A, 96, pk, 0) for pk in (0, 1)}
     for n, mode, prec in cases:
         module, method = MODES[mode]
         Nx, Ny = (10, 5) if n == 64 else (7, 5)
@@ -105,6 +107,8 @@ def tune(quick=False, log=sys.stderr):
         if n > 64:
             slc, w = slc[:, :16].contiguous(), w[:16].contiguous()
         key = (n, mode, 0 if prec == "fp64" else 1)
+        if n > 64:   # the launch schedule serves the solver with a work matrix
+            L.fringe_schedule_set(_lib.FRINGE_SCHED_GENERIC_A, n, key[2], 0, 0)
         times = {}
         for launch, wv, g in cands:
             L.fringe_schedule_set(_lib.FRINGE_SCHED_SOLVE, *key, 0 if launch == "grid" else (wv << 8) | g)
@@ -112,13 +116,15 @@ def tune(quick=False, log=sys.stderr):
             times[f"{launch}:{wv}:{g}"] = _time_stage(L, lambda: r.run(slc, w), "solve", reps)
             del r
         L.fringe_schedule_set(_lib.FRINGE_SCHED_SOLVE, *key, 0)
+        if n > 64:
+            L.fringe_schedule_set(_lib.FRINGE_SCHED_GENERIC_A, n, key[2], 0, generic_a_default[key[2]])
         best = min(times, key=times.get)
         launch, wv, g = best.split(":")
         solve.append({"n": n, "mode": mode, "precision": key[2], "estimator": f"{module}/{method}",
                       "launch": launch, "waves_per_cu": int(wv), "grab": int(g), "ms": times})
         print(f"solve n={n} {module}/{method} {prec}: {best} ({times[best]:.2f} ms, grid {times['grid:0:0']:.2f})",
               file=log, flush=True)
-    # ---- work matrix above 64 dates: LDS (where it fits) or global memory --
+    # ---- above 64 dates: work matrix in LDS or global memory, or registers --
     generic_a = []
     for n, prec in ((72, "mixed"), (96, "mixed"), (72, "fp64"), (128, "mixed")):
         if quick and n != 96:
@@ -127,13 +133,16 @@ def tune(quick=False, log=sys.stderr):
         slc, w = slc[:, :16].contiguous(), w[:16].contiguous()
         p = tile.link_params(16, cols, n, 7, 5, min_neighbors=5, precision=prec)
         pk = 0 if prec == "fp64" else 1
+        default = L.fringe_schedule_get(_lib.FRINGE_SCHED_GENERIC_A, n, pk, 0)
         times = {}
-        for place, v in (("lds", 0), ("global", 1)):
+        places = ["lds", "global"] + (["registers"] if n <= 96 and prec != "fp64" else [])
+        for place in places:
+            v = _lib.GENERIC_A_PLACES[place]
             L.fringe_schedule_set(_lib.FRINGE_SCHED_GENERIC_A, n, pk, 0, v)
             r = tile.LinkRunner(p, device=dev)
             times[place] = _time_stage(L, lambda: r.run(slc, w), "solve", reps)
             del r
-        L.fringe_schedule_set(_lib.FRINGE_SCHED_GENERIC_A, n, pk, 0, 0)
+        L.fringe_schedule_set(_lib.FRINGE_SCHED_GENERIC_A, n, pk, 0, default)
         best = min(times, key=times.get)
         generic_a.append({"n": n, "precision": pk, "work_matrix": best, "ms": times})
         print(f"work matrix n={n} {prec}: {best} {times}", file=log, flush=True)

@@ -19,6 +19,7 @@
 #include "engine_internal.hpp"
 #include "link_solve.hpp"
 #include "solve32.hpp"
+#include "solve96.hpp"
 #include "timing.hpp"
 
 namespace fringe {
@@ -60,8 +61,28 @@ static void launch_solve_generic(SolveArgs a, hipStream_t s) {
                        a);
 }
 
+// Mixed precisions, 64 < n <= 96: the register-resident two-wave solver,
+// unless the schedule FRINGE_SCHED_GENERIC_A places a work matrix instead.
+bool solve96_selected(int n, int precision) {
+    if (precision == FRINGE_PREC_FP64 || n <= 64 || n > kC96) return false;
+    return schedule_lookup(FRINGE_SCHED_GENERIC_A, n, 1, 0, generic_a_default(n, 1)) == 2;
+}
+
+static void launch_solve96(const SolveArgs& a, hipStream_t s) {
+    const dim3 grid((unsigned)a.slots), blk(128);
+    if (a.module == FRINGE_MODULE_PHASE_LINK) hipLaunchKernelGGL(solve96_kernel<MODE_EMI>, grid, blk, 0, s, a);
+    else if (a.method == FRINGE_METHOD_MLE) hipLaunchKernelGGL(solve96_kernel<MODE_MLE>, grid, blk, 0, s, a);
+    else hipLaunchKernelGGL(solve96_kernel<MODE_EVD>, grid, blk, 0, s, a);
+}
+
 template <typename T, typename Tg>
 static void dispatch_generic(const SolveArgs& a, hipStream_t s) {
+    if constexpr (sizeof(T) == 4) {
+        if (!a.awork && solve96_selected(a.n, FRINGE_PREC_MIXED)) {
+            launch_solve96(a, s);
+            return;
+        }
+    }
     if (a.n <= 128) launch_solve_generic<T, Tg, 1>(a, s);
     else launch_solve_generic<T, Tg, 2>(a, s);
 }
@@ -447,7 +468,7 @@ int link_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, float
             if (rc) return -1;
         }
         StageScope sc(STAGE_SOLVE, s);
-        if (sched && P.d_queue) {
+        if (sched && P.d_queue && !solve96_selected(P.n, P.precision)) {
             PersistArgs pa{};
             pa.module = P.module;
             pa.method = P.method;
@@ -499,6 +520,30 @@ int link_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, float
 // segments from several tiles in one buffer, one persistent solve launch
 // per date class and round.
 // ---------------------------------------------------------------------------
+static SolveArgs solve_args_of(const PJob& J, int module, int method) {
+    SolveArgs a{};
+    a.gamma = J.gamma;
+    a.npix = J.npix;
+    a.slc = J.slc;
+    a.npx = J.npx;
+    a.first = J.first;
+    a.slots = J.slots;
+    a.n = J.n;
+    a.module = module;
+    a.method = method;
+    a.ref = J.ref;
+    a.bw = J.bw;
+    a.phase = J.phase;
+    a.tcorr = J.tcorr;
+    a.comp = J.comp;
+    a.cert_list = J.cert_list;
+    a.cert_count = J.cert_count;
+    a.cert_cap = J.cert_cap;
+    a.awork = J.awork;
+    a.awork_stride = J.awork_stride;
+    return a;
+}
+
 int link_batch_launch(const BatchPlan& B, hipStream_t s) {
     const int count = (int)B.tiles.size();
     for (int i = 0; i < count; ++i) {
@@ -516,14 +561,16 @@ int link_batch_launch(const BatchPlan& B, hipStream_t s) {
     long long row = count ? B.tiles[0].P.row_begin : 0;
     int rc = 0;
     while (item < count) {
-        PersistArgs pa[3] = {};
+        // classes 0..2 by date count (solve_class); 3: the register-row
+        // solver's tiles, one grid launch each
+        PersistArgs pa[4] = {};
         size_t off = 0;      // bytes of the round buffer in use
         long long soff = 0;  // slots in use
         bool full = false;
         while (item < count && !full) {
             const BatchTile& T = B.tiles[item];
             const LinkPlan& P = T.P;
-            const int cls = solve_class(P.n);
+            const int cls = solve96_selected(P.n, P.precision) ? 3 : solve_class(P.n);
             const long long rem = P.row_end - row;
             if (rem <= 0) {
                 ++item;
@@ -562,6 +609,7 @@ int link_batch_launch(const BatchPlan& B, hipStream_t s) {
             row += rows;
         }
         StageScope sc(STAGE_SOLVE, s);
+        for (int j = 0; j < pa[3].njobs; ++j) launch_solve96(solve_args_of(pa[3].job[j], B.module, B.method), s);
         for (int c = 0; c < 3; ++c) {
             if (!pa[c].njobs) continue;
             pa[c].module = B.module;
@@ -571,27 +619,7 @@ int link_batch_launch(const BatchPlan& B, hipStream_t s) {
                                               B.precision == FRINGE_PREC_FP64 ? 0 : 1, 0);
             if (pa[c].njobs == 1 && sched == 0) {
                 // a lone job of its class: the launch the schedule prefers for one tile
-                const PJob& J = pa[c].job[0];
-                SolveArgs a{};
-                a.gamma = J.gamma;
-                a.npix = J.npix;
-                a.slc = J.slc;
-                a.npx = J.npx;
-                a.first = J.first;
-                a.slots = J.slots;
-                a.n = J.n;
-                a.module = B.module;
-                a.method = B.method;
-                a.ref = J.ref;
-                a.bw = J.bw;
-                a.phase = J.phase;
-                a.tcorr = J.tcorr;
-                a.comp = J.comp;
-                a.cert_list = J.cert_list;
-                a.cert_count = J.cert_count;
-                a.cert_cap = J.cert_cap;
-                a.awork = J.awork;
-                a.awork_stride = J.awork_stride;
+                const SolveArgs a = solve_args_of(pa[c].job[0], B.module, B.method);
                 const bool f64 = B.precision == FRINGE_PREC_FP64;
                 if (c < 2) {
                     if (f64) dispatch_reg<double, double>(a, s);

@@ -9,7 +9,9 @@
 //   one wave per unit, or a persistent grid of `waves per CU` x CUs waves
 //   that take `grab` units at a time from a queue.
 //   Work matrix above 64 dates, keyed by (date class 96 / 128 / 256,
-//   precision): in LDS where it fits (0), or in global memory (1).
+//   precision): in LDS where it fits (0), in global memory (1), or, for the
+//   mixed precisions up to 96 dates, no work matrix at all: the two-wave
+//   solver that keeps each row in registers (2).
 //
 // Unset keys use the built-in defaults (the heuristics the table replaces).
 #include <map>
@@ -34,6 +36,10 @@ std::tuple<int, int, int, int> key_of(int kind, int k0, int k1, int k2) {
 
 }  // namespace
 
+int generic_a_default(int n, int precision_key) {
+    return (n > 64 && n <= 96 && precision_key != 0) ? kGenericA96Default : 0;
+}
+
 int schedule_lookup(int kind, int k0, int k1, int k2, int fallback) {
     std::lock_guard<std::mutex> lk(g_mu);
     const auto it = g_table.find(key_of(kind, k0, k1, k2));
@@ -53,7 +59,8 @@ int fringe_schedule_set(int kind, int key0, int key1, int key2, int value) {
         const int waves = value >> 8, grab = value & 0xff;
         if (value != 0 && (waves < 1 || waves > 64 || grab < 1)) return FRINGE_ERR_ARGUMENT;
     } else if (kind == FRINGE_SCHED_GENERIC_A) {
-        if (value != 0 && value != 1) return FRINGE_ERR_ARGUMENT;
+        if (value < 0 || value > 2) return FRINGE_ERR_ARGUMENT;
+        if (value == 2 && (key0 > 96 || key1 == 0)) return FRINGE_ERR_ARGUMENT;   // FP32 rows, <= 96 columns
     } else {
         return FRINGE_ERR_ARGUMENT;
     }
@@ -67,6 +74,7 @@ int fringe_schedule_set(int kind, int key0, int key1, int key2, int value) {
 
 int fringe_schedule_get(int kind, int key0, int key1, int key2) {
     if (kind != FRINGE_SCHED_GRAM_VARIANT && kind != FRINGE_SCHED_SOLVE && kind != FRINGE_SCHED_GENERIC_A) return -1;
+    if (kind == FRINGE_SCHED_GENERIC_A) return schedule_lookup(kind, key0, key1, key2, generic_a_default(key0, key1));
     return schedule_lookup(kind, key0, key1, key2, kind == FRINGE_SCHED_SOLVE ? 0 : FRINGE_GRAM_DEFAULT);
 }
 

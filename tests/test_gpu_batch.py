@@ -1,7 +1,8 @@
 """Batched tiles and persistent solve launches: one persistent launch per
 date class drains the pixels of several tiles of different date counts,
-windows and SHP fills; the outputs equal each tile's own fringe_link_run bit
-for bit.  Also: a single tile under a persistent solver schedule equals the
+windows and SHP fills (tiles on the register-row solver above 64 dates get a
+grid launch each); the outputs equal each tile's own fringe_link_run bit for
+bit.  Also: a single tile under a persistent solver schedule equals the
 grid launch, and every Gram variant the schedule table can force gives the
 same coherence matrices."""
 import numpy as np
@@ -49,19 +50,40 @@ SPECS = [(10, 24, 40, 5, 3, None), (24, 20, 36, 7, 5, 0.4), (30, 18, 44, 7, 5, N
          (40, 14, 32, 5, 4, None), (64, 12, 26, 6, 4, 0.7), (80, 12, 22, 7, 5, None)]
 
 
-@pytest.mark.parametrize("precision,module,method", [("mixed", "phase_link", "MLE"), ("fp64", "phase_link", "MLE"),
-                                                    ("mixed", "evd", "MLE"), ("mixed", "evd", "EVD")])
-def test_batch_equals_per_tile_runs(precision, module, method):
-    print(f"case batch {len(SPECS)} tiles {module}/{method} {precision}", flush=True)
+class _Sched:
+    """Sets one schedule key for the duration of a block, restoring it."""
+
+    def __init__(self, kind, key, value):
+        self.L, self.kind, self.key, self.value = _lib.lib(), kind, key, value
+
+    def __enter__(self):
+        self.old = self.L.fringe_schedule_get(self.kind, *self.key)
+        if self.value is not None:
+            assert self.L.fringe_schedule_set(self.kind, *self.key, self.value) == 0
+
+    def __exit__(self, *exc):
+        self.L.fringe_schedule_set(self.kind, *self.key, self.old)
+
+
+@pytest.mark.parametrize("precision,module,method,place", [
+    ("mixed", "phase_link", "MLE", None), ("fp64", "phase_link", "MLE", None), ("mixed", "evd", "MLE", None),
+    ("mixed", "evd", "EVD", None), ("mixed", "phase_link", "MLE", "lds")])
+def test_batch_equals_per_tile_runs(precision, module, method, place):
+    """place: the 80-date tile's solver (FRINGE_SCHED_GENERIC_A) -- rows in
+    registers by default in the mixed precisions, one grid launch per tile;
+    with the LDS work matrix it joins the persistent launch of its class."""
+    print(f"case batch {len(SPECS)} tiles {module}/{method} {precision} {place}", flush=True)
     tiles = _tiles(SPECS, module, method, precision)
-    ref = []
-    for ds, w, p in tiles:
-        r = tile.LinkRunner(p)
-        ref.append(tuple(x.clone() for x in r.run(ds, w)))
-    torch.cuda.synchronize()
-    b = tile.BatchRunner(tiles)
-    got = b.run()
-    torch.cuda.synchronize()
+    v = None if place is None else _lib.GENERIC_A_PLACES[place]
+    with _Sched(_lib.FRINGE_SCHED_GENERIC_A, (80, 0 if precision == "fp64" else 1, 0), v):
+        ref = []
+        for ds, w, p in tiles:
+            r = tile.LinkRunner(p)
+            ref.append(tuple(x.clone() for x in r.run(ds, w)))
+        torch.cuda.synchronize()
+        b = tile.BatchRunner(tiles)
+        got = b.run()
+        torch.cuda.synchronize()
     for k, (r, g) in enumerate(zip(ref, got)):
         assert (r[1] != 0).sum() > 0
         assert _same(r, g), f"tile {k}"
@@ -96,6 +118,9 @@ def test_persistent_schedule_equals_grid(n, precision):
     p = tile.link_params(rows, cols, n, Nx, Ny, min_neighbors=5, precision=precision)
     key = (n, 0, 0 if precision == "fp64" else 1)
     old = L.fringe_schedule_get(_lib.FRINGE_SCHED_SOLVE, *key)
+    # above 64 dates the persistent launch serves the LDS work matrix solver
+    place = _Sched(_lib.FRINGE_SCHED_GENERIC_A, (n, key[2], 0), 0 if n > 64 else None)
+    place.__enter__()
     try:
         assert L.fringe_schedule_set(_lib.FRINGE_SCHED_SOLVE, *key, 0) == 0
         grid = tuple(x.clone() for x in tile.LinkRunner(p).run(ds, w))
@@ -106,6 +131,7 @@ def test_persistent_schedule_equals_grid(n, precision):
             assert _same(grid, pers)
     finally:
         L.fringe_schedule_set(_lib.FRINGE_SCHED_SOLVE, *key, old)
+        place.__exit__()
 
 
 @pytest.mark.parametrize("Nx,Ny", [(7, 5), (10, 5), (3, 2)])

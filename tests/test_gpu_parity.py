@@ -35,6 +35,26 @@ def _run_both(slc, wts, Nx, Ny, module, method, precision, min_nb=2, ref=0, bw=-
     return g, r
 
 
+class _GenericPlace:
+    """Sets FRINGE_SCHED_GENERIC_A (the solver's work matrix above 64 dates:
+    LDS, global memory, or rows in registers) for one key, restoring it."""
+
+    def __init__(self, n, precision, place):
+        from fringe_amd import _lib
+        self.L, self.key = _lib.lib(), (n, 0 if precision == "fp64" else 1, 0)
+        self.v = None if place is None else _lib.GENERIC_A_PLACES[place]
+
+    def __enter__(self):
+        from fringe_amd import _lib
+        self.old = self.L.fringe_schedule_get(_lib.FRINGE_SCHED_GENERIC_A, *self.key)
+        if self.v is not None:
+            assert self.L.fringe_schedule_set(_lib.FRINGE_SCHED_GENERIC_A, *self.key, self.v) == 0
+
+    def __exit__(self, *exc):
+        from fringe_amd import _lib
+        self.L.fringe_schedule_set(_lib.FRINGE_SCHED_GENERIC_A, *self.key, self.old)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("module,method", [("phase_link", "MLE"), ("evd", "EVD"), ("evd", "MLE")])
 def test_link_full_mask_small(precision, module, method):
@@ -144,6 +164,22 @@ def test_link_64row_solver_stbas_reference(precision):
     assert np.all(g[0][3][r["tcorr"] > 0] == 1 + 0j)
 
 
+@pytest.mark.parametrize("precision,place", [("fp64", None), ("mixed", "lds"), ("mixed", "registers")])
+def test_link_generic_solver_stbas_reference(precision, place):
+    n, rows, cols, Nx, Ny = 80, 14, 16, 7, 5
+    print(f"case generic stbas n={n} {precision} {place}", flush=True)
+    slc = synthetic.stack(n, rows, cols, seed=380)
+    wts = synthetic.full_mask(rows, cols, Nx, Ny)
+    with _GenericPlace(n, precision, place):
+        g, r = _run_both(slc, wts, Nx, Ny, "evd", "STBAS", precision, bw=12, ref=5)
+    st = compare_link(g, r, r["gap"], stack=slc, ref_index=5)
+    print(st)
+    assert st["n_well"] >= 16 and st["codes_equal"]
+    assert st["comp_err"] <= PHASE_TOL
+    assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL
+    assert np.all(g[0][5][r["tcorr"] > 0] == 1 + 0j)
+
+
 def test_link_64row_solver_bf16():
     n, rows, cols, Nx, Ny = 40, 12, 14, 4, 3
     print("case 64-row n=40 mixed_bf16", flush=True)
@@ -172,18 +208,21 @@ def test_link_generic_solver_n72(precision):
     assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL and st["comp_err"] <= PHASE_TOL
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision,place", [("fp64", None), ("mixed", "lds"), ("mixed", "registers")])
 @pytest.mark.parametrize("module,method", [("phase_link", "MLE"), ("evd", "EVD"), ("evd", "MLE")])
 @pytest.mark.parametrize("n", [72, 96])
-def test_link_generic_solver_modes(n, module, method, precision):
-    # every estimator of the LDS-resident solver above 64 dates; a 15 x 11
-    # window holds 165 samples in the interior (6 x 6 pixels), fewer at the
-    # edges, where the sample coherence of 96 dates can be rank deficient
+def test_link_generic_solver_modes(n, module, method, precision, place):
+    # every estimator of the solvers above 64 dates: the LDS-resident work
+    # matrix, and in the mixed precisions the two-wave solver with each row
+    # in a lane's registers; a 15 x 11 window holds 165 samples in the
+    # interior (6 x 6 pixels), fewer at the edges, where the sample coherence
+    # of 96 dates can be rank deficient
     rows, cols, Nx, Ny = 16, 20, 7, 5
-    print(f"case generic n={n} {module}/{method} {precision}", flush=True)
+    print(f"case generic n={n} {module}/{method} {precision} {place}", flush=True)
     slc = synthetic.stack(n, rows, cols, seed=400 + n)
     wts = synthetic.full_mask(rows, cols, Nx, Ny)
-    g, r = _run_both(slc, wts, Nx, Ny, module, method, precision)
+    with _GenericPlace(n, precision, place):
+        g, r = _run_both(slc, wts, Nx, Ny, module, method, precision)
     st = compare_link(g, r, r["gap"], stack=slc)
     print(st)
     assert st["untouched_equal"] and st["codes_equal"]
@@ -191,22 +230,24 @@ def test_link_generic_solver_modes(n, module, method, precision):
     assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL and st["comp_err"] <= PHASE_TOL
 
 
-@pytest.mark.parametrize("n,Nx,Ny,module,method", [(10, 29, 9, "evd", "MLE"), (30, 7, 5, "phase_link", "MLE"),
-                                                  (64, 5, 4, "evd", "MLE"), (64, 5, 4, "evd", "EVD"),
-                                                  (48, 5, 4, "phase_link", "MLE"), (96, 7, 5, "phase_link", "MLE"),
-                                                  (80, 6, 5, "evd", "MLE")])
-def test_mixed_resolve_equals_fp64(n, Nx, Ny, module, method):
+@pytest.mark.parametrize("n,Nx,Ny,module,method,place", [
+    (10, 29, 9, "evd", "MLE", None), (30, 7, 5, "phase_link", "MLE", None), (64, 5, 4, "evd", "MLE", None),
+    (64, 5, 4, "evd", "EVD", None), (48, 5, 4, "phase_link", "MLE", None), (96, 7, 5, "phase_link", "MLE", "lds"),
+    (96, 7, 5, "phase_link", "MLE", "registers"), (80, 6, 5, "evd", "MLE", "lds"),
+    (80, 6, 5, "evd", "MLE", "registers"), (88, 6, 5, "evd", "EVD", "registers")])
+def test_mixed_resolve_equals_fp64(n, Nx, Ny, module, method, place):
     """Pixels whose FP32 eigenpair the mixed path cannot certify are
     recomputed by the FP64 path (FP64 Gram and solver) in the same call:
     their outputs equal the plain FP64 run's bit for bit, and every other
     pixel agrees with the oracle to the tolerances."""
     rows, cols = 20, 40
-    print(f"case resolve n={n} {2 * Nx + 1}x{2 * Ny + 1} {module}/{method}", flush=True)
+    print(f"case resolve n={n} {2 * Nx + 1}x{2 * Ny + 1} {module}/{method} {place}", flush=True)
     slc = synthetic.stack(n, rows, cols, seed=700 + n, patches=6)
     wts, _ = oracle.nmap(slc, Nx, Ny, 0.05)
     out = {}
     for precision in ("fp64", "mixed"):
-        g, r, res = _run_both(slc, wts, Nx, Ny, module, method, precision, min_nb=5, want_resolved=True)
+        with _GenericPlace(n, precision, place if precision == "mixed" else None):
+            g, r, res = _run_both(slc, wts, Nx, Ny, module, method, precision, min_nb=5, want_resolved=True)
         out[precision] = (g, res)
     (g64, _), (gmx, res) = out["fp64"], out["mixed"]
     print("re-solved", res.size)

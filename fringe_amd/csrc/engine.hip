@@ -164,8 +164,8 @@ int link_check(const fringe_link_tile* t) {
     if (t->method == FRINGE_METHOD_STBAS && (t->bandwidth <= 0 || t->bandwidth >= t->nbands - 1))
         return FRINGE_ERR_BANDWIDTH;   // evd.cpp:74-91
     if ((long long)t->rows * t->cols >= (1ll << 31) - 1) return FRINGE_ERR_UNSUPPORTED;
-    // packed records are addressed with 32-bit word offsets
-    if (link_uses_mfma(t->nbands, t->precision) &&
+    // packed records are addressed with 32-bit word offsets up to 64 dates
+    if (link_uses_mfma(t->nbands, t->precision) && t->nbands <= 64 &&
         ((long long)t->rows * t->cols + 1) * 2 * t->nbands >= (1ll << 32))
         return FRINGE_ERR_UNSUPPORTED;
     // LDS footprint of the generic solver used above 64 dates (one pixel per

@@ -14,6 +14,7 @@ namespace fringe {
 
 constexpr int kMaxBandsNmap = 128;   // nmap sorts at most this many dates
 constexpr int kMaxBandsLink = 256;   // phase-linking solver limit
+constexpr int kMaxBandsMfma = 128;   // MFMA Gram (mixed precisions); the FP64 Gram above
 constexpr int kResolveGenericBlocks = 256;   // workgroups of the FP64 re-solve above 64 dates
 constexpr int kMaxMaskWords = 64;    // window up to 2048 positions (the reference sequential default 59 x 19 is 1121)
 

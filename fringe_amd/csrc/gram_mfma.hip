@@ -30,6 +30,8 @@
 //               another -- and issues 3 MFMAs per block pair.  The upper
 //               block triangle of P is folded into Gamma (the transposed
 //               cross block goes through a per-wave LDS scratch) and written.
+//               Above 64 dates (gram_mfma_pairs) the wave walks the pairs of
+//               32-date blocks one at a time instead of holding every block.
 #include <algorithm>
 
 #include "device_common.hpp"
@@ -338,6 +340,157 @@ __global__ __launch_bounds__(64 * WAVES) void gram_mfma(MfmaArgs a) {
                     }
                 }
             }
+        if (lane == 0) a.npix[slot] = cnt;
+        wave_sync_lds();
+    }
+}
+
+// 64 < n <= 128: the wave of a pixel walks the date-block pairs (I, J),
+// I <= J < nb = ceil(n / 32), diagonal pairs first.  Per pair it keeps the
+// four 32 x 32 real products P(re_I, re_J), P(im_I, im_J), P(re_I, im_J),
+// P(im_I, re_J) (64 accumulator registers whatever n; 12 MFMAs per 16
+// positions) and writes Gamma block (I, J) and its mirror; Re G_ii of the
+// diagonal pairs (per-wave LDS) normalises the later ones.  Each position's
+// (re, im) word pair of a date is one 8-byte load, and records are addressed
+// with 64-bit offsets (2n words per record exceed 32-bit word offsets on
+// large tiles).
+template <int WAVES, bool F16>
+__global__ __launch_bounds__(64 * WAVES) void gram_mfma_pairs(MfmaArgs a) {
+    typedef typename Frag<F16>::type frag_t;
+    extern __shared__ __align__(16) char smem[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const size_t wbytes = (size_t)a.list_cap * 4 + 128 * 4;
+    int* list = (int*)(smem + (size_t)wave * wbytes);
+    float* dg = (float*)(list + a.list_cap);   // Re G_ii of the pixel, 128 dates
+
+    const int n = a.n, Nx = a.Nx, Ny = a.Ny;
+    const int nb = (n + 31) >> 5;
+    const int npairs = nb * (nb + 1) / 2;
+    const size_t CS = 2 * (size_t)n;
+    const int npx_zero = a.rows * a.cols;      // index of the all-zero record
+    const int R0 = a.row0 + blockIdx.y * a.TH, C0 = blockIdx.x * a.TW;
+    const int row_end = a.row0 + a.chunk_rows;
+    const int Wx = 2 * Nx + 1, Wn = (2 * Ny + 1) * Wx;
+    const int m = lane & 31, half = lane >> 5;
+
+    for (int pid = wave; pid < a.TH * a.TW; pid += WAVES) {
+        const int r = R0 + pid / a.TW, c = C0 + pid % a.TW;
+        if (r >= row_end || r >= a.rows || c >= a.cols) continue;   // wave-uniform
+        const long long pix = (long long)r * a.cols + c;
+        const int slot = (r - a.row0) * a.cols + c;
+        const uint32_t* w = a.wts + pix * a.nul;
+        const bool cen = shp_get(w, shp_flat(0, 0, Nx, Ny));
+
+        // ---- neighbour list ----
+        int cnt = 0;
+        for (int b0 = 0; b0 < Wn; b0 += 64) {
+            const int f = b0 + lane;
+            bool take = false;
+            int q = 0;
+            if (cen && f < Wn) {
+                const int rr = r + f / Wx - Ny, cc = c + f % Wx - Nx;
+                if (rr >= 0 && rr < a.rows && cc >= 0 && cc < a.cols && shp_get(w, f)) {
+                    take = true;
+                    q = rr * a.cols + cc;
+                }
+            }
+            const uint64_t bal = __ballot(take);
+            if (take) {
+                const uint64_t below = (lane == 0) ? 0ull : (bal & ((~0ull) >> (64 - lane)));
+                list[cnt + __popcll(below)] = q;
+            }
+            cnt += __popcll(bal);
+        }
+        if (!(cen && cnt >= a.min_nb)) {
+            if (lane == 0) a.npix[slot] = -1;
+            continue;
+        }
+        const int steps = (cnt + 15) >> 4;
+        for (int e = cnt + lane; e < steps * 16; e += 64) list[e] = npx_zero;
+        wave_sync_lds();
+
+        float2* gout = (float2*)a.gamma + (size_t)slot * n * n;
+        for (int p = 0; p < npairs; ++p) {   // uniform
+            int I = p, J = p;
+            if (p >= nb) {   // off-diagonal pairs in row order
+                int q = p - nb;
+                I = 0;
+                while (q >= nb - 1 - I) {
+                    q -= nb - 1 - I;
+                    ++I;
+                }
+                J = I + 1 + q;
+            }
+            const int di = 32 * I + m, dj = 32 * J + m;
+            const bool vi = di < n, vj = dj < n;
+            const int oi = vi ? di : 0, oj = vj ? dj : 0;   // in (re, im) word pairs
+            f32x16 RR, II, RI, IR;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) RR[v] = II[v] = RI[v] = IR[v] = 0.0f;
+            for (int s = 0; s < steps; ++s) {
+                const int* lp = list + 16 * s + 8 * half;
+                uint2 wi[8], wj[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const uint2* rec = (const uint2*)(a.pk + (size_t)(uint32_t)lp[e] * CS);
+                    wi[e] = rec[oi];   // valid address for every lane
+                    wj[e] = rec[oj];
+                }
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    if (!vi) wi[e] = make_uint2(0u, 0u);
+                    if (!vj) wj[e] = make_uint2(0u, 0u);
+                }
+                u32x4 rih, ril, iih, iil, rjh, rjl, ijh, ijl;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    rih[e] = __builtin_amdgcn_perm(wi[2 * e + 1].x, wi[2 * e].x, 0x05040100u);
+                    ril[e] = __builtin_amdgcn_perm(wi[2 * e + 1].x, wi[2 * e].x, 0x07060302u);
+                    iih[e] = __builtin_amdgcn_perm(wi[2 * e + 1].y, wi[2 * e].y, 0x05040100u);
+                    iil[e] = __builtin_amdgcn_perm(wi[2 * e + 1].y, wi[2 * e].y, 0x07060302u);
+                    rjh[e] = __builtin_amdgcn_perm(wj[2 * e + 1].x, wj[2 * e].x, 0x05040100u);
+                    rjl[e] = __builtin_amdgcn_perm(wj[2 * e + 1].x, wj[2 * e].x, 0x07060302u);
+                    ijh[e] = __builtin_amdgcn_perm(wj[2 * e + 1].y, wj[2 * e].y, 0x05040100u);
+                    ijl[e] = __builtin_amdgcn_perm(wj[2 * e + 1].y, wj[2 * e].y, 0x07060302u);
+                }
+                const frag_t Rh = __builtin_bit_cast(frag_t, rih), Rl = __builtin_bit_cast(frag_t, ril);
+                const frag_t Ih = __builtin_bit_cast(frag_t, iih), Il = __builtin_bit_cast(frag_t, iil);
+                const frag_t Sh = __builtin_bit_cast(frag_t, rjh), Sl = __builtin_bit_cast(frag_t, rjl);
+                const frag_t Th = __builtin_bit_cast(frag_t, ijh), Tl = __builtin_bit_cast(frag_t, ijl);
+                RR = Frag<F16>::mfma(Rh, Sh, RR);
+                RR = Frag<F16>::mfma(Rh, Sl, RR);
+                RR = Frag<F16>::mfma(Rl, Sh, RR);
+                II = Frag<F16>::mfma(Ih, Th, II);
+                II = Frag<F16>::mfma(Ih, Tl, II);
+                II = Frag<F16>::mfma(Il, Th, II);
+                RI = Frag<F16>::mfma(Rh, Th, RI);
+                RI = Frag<F16>::mfma(Rh, Tl, RI);
+                RI = Frag<F16>::mfma(Rl, Th, RI);
+                IR = Frag<F16>::mfma(Ih, Sh, IR);
+                IR = Frag<F16>::mfma(Ih, Sl, IR);
+                IR = Frag<F16>::mfma(Il, Sh, IR);
+            }
+            if (I == J) {
+#pragma unroll
+                for (int v = 0; v < 16; ++v)
+                    if (acc_row(v, lane) == m) dg[32 * I + m] = RR[v] + II[v];
+                wave_sync_lds();
+            }
+            const int j = 32 * J + m;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int i = 32 * I + acc_row(v, lane);
+                if (i >= n || j >= n || i > j) continue;
+                float gr = 1.0f, gi = 0.0f;
+                if (i != j) {
+                    const float sc = rsqrtf(dg[i] * dg[j]);
+                    gr = (RR[v] + II[v]) * sc;
+                    gi = (IR[v] - RI[v]) * sc;   // P(im_i, re_j) - P(re_i, im_j)
+                }
+                gout[(size_t)i * n + j] = make_float2(gr, gi);
+                if (i != j) gout[(size_t)j * n + i] = make_float2(gr, -gi);
+            }
+        }
         if (lane == 0) a.npix[slot] = cnt;
         wave_sync_lds();
     }
@@ -663,7 +816,7 @@ static void launch_mfma(MfmaArgs a, hipStream_t s) {
 
 int gram_mfma_launch(const LinkPlan& P, const uint32_t* pk, const uint32_t* wts, long long first,
                      int slots, hipStream_t s) {
-    if (P.n > 64 || first % P.cols != 0 || slots % P.cols != 0) return -1;
+    if (P.n > kMaxBandsMfma || first % P.cols != 0 || slots % P.cols != 0) return -1;
     MfmaArgs a{};
     a.pk = pk;
     a.wts = wts;
@@ -705,9 +858,14 @@ int gram_mfma_launch(const LinkPlan& P, const uint32_t* pk, const uint32_t* wts,
         } else {
             launch_mfma<1, 4, false>(a, s);
         }
-    } else {
+    } else if (P.n <= 64) {
         if (f16) launch_mfma<2, 4, true>(a, s);
         else launch_mfma<2, 4, false>(a, s);
+    } else {
+        const size_t wbytes = (size_t)a.list_cap * 4 + 128 * 4;
+        const dim3 grid((unsigned)((a.cols + a.TW - 1) / a.TW), (unsigned)((a.chunk_rows + a.TH - 1) / a.TH));
+        if (f16) hipLaunchKernelGGL((gram_mfma_pairs<4, true>), grid, dim3(256), wbytes * 4, s, a);
+        else hipLaunchKernelGGL((gram_mfma_pairs<4, false>), grid, dim3(256), wbytes * 4, s, a);
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -77,7 +77,7 @@ static void launch_solve96(const SolveArgs& a, hipStream_t s) {
 
 template <typename T, typename Tg>
 static void dispatch_generic(const SolveArgs& a, hipStream_t s) {
-    if constexpr (sizeof(T) == 4) {
+    if constexpr (sizeof(T) == 4 && sizeof(Tg) == 4) {
         if (!a.awork && solve96_selected(a.n, FRINGE_PREC_MIXED)) {
             launch_solve96(a, s);
             return;
@@ -91,9 +91,14 @@ static void dispatch_generic(const SolveArgs& a, hipStream_t s) {
 // scratch); the launch checks add it to the dynamic LDS of a pixel.
 size_t solve_static_lds(int precision) {
     hipFuncAttributes fa{};
-    const void* k = precision == FRINGE_PREC_FP64 ? (const void*)solve_kernel<double, double, 128, 2, false>
-                                                   : (const void*)solve_kernel<float, double, 128, 2, false>;
-    return hipFuncGetAttributes(&fa, k) == hipSuccess ? fa.sharedSizeBytes : 256;
+    size_t most = 0;
+    const void* ks[2] = {precision == FRINGE_PREC_FP64 ? (const void*)solve_kernel<double, double, 128, 2, false>
+                                                        : (const void*)solve_kernel<float, double, 128, 2, false>,
+                         precision == FRINGE_PREC_FP64 ? (const void*)solve_kernel<double, double, 128, 1, false>
+                                                        : (const void*)solve_kernel<float, float, 128, 1, false>};
+    for (const void* k : ks) most = std::max<size_t>(most, hipFuncGetAttributes(&fa, k) == hipSuccess
+                                                               ? fa.sharedSizeBytes : 256);
+    return most;
 }
 
 // Whether the work matrix of the solver above 64 dates fits in LDS.
@@ -230,7 +235,7 @@ __global__ void zero_i32(int* p, int count) {
     if (i < count) p[i] = 0;
 }
 
-bool link_uses_mfma(int n, int precision) { return precision != FRINGE_PREC_FP64 && n <= 64; }
+bool link_uses_mfma(int n, int precision) { return precision != FRINGE_PREC_FP64 && n <= kMaxBandsMfma; }
 bool link_certifies(int n, int precision) { return precision != FRINGE_PREC_FP64; }
 
 size_t link_gamma_bytes_per_pixel(int n, int precision) {
@@ -323,8 +328,10 @@ static int cu_count() {
 }
 
 // Date class of the solvers: 0 register NR = 32 (n <= 32), 1 register
-// NR = 64 (n <= 64), 2 generic (n > 64).
-static int solve_class(int n) { return n <= 32 ? 0 : (n <= 64 ? 1 : 2); }
+// NR = 64 (n <= 64), 2 generic one row per thread (n <= 128: the MFMA
+// Gram's complex64 coherence in the mixed precisions), 3 generic two rows
+// per thread (n > 128, complex128 coherence).
+static int solve_class(int n) { return n <= 32 ? 0 : (n <= 64 ? 1 : (n <= kMaxBandsMfma ? 2 : 3)); }
 
 static PJob make_job(const LinkPlan& P, const float2* slc, float2* phase, float* tcorr, float2* comp,
                      long long first, int slots, const void* gamma, const int* npix, void* awork, size_t awork_stride) {
@@ -384,20 +391,19 @@ static void launch_persistent(PersistArgs pa, int cls, int mode, int sched, hipS
         const int per_cu = waves_cu ? std::max(1, waves_cu / (threads / 64)) : occupancy_blocks(kernel, threads, dyn);
         return dim3((unsigned)std::min<long long>((long long)per_cu * cus, (units + grab - 1) / grab));
     };
-    // coherence in double above 64 dates (both precisions), in float below in the mixed ones
-    if constexpr (sizeof(T) == 8 || sizeof(Tg) == 8) {
-        if (cls == 2) {
-            int nmax = 0;
-            for (int j = 0; j < pa.njobs; ++j) nmax = std::max(nmax, pa.job[j].n);
-            if (nmax <= 128) {
-                auto k = solve_generic_persistent<T, 1>;
-                hipLaunchKernelGGL(k, grid_for(k, 128, lds), dim3(128), lds, s, pa);
-            } else {
-                auto k = solve_generic_persistent<T, 2>;
-                hipLaunchKernelGGL(k, grid_for(k, 128, lds), dim3(128), lds, s, pa);
-            }
-            return;
+    if (cls == 2) {   // complex64 coherence in the mixed precisions, complex128 in FP64
+        if constexpr (sizeof(T) == sizeof(Tg)) {
+            auto k = solve_generic_persistent<T, Tg, 1>;
+            hipLaunchKernelGGL(k, grid_for(k, 128, lds), dim3(128), lds, s, pa);
         }
+        return;
+    }
+    if (cls == 3) {   // complex128 coherence (FP64 Gram) in every precision
+        if constexpr (sizeof(Tg) == 8) {
+            auto k = solve_generic_persistent<T, Tg, 2>;
+            hipLaunchKernelGGL(k, grid_for(k, 128, lds), dim3(128), lds, s, pa);
+        }
+        return;
     }
     if constexpr (sizeof(T) == sizeof(Tg)) {
 #define FRINGE_PERSIST(MODE_, NR_)                                                   \
@@ -420,7 +426,7 @@ static void launch_persistent(PersistArgs pa, int cls, int mode, int sched, hipS
 
 static void launch_persistent_any(const PersistArgs& pa, int cls, int mode, int precision, int sched, hipStream_t s) {
     if (precision == FRINGE_PREC_FP64) launch_persistent<double, double>(pa, cls, mode, sched, s);
-    else if (cls == 2) launch_persistent<float, double>(pa, cls, mode, sched, s);
+    else if (cls == 3) launch_persistent<float, double>(pa, cls, mode, sched, s);
     else launch_persistent<float, float>(pa, cls, mode, sched, s);
 }
 
@@ -503,6 +509,7 @@ int link_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, float
                 a.awork = P.d_awork;
                 a.awork_stride = P.awork_stride;
                 if (fp64) dispatch_generic<double, double>(a, s);
+                else if (mfma) dispatch_generic<float, float>(a, s);
                 else dispatch_generic<float, double>(a, s);
             }
         }
@@ -561,16 +568,16 @@ int link_batch_launch(const BatchPlan& B, hipStream_t s) {
     long long row = count ? B.tiles[0].P.row_begin : 0;
     int rc = 0;
     while (item < count) {
-        // classes 0..2 by date count (solve_class); 3: the register-row
+        // classes 0..3 by date count (solve_class); 4: the register-row
         // solver's tiles, one grid launch each
-        PersistArgs pa[4] = {};
+        PersistArgs pa[5] = {};
         size_t off = 0;      // bytes of the round buffer in use
         long long soff = 0;  // slots in use
         bool full = false;
         while (item < count && !full) {
             const BatchTile& T = B.tiles[item];
             const LinkPlan& P = T.P;
-            const int cls = solve96_selected(P.n, P.precision) ? 3 : solve_class(P.n);
+            const int cls = solve96_selected(P.n, P.precision) ? 4 : solve_class(P.n);
             const long long rem = P.row_end - row;
             if (rem <= 0) {
                 ++item;
@@ -609,8 +616,8 @@ int link_batch_launch(const BatchPlan& B, hipStream_t s) {
             row += rows;
         }
         StageScope sc(STAGE_SOLVE, s);
-        for (int j = 0; j < pa[3].njobs; ++j) launch_solve96(solve_args_of(pa[3].job[j], B.module, B.method), s);
-        for (int c = 0; c < 3; ++c) {
+        for (int j = 0; j < pa[4].njobs; ++j) launch_solve96(solve_args_of(pa[4].job[j], B.module, B.method), s);
+        for (int c = 0; c < 4; ++c) {
             if (!pa[c].njobs) continue;
             pa[c].module = B.module;
             pa[c].method = B.method;
@@ -624,9 +631,12 @@ int link_batch_launch(const BatchPlan& B, hipStream_t s) {
                 if (c < 2) {
                     if (f64) dispatch_reg<double, double>(a, s);
                     else dispatch_reg<float, float>(a, s);
+                } else if (f64) {
+                    dispatch_generic<double, double>(a, s);
+                } else if (c == 2) {   // the MFMA Gram's complex64 coherence
+                    dispatch_generic<float, float>(a, s);
                 } else {
-                    if (f64) dispatch_generic<double, double>(a, s);
-                    else dispatch_generic<float, double>(a, s);
+                    dispatch_generic<float, double>(a, s);
                 }
             } else {
                 launch_persistent_any(pa[c], c, mode, B.precision, sched, s);

@@ -831,7 +831,7 @@ __global__ __launch_bounds__(G > 64 ? G : 64) void solve_kernel(SolveArgs a) {
 
 // Persistent launch above 64 dates: one pixel per two-wave workgroup per
 // unit; each job's dynamic LDS fits the launch's (the largest of its jobs).
-template <typename T, int R>
+template <typename T, typename Tg, int R>
 __global__ __launch_bounds__(128) void solve_generic_persistent(PersistArgs pa) {
     __shared__ int s_u;
     extern __shared__ __align__(16) char smem[];
@@ -848,8 +848,8 @@ __global__ __launch_bounds__(128) void solve_generic_persistent(PersistArgs pa) 
             const int slot = u - (j ? pa.unit_end[j - 1] : 0);
             cx<T>* aglob = a.awork ? (cx<T>*)((char*)a.awork + a.awork_stride * (size_t)slot) : nullptr;
             SolveLds<T> L = solve_carve<T>(smem, a.n, aglob);
-            const cx<double>* gam = (const cx<double>*)a.gamma + (size_t)slot * a.n * a.n;
-            solve_generic_pixel<T, double, 128, R, sizeof(T) == 4>(a, L, true, a.npix[slot], gam, a.first + slot);
+            const cx<Tg>* gam = (const cx<Tg>*)a.gamma + (size_t)slot * a.n * a.n;
+            solve_generic_pixel<T, Tg, 128, R, sizeof(T) == 4>(a, L, true, a.npix[slot], gam, a.first + slot);
             __syncthreads();
         }
     }

@@ -76,7 +76,7 @@ __device__ __forceinline__ int bfirst(L96& L, BRed& br, bool p) {
 
 // ---- FP64 inverse of |Gamma| - shift*I, one row per lane ------------------
 // m[c] = |Gamma_rc| (c != r), 1 - shift on the diagonal, identity outside n.
-__device__ __forceinline__ bool abs_gamma_row96(double (&m)[kC96], const cx<double>* grow, int n, int r, bool act,
+__device__ __forceinline__ bool abs_gamma_row96(double (&m)[kC96], const cx<float>* grow, int n, int r, bool act,
                                                 double shift) {
     bool fin = true;
 #pragma unroll
@@ -85,8 +85,8 @@ __device__ __forceinline__ bool abs_gamma_row96(double (&m)[kC96], const cx<doub
         for (int c = c0; c < c0 + 16; ++c) {
             double v = (c == r) ? 1.0 - shift : 0.0;
             if (act && r < n && c < n && c != r) {
-                const cx<double> g = grow[c];
-                v = sqrt(g.re * g.re + g.im * g.im);
+                const cx<float> g = grow[c];
+                v = (double)__builtin_amdgcn_sqrtf(g.re * g.re + g.im * g.im);   // one v_sqrt_f32 (~1 ulp)
                 fin = fin && isfinite(v);
             }
             m[c] = v;
@@ -372,29 +372,8 @@ __device__ __forceinline__ bool eig_uncertain96(L96& L, BRed& br, int n, int r, 
     return largest ? (c <= n - 2) : (c >= 2);
 }
 
-__device__ __forceinline__ bool load_rows96(L96& L, BRed& br, cx<float> (&h)[kC96], const cx<double>* grow, int n,
+__device__ __forceinline__ bool load_rows96(L96& L, BRed& br, cx<float> (&h)[kC96], const cx<float>* grow, int n,
                                             int r, bool act, int band) {
-    bool fin = true;
-#pragma unroll
-    for (int c0 = 0; c0 < kC96; c0 += 16) {
-#pragma unroll
-        for (int c = c0; c < c0 + 16; ++c) {
-            cx<double> v = {0.0, 0.0};
-            if (r < n && c < n) {
-                if (act) v = grow[c];
-                else if (c == r) v = cx<double>{1.0, 0.0};
-                if (band >= 0 && (c - r > band || r - c > band)) v = cx<double>{0.0, 0.0};
-            }
-            fin = fin && isfinite(v.re) && isfinite(v.im);
-            h[c] = cx<float>{(float)v.re, (float)v.im};
-        }
-        asm volatile("" ::: "memory");
-    }
-    return bred<OpMin>(L, br, fin ? 1.0f : 0.0f) > 0.5f;
-}
-
-__device__ __forceinline__ bool hadamard_rows96(L96& L, BRed& br, cx<float> (&h)[kC96], const double (&m)[kC96],
-                                                const cx<double>* grow, int n, int r, bool act) {
     bool fin = true;
 #pragma unroll
     for (int c0 = 0; c0 < kC96; c0 += 16) {
@@ -402,10 +381,31 @@ __device__ __forceinline__ bool hadamard_rows96(L96& L, BRed& br, cx<float> (&h)
         for (int c = c0; c < c0 + 16; ++c) {
             cx<float> v = {0.0f, 0.0f};
             if (r < n && c < n) {
-                cx<double> g = (c == r) ? cx<double>{1.0, 0.0} : cx<double>{0.0, 0.0};
+                if (act) v = grow[c];
+                else if (c == r) v = cx<float>{1.0f, 0.0f};
+                if (band >= 0 && (c - r > band || r - c > band)) v = cx<float>{0.0f, 0.0f};
+            }
+            fin = fin && isfinite(v.re) && isfinite(v.im);
+            h[c] = v;
+        }
+        asm volatile("" ::: "memory");
+    }
+    return bred<OpMin>(L, br, fin ? 1.0f : 0.0f) > 0.5f;
+}
+
+__device__ __forceinline__ bool hadamard_rows96(L96& L, BRed& br, cx<float> (&h)[kC96], const double (&m)[kC96],
+                                                const cx<float>* grow, int n, int r, bool act) {
+    bool fin = true;
+#pragma unroll
+    for (int c0 = 0; c0 < kC96; c0 += 16) {
+#pragma unroll
+        for (int c = c0; c < c0 + 16; ++c) {
+            cx<float> v = {0.0f, 0.0f};
+            if (r < n && c < n) {
+                cx<float> g = (c == r) ? cx<float>{1.0f, 0.0f} : cx<float>{0.0f, 0.0f};
                 if (act && c != r) g = grow[c];
                 const float mm = (float)(-m[c]);   // m holds -|Gamma|^-1
-                v = cx<float>{mm * (float)g.re, mm * (float)g.im};
+                v = cx<float>{mm * g.re, mm * g.im};
                 fin = fin && isfinite(v.re + v.im);
             }
             h[c] = v;
@@ -427,8 +427,8 @@ __global__ __launch_bounds__(128) void solve96_kernel(SolveArgs a) {
     const int n = a.n;
     const int np = a.npix[slot];
     const bool act = np >= 0;
-    const cx<double>* gam = (const cx<double>*)a.gamma + (size_t)slot * n * n;
-    const cx<double>* grow = gam + (size_t)min(r, n - 1) * n;
+    const cx<float>* gam = (const cx<float>*)a.gamma + (size_t)slot * n * n;   // the MFMA Gram's complex64
+    const cx<float>* grow = gam + (size_t)min(r, n - 1) * n;
     const long long pix = a.first + slot;
 
     float code = 0.0f;
@@ -537,7 +537,7 @@ __global__ __launch_bounds__(128) void solve96_kernel(SolveArgs a) {
             jd = jd + 1 == n ? 0 : jd + 1;
             const int lo = min(r, jd), hi = max(r, jd);
             if (stbas && hi - lo > a.bw) continue;
-            const cx<double> gv = grow[jd];
+            const cx<float> gv = grow[jd];
             const cx<float> ph = unit_or(cx<float>{(float)gv.re, (float)gv.im}, cx<float>{1.0f, 0.0f}, okx);
             const cx<float> oj = L.OUT[jd];
             cx<float> t = cmul(cmul(ph, ci), oj);

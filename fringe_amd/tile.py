@@ -156,7 +156,7 @@ def coherence(slc, wts, Nx, Ny, precision="mixed", module="phase_link", min_neig
         raise RuntimeError("invalid or unsupported coherence parameters")
     work = torch.empty(wb, dtype=torch.uint8, device=slc.device)
     npx = (p.row_end - p.row_begin) * cols
-    c64 = p.precision != _lib.FRINGE_PREC_FP64 and n <= 64
+    c64 = p.precision != _lib.FRINGE_PREC_FP64 and n <= 128
     gam = torch.empty((npx, n, n), dtype=torch.complex64 if c64 else torch.complex128, device=slc.device)
     npix = torch.empty(npx, dtype=torch.int32, device=slc.device)
     rc = L.fringe_coherence_run(ctypes.byref(p), _ptr(slc.contiguous()), _ptr(wts.contiguous()), _ptr(gam),

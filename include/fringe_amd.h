@@ -132,7 +132,7 @@ typedef struct fringe_link_tile {
 size_t fringe_link_workspace_bytes(const fringe_link_tile* t);
 /* Replaces the per-block pixel loops of evd_process (evd.cpp:512-788) and
  * phase_link's evd_process (phase_link.cpp:479-666) on a device tile.
- * Mixed precisions (nbands <= 64) certify every FP32 eigenpair: a pixel whose
+ * The mixed precisions certify every FP32 eigenpair: a pixel whose
  * eigen-gap, smallest eigenvector component or distance from a 1e-6
  * eigenvalue threshold / positive-definiteness boundary does not guarantee
  * 5e-4 rad is recomputed end to end by the FP64 path (FP64 Gram and FP64
@@ -165,9 +165,11 @@ long long fringe_link_resolved_pixels(const fringe_link_tile* t, const void* d_w
  * nbands, window, size, min_neighbors, ref_index, bandwidth; one module,
  * method and precision for the whole batch) on one stream: the coherence
  * matrices of several tiles share each round's buffer, and every round's
- * Hermitian solves of one date class (<= 32, <= 64, > 64 dates) run in ONE
- * persistent launch that drains a queue of all tiles' pixels, whatever their
- * date count and neighbour count.  Outputs equal fringe_link_run's per tile
+ * Hermitian solves of one date class (<= 32, <= 64, <= 128, > 128 dates) run
+ * in ONE persistent launch that drains a queue of all tiles' pixels, whatever
+ * their date count and neighbour count (tiles on the mixed precisions'
+ * register-row solver, 65..96 dates, get one grid launch each).  Outputs
+ * equal fringe_link_run's per tile
  * bit for bit.  Returns 202 for a batch that mixes modules, methods or
  * precisions. */
 typedef struct fringe_link_batch_item {
@@ -193,7 +195,9 @@ int fringe_link_batch_run(const fringe_link_batch_item* items, int count, void* 
  * units at a time from a queue.  Unset keys keep the built-in defaults. */
 #define FRINGE_SCHED_GRAM_VARIANT 0
 #define FRINGE_SCHED_SOLVE 1
-#define FRINGE_SCHED_GENERIC_A 2    /* (nbands class 96/128/256, precision): 1 = solver work matrix in global memory */
+#define FRINGE_SCHED_GENERIC_A 2    /* (nbands class 96/128/256, precision): the solver above 64 dates keeps its
+                                       work matrix in LDS (0) or global memory (1), or (mixed precisions, <= 96
+                                       dates; the default there) each row in a lane's registers (2) */
 #define FRINGE_GRAM_DEFAULT 0
 #define FRINGE_GRAM_STAGED12 1      /* 12 waves, 8 x 16-pixel tiles staged in LDS */
 #define FRINGE_GRAM_STAGED8 2       /* 8 waves, 4 x 16 */
@@ -209,7 +213,7 @@ void fringe_schedule_clear(void);
  * diagonal) and d_npix[slot] = number of neighbours, or -1 when the pixel is
  * not linked (centre bit clear, too few neighbours).  Elements are complex64
  * when the precision uses the MFMA Gram (MIXED / MIXED_BF16 with nbands <=
- * 64) and complex128 otherwise. */
+ * 128) and complex128 otherwise. */
 size_t fringe_coherence_workspace_bytes(const fringe_link_tile* t);
 int fringe_coherence_run(const fringe_link_tile* t, const void* d_slc, const uint32_t* d_wts, void* d_gamma,
                          int* d_npix, void* d_work, size_t work_bytes, void* stream);

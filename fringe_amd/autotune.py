@@ -58,7 +58,9 @@ def tune(quick=False, log=sys.stderr):
     L.fringe_schedule_clear()   # tune from the built-in defaults
     dev = torch.device("cuda:0")
     reps = 2 if quick else 3
-    rows = 96 if quick else 256
+    # the full 2048 x 2048 tile: on a 256-row tile the persistent solver
+    # launch measured 4 % faster than the grid, on the full tile 7 % slower
+    rows = 96 if quick else 2048
     cols = 2048
     gram, solve = [], []
     stacks = {}

@@ -206,9 +206,10 @@ static int link_chunk_pixels(const fringe_link_tile* t) {
 
 struct LinkLayout {
     size_t gamma, npix, pack, cert, queue, awork, awork_stride, rwork, total;
+    int resolve_blocks;
 };
 
-static LinkLayout link_layout(const fringe_link_tile* t, int chunk) {
+static LinkLayout link_layout(const fringe_link_tile* t, int chunk, size_t area_budget) {
     const int n = t->nbands;
     const size_t e = link_uses_mfma(n, t->precision) ? 8 : 16;
     LinkLayout L{};
@@ -231,23 +232,28 @@ static LinkLayout link_layout(const fringe_link_tile* t, int chunk) {
         b += (size_t)chunk * L.awork_stride;
     }
     L.rwork = b;
-    if (n > 64 && link_certifies(n, t->precision)) b += (size_t)kResolveGenericBlocks * resolve_generic_area_bytes(n);
+    if (n > 64 && link_certifies(n, t->precision)) {
+        L.resolve_blocks = resolve_generic_blocks(resolve_generic_area_bytes(n), area_budget);
+        b += (size_t)L.resolve_blocks * resolve_generic_area_bytes(n);
+    }
     L.total = b;
     return L;
 }
 
-size_t link_workspace_for_chunk(const fringe_link_tile* t, int chunk) { return link_layout(t, chunk).total; }
+size_t link_workspace_for_chunk(const fringe_link_tile* t, int chunk, size_t area_budget) {
+    return link_layout(t, chunk, area_budget).total;
+}
 
 // Largest chunk (whole rows, whole tiles of the staged Gram when possible)
 // whose workspace fits in `bytes`; 0 when not even one row fits.
-int link_chunk_for_bytes(const fringe_link_tile* t, size_t bytes) {
+int link_chunk_for_bytes(const fringe_link_tile* t, size_t bytes, size_t area_budget) {
     const long long out_rows = t->row_end - t->row_begin;
     if (out_rows <= 0) return t->cols;
     const int th = link_uses_mfma(t->nbands, t->precision) ? gram_stage_rows(t->nbands, t->Nx, t->Ny) : 1;
     long long lo = 0, hi = out_rows;   // lo rows fit (0 = nothing yet)
     while (lo < hi) {
         const long long mid = (lo + hi + 1) / 2;
-        if (link_workspace_for_chunk(t, (int)(mid * t->cols)) <= bytes) lo = mid;
+        if (link_workspace_for_chunk(t, (int)(mid * t->cols), area_budget) <= bytes) lo = mid;
         else hi = mid - 1;
     }
     if (lo > th && lo < out_rows) lo -= lo % th;
@@ -257,8 +263,9 @@ int link_chunk_for_bytes(const fringe_link_tile* t, size_t bytes) {
 // Body of fringe_link_run with an explicit chunk (pixels per Gram/solve
 // round, a multiple of cols); the workspace must hold link_layout(chunk).
 int link_run_chunked(const fringe_link_tile* t, const void* d_slc, const uint32_t* d_wts, void* d_phase,
-                     float* d_tcorr, void* d_comp, void* d_work, size_t work_bytes, int chunk, void* stream) {
-    const LinkLayout LL = link_layout(t, chunk);
+                     float* d_tcorr, void* d_comp, void* d_work, size_t work_bytes, int chunk, void* stream,
+                     size_t area_budget) {
+    const LinkLayout LL = link_layout(t, chunk, area_budget);
     if (work_bytes < LL.total) return FRINGE_ERR_WORKSPACE;
     if (t->row_end == t->row_begin) return 0;
     LinkPlan P{};
@@ -291,7 +298,10 @@ int link_run_chunked(const fringe_link_tile* t, const void* d_slc, const uint32_
         P.d_awork = (char*)d_work + LL.awork;
         P.awork_stride = LL.awork_stride;
     }
-    if (P.n > 64 && link_certifies(P.n, P.precision)) P.d_rwork = (char*)d_work + LL.rwork;
+    if (P.n > 64 && link_certifies(P.n, P.precision)) {
+        P.d_rwork = (char*)d_work + LL.rwork;
+        P.resolve_blocks = LL.resolve_blocks;
+    }
     P.d_queue = (int*)((char*)d_work + LL.queue);
     hipStream_t s = (hipStream_t)stream;
     if (link_launch(P, (const float2*)d_slc, d_wts, (float2*)d_phase, d_tcorr, (float2*)d_comp, s) != 0) {
@@ -381,7 +391,8 @@ static BatchLayout batch_layout(const fringe_link_batch_item* items, int count) 
         max_row = std::max(max_row, (size_t)t->cols * per_px + 1024);
         pixels += (size_t)out;
         if (n > 64 && link_certifies(n, t->precision))
-            max_rwork = std::max(max_rwork, (size_t)kResolveGenericBlocks * resolve_generic_area_bytes(n));
+            max_rwork = std::max(max_rwork, (size_t)resolve_generic_blocks(resolve_generic_area_bytes(n)) *
+                                                resolve_generic_area_bytes(n));
     }
     L.round = b;
     L.round_bytes = std::max(std::min(need, (size_t)2048 << 20), max_row);

@@ -15,7 +15,16 @@ namespace fringe {
 constexpr int kMaxBandsNmap = 128;   // nmap sorts at most this many dates
 constexpr int kMaxBandsLink = 256;   // phase-linking solver limit
 constexpr int kMaxBandsMfma = 128;   // MFMA Gram (mixed precisions); the FP64 Gram above
-constexpr int kResolveGenericBlocks = 256;   // workgroups of the FP64 re-solve above 64 dates
+// Workgroups of the FP64 re-solve above 64 dates: as many as `budget`
+// bytes of per-workgroup global areas allow, between 256 and 1024 (several
+// per CU: the FP64 solver with its work matrix in global memory is latency
+// bound).  fringe_link_run's budget is 512 MB; the fixed-scratch entry
+// point passes a smaller one.
+constexpr size_t kResolveAreaBudget = (size_t)512 << 20;
+inline int resolve_generic_blocks(size_t area_bytes, size_t budget = kResolveAreaBudget) {
+    const size_t b = budget / (area_bytes ? area_bytes : 1);
+    return (int)(b < 256 ? 256 : (b > 1024 ? 1024 : b));
+}
 constexpr int kMaxMaskWords = 64;    // window up to 2048 positions (the reference sequential default 59 x 19 is 1121)
 
 // Decision table of the integer KS statistic, one bit per k = D*n.
@@ -63,6 +72,7 @@ struct LinkPlan {
     void* d_awork;
     size_t awork_stride;
     void* d_rwork;
+    int resolve_blocks;  // workgroups (and areas at d_rwork) of that re-solve; 0 = resolve_generic_blocks
     int* d_queue;       // work queue counter of a persistent solve launch
 };
 
@@ -131,11 +141,13 @@ int link_check(const fringe_link_tile* t);
 // Workspace bytes of fringe_link_run when each Gram/solve round covers
 // `chunk` pixels (a multiple of cols), and the largest such chunk that fits
 // in `bytes` (0 when not one row fits).
-size_t link_workspace_for_chunk(const fringe_link_tile* t, int chunk);
-int link_chunk_for_bytes(const fringe_link_tile* t, size_t bytes);
+// `area_budget` bounds the FP64 re-solve's areas above 64 dates.
+size_t link_workspace_for_chunk(const fringe_link_tile* t, int chunk, size_t area_budget = kResolveAreaBudget);
+int link_chunk_for_bytes(const fringe_link_tile* t, size_t bytes, size_t area_budget = kResolveAreaBudget);
 // fringe_link_run with an explicit chunk (arguments already checked).
 int link_run_chunked(const fringe_link_tile* t, const void* d_slc, const uint32_t* d_wts, void* d_phase,
-                     float* d_tcorr, void* d_comp, void* d_work, size_t work_bytes, int chunk, void* stream);
+                     float* d_tcorr, void* d_comp, void* d_work, size_t work_bytes, int chunk, void* stream,
+                     size_t area_budget = kResolveAreaBudget);
 void set_last_error(const std::string& what);
 
 int coherence_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, void* gamma, int* npix,

@@ -118,7 +118,7 @@ size_t resolve_generic_area_bytes(int n) {
 }
 
 size_t resolve_generic_lds_bytes(int n, int Nx, int Ny) {
-    return solve_group_bytes<double>(n, false) + 2 * gram_f64_group_bytes(n, Nx, Ny, 16) +
+    return solve_group_bytes<double>(n, false) + 2 * gram_f64_group_bytes(n, Nx, Ny, kResolveCB) +
            solve_static_lds(FRINGE_PREC_FP64);
 }
 
@@ -212,8 +212,8 @@ static void launch_resolve_generic(const LinkPlan& P, const uint32_t* wts, Solve
     a.awork = P.d_rwork;
     a.awork_stride = resolve_generic_area_bytes(P.n);
     a.group_bytes = solve_group_bytes<double>(P.n, false);
-    const size_t lds = a.group_bytes + 2 * gram_f64_group_bytes(P.n, P.Nx, P.Ny, 16);
-    const dim3 grid((unsigned)kResolveGenericBlocks);
+    const size_t lds = a.group_bytes + 2 * gram_f64_group_bytes(P.n, P.Nx, P.Ny, kResolveCB);
+    const dim3 grid((unsigned)(P.resolve_blocks ? P.resolve_blocks : resolve_generic_blocks(a.awork_stride)));
     if (P.n <= 128)
         hipLaunchKernelGGL((solve_kernel<double, double, 128, 1, true>), grid, dim3(128), lds, s, a);
     else

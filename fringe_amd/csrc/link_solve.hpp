@@ -792,6 +792,13 @@ __device__ inline int job_of_unit(const PersistArgs& pa, int u) {
 // listed pixels; both waves of a workgroup compute the listed pixel's FP64
 // coherence (the same values) into the workgroup's global area, then solve
 // it, looping over the list.
+// Column block of the re-solve's FP64 Gram, the plain path's 16.  (Each
+// coherence entry sums the same products in the same order whatever the
+// block; smaller blocks, which let the re-solve run three waves per SIMD
+// instead of one, cost more in passes, two barriers per position each,
+// than the occupancy gained: measured slower.)
+constexpr int kResolveCB = 16;
+
 template <typename T, typename Tg, int G, int R, bool LIST>
 __global__ __launch_bounds__(G > 64 ? G : 64) void solve_kernel(SolveArgs a) {
     constexpr int GPW = G >= 64 ? 1 : 64 / G;
@@ -815,12 +822,12 @@ __global__ __launch_bounds__(G > 64 ? G : 64) void solve_kernel(SolveArgs a) {
         cx<double>* gam = (cx<double>*)area;
         cx<double>* aglob = (cx<double>*)(area + (((size_t)n * n * sizeof(cx<double>) + 255) & ~(size_t)255));
         SolveLds<T> L = solve_carve<T>(smem, n, aglob);
-        const size_t gbytes = gram_f64_group_bytes(n, a.g64.Nx, a.g64.Ny, 16);
+        const size_t gbytes = gram_f64_group_bytes(n, a.g64.Nx, a.g64.Ny, kResolveCB);
         char* gscratch = smem + a.group_bytes + (threadIdx.x / 64) * gbytes;
         const int cnt = min(*a.list_count, a.list_cap);
         for (int e = blockIdx.x; e < cnt; e += gridDim.x) {
             const long long pix = a.list[e];
-            const int np = gram_f64_pixel<64, RG, 16>(a.g64, true, pix, gam, gscratch);
+            const int np = gram_f64_pixel<64, RG, kResolveCB>(a.g64, true, pix, gam, gscratch);
             __threadfence();
             __syncthreads();
             solve_generic_pixel<T, Tg, G, R, false>(a, L, true, np, gam, pix);

@@ -55,8 +55,10 @@ extern "C" int fringe_link_run_noalloc(const fringe_link_tile* t, const void* d_
     const int st = link_check(t);
     if (st != 0) return st;
     if (!d_slc || !d_wts || !d_phase || !d_tcorr || !d_comp) return FRINGE_ERR_ARGUMENT;
-    // the largest round (whole rows) whose workspace fits the fixed scratch
-    const int chunk = link_chunk_for_bytes(t, kScratchBytes);
+    // the largest round (whole rows) whose workspace fits the fixed scratch;
+    // above 64 dates the FP64 re-solve keeps its minimum of 256 areas
+    constexpr size_t kAreaBudget = kScratchBytes / 4;
+    const int chunk = link_chunk_for_bytes(t, kScratchBytes, kAreaBudget);
     if (chunk <= 0) {
         set_last_error("fringe_link_run_noalloc: the tile needs more than the 256 MiB fixed scratch "
                        "(packed records of the whole tile plus one row of coherence matrices)");
@@ -65,5 +67,5 @@ extern "C" int fringe_link_run_noalloc(const fringe_link_tile* t, const void* d_
     void* p = nullptr;
     const int rc = scratch_ptr(&p);
     if (rc != 0) return rc;
-    return link_run_chunked(t, d_slc, d_wts, d_phase, d_tcorr, d_comp, p, kScratchBytes, chunk, stream);
+    return link_run_chunked(t, d_slc, d_wts, d_phase, d_tcorr, d_comp, p, kScratchBytes, chunk, stream, kAreaBudget);
 }

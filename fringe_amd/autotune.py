@@ -13,7 +13,8 @@ stage's launches), and keeps the fastest:
   from the work queue.
 * Above 64 dates, per (date class, precision): the solver's work matrix in
   LDS (where it fits) or in global memory (more pixels in flight), or, for
-  the mixed precisions up to 96 dates, each row in a lane's registers.
+  the mixed precisions up to 128 dates, each row in registers (half a row
+  per thread, four waves per pixel).
 
 The chosen table (with every candidate's time) is written as JSON; the
 library installs it when it is loaded (fringe_amd._lib.load_schedules).
@@ -137,7 +138,7 @@ def tune(quick=False, log=sys.stderr):
         pk = 0 if prec == "fp64" else 1
         default = L.fringe_schedule_get(_lib.FRINGE_SCHED_GENERIC_A, n, pk, 0)
         times = {}
-        places = ["lds", "global"] + (["registers"] if n <= 96 and prec != "fp64" else [])
+        places = ["lds", "global"] + (["registers"] if n <= 128 and prec != "fp64" else [])
         for place in places:
             v = _lib.GENERIC_A_PLACES[place]
             L.fringe_schedule_set(_lib.FRINGE_SCHED_GENERIC_A, n, pk, 0, v)

@@ -19,7 +19,7 @@
 #include "engine_internal.hpp"
 #include "link_solve.hpp"
 #include "solve32.hpp"
-#include "solve96.hpp"
+#include "solve128.hpp"
 #include "timing.hpp"
 
 namespace fringe {
@@ -61,25 +61,35 @@ static void launch_solve_generic(SolveArgs a, hipStream_t s) {
                        a);
 }
 
-// Mixed precisions, 64 < n <= 96: the register-resident two-wave solver,
-// unless the schedule FRINGE_SCHED_GENERIC_A places a work matrix instead.
-bool solve96_selected(int n, int precision) {
-    if (precision == FRINGE_PREC_FP64 || n <= 64 || n > kC96) return false;
+// Mixed precisions, 64 < n <= 128: the register-row solver (four waves per
+// pixel, half a row per thread), unless the schedule FRINGE_SCHED_GENERIC_A
+// places a work matrix instead.
+bool rows_solver_selected(int n, int precision) {
+    if (precision == FRINGE_PREC_FP64 || n <= 64 || n > kMaxBandsMfma) return false;
     return schedule_lookup(FRINGE_SCHED_GENERIC_A, n, 1, 0, generic_a_default(n, 1)) == 2;
 }
 
-static void launch_solve96(const SolveArgs& a, hipStream_t s) {
-    const dim3 grid((unsigned)a.slots), blk(128);
-    if (a.module == FRINGE_MODULE_PHASE_LINK) hipLaunchKernelGGL(solve96_kernel<MODE_EMI>, grid, blk, 0, s, a);
-    else if (a.method == FRINGE_METHOD_MLE) hipLaunchKernelGGL(solve96_kernel<MODE_MLE>, grid, blk, 0, s, a);
-    else hipLaunchKernelGGL(solve96_kernel<MODE_EVD>, grid, blk, 0, s, a);
+static void launch_rows_solver(const SolveArgs& a, hipStream_t s) {
+    const dim3 grid((unsigned)a.slots), blk(256);
+#define FRINGE_ROWS(CH_)                                                                                     \
+    do {                                                                                                     \
+        if (a.module == FRINGE_MODULE_PHASE_LINK)                                                            \
+            hipLaunchKernelGGL((solve_rows_kernel<CH_, MODE_EMI>), grid, blk, 0, s, a);                      \
+        else if (a.method == FRINGE_METHOD_MLE)                                                              \
+            hipLaunchKernelGGL((solve_rows_kernel<CH_, MODE_MLE>), grid, blk, 0, s, a);                      \
+        else                                                                                                 \
+            hipLaunchKernelGGL((solve_rows_kernel<CH_, MODE_EVD>), grid, blk, 0, s, a);                      \
+    } while (0)
+    if (a.n <= 96) FRINGE_ROWS(48);
+    else FRINGE_ROWS(64);
+#undef FRINGE_ROWS
 }
 
 template <typename T, typename Tg>
 static void dispatch_generic(const SolveArgs& a, hipStream_t s) {
     if constexpr (sizeof(T) == 4 && sizeof(Tg) == 4) {
-        if (!a.awork && solve96_selected(a.n, FRINGE_PREC_MIXED)) {
-            launch_solve96(a, s);
+        if (!a.awork && rows_solver_selected(a.n, FRINGE_PREC_MIXED)) {
+            launch_rows_solver(a, s);
             return;
         }
     }
@@ -474,7 +484,7 @@ int link_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, float
             if (rc) return -1;
         }
         StageScope sc(STAGE_SOLVE, s);
-        if (sched && P.d_queue && !solve96_selected(P.n, P.precision)) {
+        if (sched && P.d_queue && !rows_solver_selected(P.n, P.precision)) {
             PersistArgs pa{};
             pa.module = P.module;
             pa.method = P.method;
@@ -577,7 +587,7 @@ int link_batch_launch(const BatchPlan& B, hipStream_t s) {
         while (item < count && !full) {
             const BatchTile& T = B.tiles[item];
             const LinkPlan& P = T.P;
-            const int cls = solve96_selected(P.n, P.precision) ? 4 : solve_class(P.n);
+            const int cls = rows_solver_selected(P.n, P.precision) ? 4 : solve_class(P.n);
             const long long rem = P.row_end - row;
             if (rem <= 0) {
                 ++item;
@@ -616,7 +626,7 @@ int link_batch_launch(const BatchPlan& B, hipStream_t s) {
             row += rows;
         }
         StageScope sc(STAGE_SOLVE, s);
-        for (int j = 0; j < pa[4].njobs; ++j) launch_solve96(solve_args_of(pa[4].job[j], B.module, B.method), s);
+        for (int j = 0; j < pa[4].njobs; ++j) launch_rows_solver(solve_args_of(pa[4].job[j], B.module, B.method), s);
         for (int c = 0; c < 4; ++c) {
             if (!pa[c].njobs) continue;
             pa[c].module = B.module;

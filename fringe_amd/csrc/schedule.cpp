@@ -10,8 +10,8 @@
 //   that take `grab` units at a time from a queue.
 //   Work matrix above 64 dates, keyed by (date class 96 / 128 / 256,
 //   precision): in LDS where it fits (0), in global memory (1), or, for the
-//   mixed precisions up to 96 dates, no work matrix at all: the two-wave
-//   solver that keeps each row in registers (2).
+//   mixed precisions up to 128 dates, no work matrix at all: the four-wave
+//   solver that keeps half a row per thread in registers (2).
 //
 // Unset keys use the built-in defaults (the heuristics the table replaces).
 #include <map>
@@ -37,7 +37,7 @@ std::tuple<int, int, int, int> key_of(int kind, int k0, int k1, int k2) {
 }  // namespace
 
 int generic_a_default(int n, int precision_key) {
-    return (n > 64 && n <= 96 && precision_key != 0) ? kGenericA96Default : 0;
+    return (n > 64 && n <= 128 && precision_key != 0) ? kGenericARowsDefault : 0;
 }
 
 int schedule_lookup(int kind, int k0, int k1, int k2, int fallback) {
@@ -60,7 +60,7 @@ int fringe_schedule_set(int kind, int key0, int key1, int key2, int value) {
         if (value != 0 && (waves < 1 || waves > 64 || grab < 1)) return FRINGE_ERR_ARGUMENT;
     } else if (kind == FRINGE_SCHED_GENERIC_A) {
         if (value < 0 || value > 2) return FRINGE_ERR_ARGUMENT;
-        if (value == 2 && (key0 > 96 || key1 == 0)) return FRINGE_ERR_ARGUMENT;   // FP32 rows, <= 96 columns
+        if (value == 2 && (key0 > 128 || key1 == 0)) return FRINGE_ERR_ARGUMENT;   // FP32 rows, <= 128 dates
     } else {
         return FRINGE_ERR_ARGUMENT;
     }

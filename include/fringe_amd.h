@@ -168,7 +168,7 @@ long long fringe_link_resolved_pixels(const fringe_link_tile* t, const void* d_w
  * Hermitian solves of one date class (<= 32, <= 64, <= 128, > 128 dates) run
  * in ONE persistent launch that drains a queue of all tiles' pixels, whatever
  * their date count and neighbour count (tiles on the mixed precisions'
- * register-row solver, 65..96 dates, get one grid launch each).  Outputs
+ * register-row solver, 65..128 dates, get one grid launch each).  Outputs
  * equal fringe_link_run's per tile
  * bit for bit.  Returns 202 for a batch that mixes modules, methods or
  * precisions. */
@@ -196,8 +196,8 @@ int fringe_link_batch_run(const fringe_link_batch_item* items, int count, void* 
 #define FRINGE_SCHED_GRAM_VARIANT 0
 #define FRINGE_SCHED_SOLVE 1
 #define FRINGE_SCHED_GENERIC_A 2    /* (nbands class 96/128/256, precision): the solver above 64 dates keeps its
-                                       work matrix in LDS (0) or global memory (1), or (mixed precisions, <= 96
-                                       dates; the default there) each row in a lane's registers (2) */
+                                       work matrix in LDS (0) or global memory (1), or (mixed precisions, <= 128
+                                       dates; the default there) each row in registers, half a row per thread (2) */
 #define FRINGE_GRAM_DEFAULT 0
 #define FRINGE_GRAM_STAGED12 1      /* 12 waves, 8 x 16-pixel tiles staged in LDS */
 #define FRINGE_GRAM_STAGED8 2       /* 8 waves, 4 x 16 */

@@ -46,8 +46,11 @@ def _tiles(specs, module, method, precision):
 
 
 # (n, rows, cols, Nx, Ny, fill); fill None = the SHP map of the stack
+# (100 dates: the MFMA Gram's complex64 coherence into the class of up to
+# 128 dates; 136: the FP64 Gram's complex128 into the class above)
 SPECS = [(10, 24, 40, 5, 3, None), (24, 20, 36, 7, 5, 0.4), (30, 18, 44, 7, 5, None), (17, 16, 30, 3, 2, 0.9),
-         (40, 14, 32, 5, 4, None), (64, 12, 26, 6, 4, 0.7), (80, 12, 22, 7, 5, None)]
+         (40, 14, 32, 5, 4, None), (64, 12, 26, 6, 4, 0.7), (80, 12, 22, 7, 5, None), (100, 10, 18, 5, 4, None),
+         (136, 10, 16, 5, 4, 0.8)]
 
 
 class _Sched:
@@ -107,7 +110,8 @@ def test_batch_across_rounds():
         assert _same(r, g)
 
 
-@pytest.mark.parametrize("n,precision", [(30, "mixed"), (48, "mixed"), (20, "fp64"), (80, "mixed")])
+@pytest.mark.parametrize("n,precision", [(30, "mixed"), (48, "mixed"), (20, "fp64"), (80, "mixed"), (100, "mixed"),
+                                         (136, "mixed"), (100, "fp64")])
 def test_persistent_schedule_equals_grid(n, precision):
     L = _lib.lib()
     rows, cols, Nx, Ny = 40, 64, 7, 5

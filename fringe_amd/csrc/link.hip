@@ -80,7 +80,10 @@ static void launch_rows_solver(const SolveArgs& a, hipStream_t s) {
         else                                                                                                 \
             hipLaunchKernelGGL((solve_rows_kernel<CH_, MODE_EVD>), grid, blk, 0, s, a);                      \
     } while (0)
-    if (a.n <= 96) FRINGE_ROWS(48);
+    // 2 CH columns per row, the fewest that hold n
+    if (a.n <= 80) FRINGE_ROWS(40);
+    else if (a.n <= 96) FRINGE_ROWS(48);
+    else if (a.n <= 112) FRINGE_ROWS(56);
     else FRINGE_ROWS(64);
 #undef FRINGE_ROWS
 }

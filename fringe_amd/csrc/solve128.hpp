@@ -11,8 +11,8 @@
 // DPP and the four waves' partials through alternating LDS slots (one
 // barrier each).  The FP64 inverse of |Gamma| keeps each lane's half row of
 // |Gamma| in CH FP64 registers and applies the symmetric sweep operator one
-// published pivot row at a time.  CH = 48 covers 96 dates at three waves per
-// SIMD, CH = 64 covers 128 dates.
+// published pivot row at a time.  CH = 40 and 48 cover 80 and 96 dates at
+// three waves per SIMD, CH = 56 and 64 cover 112 and 128 dates at two.
 #pragma once
 #include "solve32.hpp"
 
@@ -444,7 +444,7 @@ __device__ __forceinline__ bool hadamard_half(LRows<CH>& L, BRed4& br, cx<float>
 // One pixel per 256-thread workgroup; slot s of the round is pixel first + s.
 // The FP32 result is certified; uncertified pixels go to the FP64 re-solve.
 template <int CH, int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CH == 48 ? 3 : 2)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CH <= 48 ? 3 : 2)))
 void solve_rows_kernel(SolveArgs a) {
     constexpr int C = 2 * CH;
     __shared__ LRows<CH> L;

@@ -13,8 +13,8 @@ stage's launches), and keeps the fastest:
   from the work queue.
 * Above 64 dates, per (date class, precision): the solver's work matrix in
   LDS (where it fits) or in global memory (more pixels in flight), or, for
-  the mixed precisions up to 128 dates, each row in registers (half a row
-  per thread, four waves per pixel).
+  up to 128 dates, each row in registers (two threads per row in FP32, four
+  in FP64).
 
 The chosen table (with every candidate's time) is written as JSON; the
 library installs it when it is loaded (fringe_amd._lib.load_schedules).
@@ -129,7 +129,7 @@ def tune(quick=False, log=sys.stderr):
               file=log, flush=True)
     # ---- above 64 dates: work matrix in LDS or global memory, or registers --
     generic_a = []
-    for n, prec in ((72, "mixed"), (96, "mixed"), (72, "fp64"), (128, "mixed")):
+    for n, prec in ((72, "mixed"), (96, "mixed"), (72, "fp64"), (96, "fp64"), (128, "mixed"), (128, "fp64")):
         if quick and n != 96:
             continue
         slc, w = data(n, 7, 5, fill=0.9)
@@ -138,7 +138,7 @@ def tune(quick=False, log=sys.stderr):
         pk = 0 if prec == "fp64" else 1
         default = L.fringe_schedule_get(_lib.FRINGE_SCHED_GENERIC_A, n, pk, 0)
         times = {}
-        places = ["lds", "global"] + (["registers"] if n <= 128 and prec != "fp64" else [])
+        places = ["lds", "global"] + (["registers"] if n <= 128 else [])
         for place in places:
             v = _lib.GENERIC_A_PLACES[place]
             L.fringe_schedule_set(_lib.FRINGE_SCHED_GENERIC_A, n, pk, 0, v)

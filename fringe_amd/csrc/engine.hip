@@ -181,8 +181,12 @@ int link_check(const fringe_link_tile* t) {
                                                               : solve_group_bytes<float>(t->nbands, false)) +
                             solve_static_lds(t->precision);
         if (rest > lds) return FRINGE_ERR_UNSUPPORTED;
-        if (link_certifies(t->nbands, t->precision) && resolve_generic_lds_bytes(t->nbands, t->Nx, t->Ny) > lds)
-            return FRINGE_ERR_UNSUPPORTED;
+        if (link_certifies(t->nbands, t->precision)) {
+            const size_t r = rows_solver_selected(t->nbands, FRINGE_PREC_FP64)
+                                 ? resolve_rows_lds_bytes(t->nbands, t->Nx, t->Ny)
+                                 : resolve_generic_lds_bytes(t->nbands, t->Nx, t->Ny);
+            if (r > lds) return FRINGE_ERR_UNSUPPORTED;
+        }
     }
     return 0;
 }

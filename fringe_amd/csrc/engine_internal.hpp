@@ -114,12 +114,14 @@ size_t resolve_generic_lds_bytes(int n, int Nx, int Ny);
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 // Value of a schedule key (fringe_schedule_set), or fallback when unset.
 int schedule_lookup(int kind, int k0, int k1, int k2, int fallback);
-// FRINGE_SCHED_GENERIC_A when unset: the register-row solver (2) for the
-// mixed precisions at 65..128 dates, the LDS work matrix (0) otherwise.
+// FRINGE_SCHED_GENERIC_A when unset: the register-row solver (2) at
+// 65..128 dates, the LDS work matrix (0) otherwise.
 constexpr int kGenericARowsDefault = 2;
 int generic_a_default(int n, int precision_key);
-// Mixed precisions at 65..128 dates under FRINGE_SCHED_GENERIC_A = 2.
+// 65..128 dates under FRINGE_SCHED_GENERIC_A = 2 (either precision; the
+// FP64 re-solve follows the FP64 choice, whose outputs it must equal).
 bool rows_solver_selected(int n, int precision);
+size_t resolve_rows_lds_bytes(int n, int Nx, int Ny);
 // Estimator index of the solver schedules: 0 phase_link EMI, 1 evd EVD/STBAS, 2 evd MLE.
 inline int solve_mode_index(int module, int method) {
     return module == FRINGE_MODULE_PHASE_LINK ? 0 : (method == FRINGE_METHOD_MLE ? 2 : 1);

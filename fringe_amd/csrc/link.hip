@@ -61,38 +61,25 @@ static void launch_solve_generic(SolveArgs a, hipStream_t s) {
                        a);
 }
 
-// Mixed precisions, 64 < n <= 128: the register-row solver (four waves per
-// pixel, half a row per thread), unless the schedule FRINGE_SCHED_GENERIC_A
-// places a work matrix instead.
+// 64 < n <= 128: the register-row solver (FP32 with two threads per row in
+// the mixed precisions, FP64 with four in the plain path and the re-solve),
+// unless the schedule FRINGE_SCHED_GENERIC_A places a work matrix instead.
 bool rows_solver_selected(int n, int precision) {
-    if (precision == FRINGE_PREC_FP64 || n <= 64 || n > kMaxBandsMfma) return false;
-    return schedule_lookup(FRINGE_SCHED_GENERIC_A, n, 1, 0, generic_a_default(n, 1)) == 2;
+    if (n <= 64 || n > kMaxBandsMfma) return false;
+    const int pk = precision == FRINGE_PREC_FP64 ? 0 : 1;
+    return schedule_lookup(FRINGE_SCHED_GENERIC_A, n, pk, 0, generic_a_default(n, pk)) == 2;
 }
 
-static void launch_rows_solver(const SolveArgs& a, hipStream_t s) {
-    const dim3 grid((unsigned)a.slots), blk(256);
-#define FRINGE_ROWS(CH_)                                                                                     \
-    do {                                                                                                     \
-        if (a.module == FRINGE_MODULE_PHASE_LINK)                                                            \
-            hipLaunchKernelGGL((solve_rows_kernel<CH_, MODE_EMI>), grid, blk, 0, s, a);                      \
-        else if (a.method == FRINGE_METHOD_MLE)                                                              \
-            hipLaunchKernelGGL((solve_rows_kernel<CH_, MODE_MLE>), grid, blk, 0, s, a);                      \
-        else                                                                                                 \
-            hipLaunchKernelGGL((solve_rows_kernel<CH_, MODE_EVD>), grid, blk, 0, s, a);                      \
-    } while (0)
-    // 2 CH columns per row, the fewest that hold n
-    if (a.n <= 80) FRINGE_ROWS(40);
-    else if (a.n <= 96) FRINGE_ROWS(48);
-    else if (a.n <= 112) FRINGE_ROWS(56);
-    else FRINGE_ROWS(64);
-#undef FRINGE_ROWS
+static void launch_rows_solver(const SolveArgs& a, bool fp64, hipStream_t s) {
+    if (fp64) launch_rows_solver_f64(a, s);
+    else launch_rows_solver_f32(a, s);
 }
 
 template <typename T, typename Tg>
 static void dispatch_generic(const SolveArgs& a, hipStream_t s) {
-    if constexpr (sizeof(T) == 4 && sizeof(Tg) == 4) {
-        if (!a.awork && rows_solver_selected(a.n, FRINGE_PREC_MIXED)) {
-            launch_rows_solver(a, s);
+    if constexpr (sizeof(T) == sizeof(Tg)) {   // FP32 on the MFMA Gram's coherence, or FP64
+        if (!a.awork && rows_solver_selected(a.n, sizeof(T) == 8 ? FRINGE_PREC_FP64 : FRINGE_PREC_MIXED)) {
+            launch_rows_solver(a, sizeof(T) == 8, s);
             return;
         }
     }
@@ -118,6 +105,7 @@ size_t solve_static_lds(int precision) {
 // The schedule FRINGE_SCHED_GENERIC_A may move it to global memory even
 // when it fits: the pixels in flight are then limited by registers, not LDS.
 bool solve_a_in_lds(int n, int precision) {
+    if (rows_solver_selected(n, precision)) return true;   // no work matrix at all
     if (schedule_lookup(FRINGE_SCHED_GENERIC_A, n, precision == FRINGE_PREC_FP64 ? 0 : 1, 0, 0) == 1) return false;
     const size_t b = precision == FRINGE_PREC_FP64 ? solve_group_bytes<double>(n) : solve_group_bytes<float>(n);
     return b + solve_static_lds(precision) <= (size_t)device_lds_limit();
@@ -226,7 +214,12 @@ static void launch_resolve_generic(const LinkPlan& P, const uint32_t* wts, Solve
     a.awork_stride = resolve_generic_area_bytes(P.n);
     a.group_bytes = solve_group_bytes<double>(P.n, false);
     const size_t lds = a.group_bytes + 2 * gram_f64_group_bytes(P.n, P.Nx, P.Ny, kResolveCB);
-    const dim3 grid((unsigned)(P.resolve_blocks ? P.resolve_blocks : resolve_generic_blocks(a.awork_stride)));
+    const unsigned blocks = (unsigned)(P.resolve_blocks ? P.resolve_blocks : resolve_generic_blocks(a.awork_stride));
+    const dim3 grid(blocks);
+    if (rows_solver_selected(P.n, FRINGE_PREC_FP64)) {   // the plain FP64 path's solver
+        launch_resolve_rows(a, blocks, s);
+        return;
+    }
     if (P.n <= 128)
         hipLaunchKernelGGL((solve_kernel<double, double, 128, 1, true>), grid, dim3(128), lds, s, a);
     else
@@ -629,7 +622,8 @@ int link_batch_launch(const BatchPlan& B, hipStream_t s) {
             row += rows;
         }
         StageScope sc(STAGE_SOLVE, s);
-        for (int j = 0; j < pa[4].njobs; ++j) launch_rows_solver(solve_args_of(pa[4].job[j], B.module, B.method), s);
+        for (int j = 0; j < pa[4].njobs; ++j)
+            launch_rows_solver(solve_args_of(pa[4].job[j], B.module, B.method), B.precision == FRINGE_PREC_FP64, s);
         for (int c = 0; c < 4; ++c) {
             if (!pa[c].njobs) continue;
             pa[c].module = B.module;

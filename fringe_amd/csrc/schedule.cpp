@@ -9,9 +9,9 @@
 //   one wave per unit, or a persistent grid of `waves per CU` x CUs waves
 //   that take `grab` units at a time from a queue.
 //   Work matrix above 64 dates, keyed by (date class 96 / 128 / 256,
-//   precision): in LDS where it fits (0), in global memory (1), or, for the
-//   mixed precisions up to 128 dates, no work matrix at all: the four-wave
-//   solver that keeps half a row per thread in registers (2).
+//   precision): in LDS where it fits (0), in global memory (1), or, up to
+//   128 dates, no work matrix at all: the solver that keeps each row in
+//   registers, two (FP32) or four (FP64) threads per row (2).
 //
 // Unset keys use the built-in defaults (the heuristics the table replaces).
 #include <map>
@@ -37,7 +37,8 @@ std::tuple<int, int, int, int> key_of(int kind, int k0, int k1, int k2) {
 }  // namespace
 
 int generic_a_default(int n, int precision_key) {
-    return (n > 64 && n <= 128 && precision_key != 0) ? kGenericARowsDefault : 0;
+    (void)precision_key;
+    return (n > 64 && n <= 128) ? kGenericARowsDefault : 0;
 }
 
 int schedule_lookup(int kind, int k0, int k1, int k2, int fallback) {
@@ -60,7 +61,7 @@ int fringe_schedule_set(int kind, int key0, int key1, int key2, int value) {
         if (value != 0 && (waves < 1 || waves > 64 || grab < 1)) return FRINGE_ERR_ARGUMENT;
     } else if (kind == FRINGE_SCHED_GENERIC_A) {
         if (value < 0 || value > 2) return FRINGE_ERR_ARGUMENT;
-        if (value == 2 && (key0 > 128 || key1 == 0)) return FRINGE_ERR_ARGUMENT;   // FP32 rows, <= 128 dates
+        if (value == 2 && key0 > 128) return FRINGE_ERR_ARGUMENT;   // rows in registers: <= 128 dates
     } else {
         return FRINGE_ERR_ARGUMENT;
     }

@@ -1,56 +1,81 @@
-// solve128.hpp -- register-resident estimator chain for 64 < n <= 128 dates
-// in FP32 (the mixed precisions), one pixel per four-wave workgroup.
+// solve128.hpp -- register-resident estimator chain for 64 < n <= 128 dates,
+// one pixel per workgroup of Q threads per row.
 //
-// Thread t (0..255) holds half of row r = t / 2 of its pixel's matrices:
-// the columns c = 2k + h (k < CH, h = t % 2), interleaved so that the live
-// columns of every Householder step are split evenly between the two lanes
+// Thread t holds a 1/Q share of row r = t / Q of its pixel's matrices: the
+// columns c = Q k + q (k < CH, q = t % Q), interleaved so that the live
+// columns of every Householder step are split evenly between the Q threads
 // of a row.  Values that belong to a row (the reflector entry, the product
-// p = H v, the eigenvector entry) are held by both lanes of the row,
-// bitwise alike: a lane pair combines its halves with one DPP swap, and
-// a + b == b + a.  Sums over the rows reduce the even lanes of each wave by
-// DPP and the four waves' partials through alternating LDS slots (one
-// barrier each).  The FP64 inverse of |Gamma| keeps each lane's half row of
-// |Gamma| in CH FP64 registers and applies the symmetric sweep operator one
-// published pivot row at a time.  CH = 40 and 48 cover 80 and 96 dates at
-// three waves per SIMD, CH = 56 and 64 cover 112 and 128 dates at two.
+// p = H v, the eigenvector entry) are held by all Q threads of the row,
+// bitwise alike: the threads combine their shares by DPP swaps within a
+// quad, and each stage adds two equal partials commutatively.  Sums over the
+// rows reduce the q = 0 threads of each wave by DPP and the waves' partials
+// through alternating LDS slots (one barrier each).  The FP64 inverse of
+// |Gamma| keeps each thread's share of its row of |Gamma| in CH FP64
+// registers and applies the symmetric sweep operator one published pivot row
+// at a time.
+//
+//   T = float, Q = 2 (four waves): the mixed precisions, coherence from the
+//   MFMA Gram, FP32 results certified (uncertified pixels are listed for the
+//   FP64 re-solve).  CH = 40, 48: three waves per SIMD; 56, 64: two.
+//   T = double, Q = 4 (eight waves): the plain FP64 path, and the FP64
+//   re-solve of the listed pixels (each workgroup computes the listed pixel's
+//   FP64 coherence into its global area first), so that re-solved pixels
+//   equal the FP64 path bit for bit.  CH = 20 .. 32.
 #pragma once
+#include "gram64.hpp"
 #include "solve32.hpp"
 
 namespace fringe {
 
-template <int CH> struct LRows {
-    static constexpr int C = 2 * CH;   // columns (dates) held
-    cx<float> R[C];     // published row
-    cx<float> V[C];
-    cx<float> W[C];
-    cx<float> TAU[C];
+template <typename T, int Q, int CH> struct LRows {
+    static constexpr int C = Q * CH;   // columns (dates) held
+    static constexpr int NW = 2 * Q;   // waves (128 rows of Q threads)
+    cx<T> R[C];         // published row
+    cx<T> V[C];
+    cx<T> W[C];
+    cx<T> TAU[C];
     cx<float> OUT[C];
-    cx<float> YREF;
-    float D[C], E[C], E2[C];
-    float B[C];
-    float FP[C], FQ[C], FS[C], FX[C], FF[C], FK[C];
+    cx<T> YREF;
+    T D[C], E[C], E2[C];
+    T B[C];
+    T FP[C], FQ[C], FS[C], FX[C], FF[C], FK[C];
     double TR[C];       // sweep: published pivot row
     double PV;
-    double red[2][4][2];   // [slot][wave][value]: block reduction partials
+    double red[2][NW][2];   // [slot][wave][value]: block reduction partials
 };
 
-struct BRed4 {
+struct BRedN {
     int seq = 0;
 };
 
-// ---- lane pairs and reductions over the four waves ------------------------
-__device__ __forceinline__ float pswap(float x) { return __uint_as_float(dpp32<0xB1>(__float_as_uint(x))); }
-__device__ __forceinline__ cx<float> pswap(cx<float> x) { return cx<float>{pswap(x.re), pswap(x.im)}; }
-__device__ __forceinline__ cx<float> psum(cx<float> x) { return cadd(x, pswap(x)); }
+// ---- the Q threads of a row and reductions over the waves ------------------
+template <int CTRL> __device__ __forceinline__ float dppf(float x) { return __uint_as_float(dpp32<CTRL>(__float_as_uint(x))); }
+template <int CTRL> __device__ __forceinline__ double dppf(double x) { return dpp64<CTRL>(x); }
+// sum of the row's Q shares, the same bits in every thread of the row
+template <int Q, typename T> __device__ __forceinline__ T qsum(T x) {
+    x = x + dppf<0xB1>(x);                       // quad_perm [1,0,3,2]
+    if constexpr (Q == 4) x = x + dppf<0x4E>(x);  // quad_perm [2,3,0,1]
+    return x;
+}
+template <int Q, typename T> __device__ __forceinline__ cx<T> qsum(cx<T> x) {
+    return cx<T>{qsum<Q>(x.re), qsum<Q>(x.im)};
+}
 
-template <typename Op, typename T, int CH> __device__ __forceinline__ T bred4(LRows<CH>& L, BRed4& br, T x) {
+template <typename Op, typename T, typename S, int Q, int CH>
+__device__ __forceinline__ S bredn(LRows<T, Q, CH>& L, BRedN& br, S x) {
+    constexpr int NW = 2 * Q;
     x = redR<Op, 64>(x);
     const int s = br.seq++ & 1, w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) L.red[s][w][0] = (double)x;
     __syncthreads();
-    return Op::f(Op::f((T)L.red[s][0][0], (T)L.red[s][1][0]), Op::f((T)L.red[s][2][0], (T)L.red[s][3][0]));
+    S a = (S)L.red[s][0][0];
+#pragma unroll
+    for (int q = 1; q < NW; ++q) a = Op::f(a, (S)L.red[s][q][0]);
+    return a;
 }
-template <typename T, int CH> __device__ __forceinline__ cx<T> bsum4(LRows<CH>& L, BRed4& br, cx<T> x) {
+template <typename T, typename S, int Q, int CH>
+__device__ __forceinline__ cx<S> bsumn(LRows<T, Q, CH>& L, BRedN& br, cx<S> x) {
+    constexpr int NW = 2 * Q;
     x.re = redR<OpAdd, 64>(x.re);
     x.im = redR<OpAdd, 64>(x.im);
     const int s = br.seq++ & 1, w = threadIdx.x >> 6;
@@ -59,39 +84,50 @@ template <typename T, int CH> __device__ __forceinline__ cx<T> bsum4(LRows<CH>& 
         L.red[s][w][1] = (double)x.im;
     }
     __syncthreads();
-    return cx<T>{(T)((L.red[s][0][0] + L.red[s][1][0]) + (L.red[s][2][0] + L.red[s][3][0])),
-                 (T)((L.red[s][0][1] + L.red[s][1][1]) + (L.red[s][2][1] + L.red[s][3][1]))};
+    double re = L.red[s][0][0], im = L.red[s][0][1];
+#pragma unroll
+    for (int q = 1; q < NW; ++q) {
+        re += L.red[s][q][0];
+        im += L.red[s][q][1];
+    }
+    return cx<S>{(S)re, (S)im};
 }
-// Index of the first thread (0..255) whose predicate holds, or 256.
-template <int CH> __device__ __forceinline__ int bfirst4(LRows<CH>& L, BRed4& br, bool p) {
+// Index of the first thread whose predicate holds, or 64 NW.
+template <typename T, int Q, int CH> __device__ __forceinline__ int bfirstn(LRows<T, Q, CH>& L, BRedN& br, bool p) {
+    constexpr int NW = 2 * Q;
     const uint64_t b = __ballot(p);
     const int s = br.seq++ & 1, w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) L.red[s][w][0] = __builtin_bit_cast(double, b);
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NW; ++q) {
         const uint64_t bq = __builtin_bit_cast(uint64_t, L.red[s][q][0]);
         if (bq) return 64 * q + (int)__builtin_ctzll(bq);
     }
-    return 256;
+    return 64 * NW;
 }
 
-// ---- FP64 inverse of |Gamma| - shift*I, half a row per thread -------------
-// m[k] = |Gamma_{r,2k+h}| (off the diagonal), 1 - shift on the diagonal,
+template <typename T> __device__ __forceinline__ T sqrt_t(T x) {
+    if constexpr (sizeof(T) == 4) return __builtin_amdgcn_sqrtf(x);   // one v_sqrt_f32 (~1 ulp)
+    else return sqrt(x);
+}
+
+// ---- FP64 inverse of |Gamma| - shift*I, a share of a row per thread --------
+// m[k] = |Gamma_{r,Qk+q}| (off the diagonal), 1 - shift on the diagonal,
 // identity outside n.
-template <int CH>
-__device__ __forceinline__ bool abs_gamma_half(double (&m)[CH], const cx<float>* grow, int n, int r, int hf, bool act,
-                                               double shift) {
+template <int Q, int CH, typename Tg>
+__device__ __forceinline__ bool abs_gamma_share(double (&m)[CH], const cx<Tg>* grow, int n, int r, int q, bool act,
+                                                double shift) {
     bool fin = true;
 #pragma unroll
     for (int k0 = 0; k0 < CH; k0 += 16) {
 #pragma unroll
         for (int k = k0; k < (k0 + 16 < CH ? k0 + 16 : CH); ++k) {
-            const int c = 2 * k + hf;
+            const int c = Q * k + q;
             double v = (c == r) ? 1.0 - shift : 0.0;
             if (act && r < n && c < n && c != r) {
-                const cx<float> g = grow[c];
-                v = (double)__builtin_amdgcn_sqrtf(g.re * g.re + g.im * g.im);   // one v_sqrt_f32 (~1 ulp)
+                const cx<Tg> g = grow[c];
+                v = (double)sqrt_t<Tg>(g.re * g.re + g.im * g.im);
                 fin = fin && isfinite(v);
             }
             m[k] = v;
@@ -107,15 +143,15 @@ __device__ __forceinline__ bool abs_gamma_half(double (&m)[CH], const cx<float>*
 // read it, so subtracting 2 from each swept diagonal at the end gives
 // -A^-1.  Returns whether every pivot was positive; pmin receives the
 // smallest |pivot| up to the first one that is not.
-template <int CH>
-__device__ __forceinline__ bool sweep_half(LRows<CH>& L, double (&m)[CH], int n, int r, int hf, double& pmin) {
-    constexpr int C = 2 * CH;
+template <typename T, int Q, int CH>
+__device__ __forceinline__ bool sweep_share(LRows<T, Q, CH>& L, double (&m)[CH], int n, int r, int q, double& pmin) {
+    constexpr int C = Q * CH;
     bool ok = true;
     pmin = 1.0;
     for (int k = 0; k < n; ++k) {
         if (r == k) {
 #pragma unroll
-            for (int j = 0; j < CH; ++j) L.TR[2 * j + hf] = m[j];
+            for (int j = 0; j < CH; ++j) L.TR[Q * j + q] = m[j];
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -132,182 +168,188 @@ __device__ __forceinline__ bool sweep_half(LRows<CH>& L, double (&m)[CH], int n,
 #pragma unroll
         for (int j0 = 0; j0 < CH; j0 += 16) {
 #pragma unroll
-            for (int j = j0; j < (j0 + 16 < CH ? j0 + 16 : CH); ++j) m[j] = fma(-g, L.TR[2 * j + hf], m[j]);
+            for (int j = j0; j < (j0 + 16 < CH ? j0 + 16 : CH); ++j) m[j] = fma(-g, L.TR[Q * j + q], m[j]);
             asm volatile("" ::: "memory");
         }
         __syncthreads();   // every thread has read TR before the next pivot row replaces it
     }
 #pragma unroll
     for (int j = 0; j < CH; ++j)
-        if (2 * j + hf == r && r < n) m[j] -= 2.0;
+        if (Q * j + q == r && r < n) m[j] -= 2.0;
     return ok;
 }
 
 // ---- Householder tridiagonalisation of the rows in h ----------------------
-// Step j, for 2 K0 - 1 <= j <= 2 K0 + 6: local columns k < K0 (global
-// columns below 2 K0) hold only finished entries and parked reflectors.
-template <int CH, int K0>
-__device__ __forceinline__ void tri_step_half(LRows<CH>& L, BRed4& br, cx<float> (&h)[CH], int j, int n, int r,
-                                              int hf) {
-    constexpr int C = 2 * CH;
+// Step j, for Q K0 - 1 <= j <= Q K0 + 6: local columns k < K0 (global columns
+// below Q K0, a multiple of 8) hold only finished entries and parked
+// reflectors.
+template <typename T, int Q, int CH, int K0>
+__device__ __forceinline__ void tri_step_share(LRows<T, Q, CH>& L, BRedN& br, cx<T> (&h)[CH], int j, int n, int r,
+                                               int q) {
+    constexpr int C = Q * CH;
+    constexpr int KB = 8 / Q;   // local columns per 8 global ones
     const int rc = r < C ? r : C - 1;
-    const cx<float> alpha = cconj(L.R[j + 1]);
-    const cx<float> xr = (r > j + 1 && r < n) ? cconj(L.R[rc]) : cx<float>{0.0f, 0.0f};
-    const float xn2 = bred4<OpAdd>(L, br, hf ? 0.0f : cabs2(xr));
-    const float ar = alpha.re, ai = alpha.im;
-    cx<float> tau = {0.0f, 0.0f}, scal = {0.0f, 0.0f};
-    float beta = ar;
-    if (xn2 != 0.0f || ai != 0.0f) {
-        const float nrm = __builtin_amdgcn_sqrtf(ar * ar + ai * ai + xn2);
-        beta = (ar >= 0.0f) ? -nrm : nrm;
-        const float ib = fast_rcp(beta);
-        tau = cx<float>{(beta - ar) * ib, -ai * ib};
-        const float dr_ = ar - beta;
-        const float id = fast_rcp(dr_ * dr_ + ai * ai);
-        scal = cx<float>{dr_ * id, -ai * id};
+    const cx<T> alpha = cconj(L.R[j + 1]);
+    const cx<T> xr = (r > j + 1 && r < n) ? cconj(L.R[rc]) : cx<T>{T(0), T(0)};
+    const T xn2 = bredn<OpAdd>(L, br, q ? T(0) : cabs2(xr));
+    const T ar = alpha.re, ai = alpha.im;
+    cx<T> tau = {T(0), T(0)}, scal = {T(0), T(0)};
+    T beta = ar;
+    if (xn2 != T(0) || ai != T(0)) {
+        const T nrm = sqrt_t<T>(ar * ar + ai * ai + xn2);
+        beta = (ar >= T(0)) ? -nrm : nrm;
+        const T ib = fast_rcp(beta);
+        tau = cx<T>{(beta - ar) * ib, -ai * ib};
+        const T dr_ = ar - beta;
+        const T id = fast_rcp(dr_ * dr_ + ai * ai);
+        scal = cx<T>{dr_ * id, -ai * id};
     }
-    const cx<float> vr = (r == j + 1) ? cx<float>{1.0f, 0.0f}
-                         : (r > j + 1 && r < n) ? cmul(xr, scal) : cx<float>{0.0f, 0.0f};
-    if (hf == 0 && r < C) L.V[r] = vr;
+    const cx<T> vr = (r == j + 1) ? cx<T>{T(1), T(0)} : (r > j + 1 && r < n) ? cmul(xr, scal) : cx<T>{T(0), T(0)};
+    if (q == 0 && r < C) L.V[r] = vr;
     if (threadIdx.x == 0) {
         L.D[j] = L.R[j].re;
         L.E[j] = beta;
         L.TAU[j] = tau;
     }
     __syncthreads();
-    cx<float> p = {0.0f, 0.0f};
+    cx<T> p = {T(0), T(0)};
 #pragma unroll
-    for (int k0 = (K0 / 8) * 8; k0 < CH; k0 += 8) {
+    for (int k0 = (K0 / (2 * KB)) * (2 * KB); k0 < CH; k0 += 2 * KB) {
 #pragma unroll
-        for (int k = (k0 > K0 ? k0 : K0); k < (k0 + 8 < CH ? k0 + 8 : CH); ++k) cfma(p, h[k], L.V[2 * k + hf]);
+        for (int k = (k0 > K0 ? k0 : K0); k < (k0 + 2 * KB < CH ? k0 + 2 * KB : CH); ++k)
+            cfma(p, h[k], L.V[Q * k + q]);
         asm volatile("" ::: "memory");
     }
-    p = cmul(tau, psum(p));
-    const cx<float> s = bsum4(L, br, hf ? cx<float>{0.0f, 0.0f} : cmulc(vr, p));
-    const cx<float> a2 = cscale(cmul(tau, s), -0.5f);
-    const cx<float> wr = (r > j && r < n) ? cadd(p, cmul(a2, vr)) : cx<float>{0.0f, 0.0f};
-    if (hf == 0 && r < C) L.W[r] = wr;
+    p = cmul(tau, qsum<Q>(p));
+    const cx<T> s = bsumn(L, br, q ? cx<T>{T(0), T(0)} : cmulc(vr, p));
+    const cx<T> a2 = cscale(cmul(tau, s), T(-0.5));
+    const cx<T> wr = (r > j && r < n) ? cadd(p, cmul(a2, vr)) : cx<T>{T(0), T(0)};
+    if (q == 0 && r < C) L.W[r] = wr;
     __syncthreads();
 #pragma unroll
-    for (int k0 = (K0 / 8) * 8; k0 < CH; k0 += 8) {
+    for (int k0 = (K0 / (2 * KB)) * (2 * KB); k0 < CH; k0 += 2 * KB) {
 #pragma unroll
-        for (int k = (k0 > K0 ? k0 : K0); k < (k0 + 8 < CH ? k0 + 8 : CH); ++k) {
-            cfms_c(h[k], vr, L.W[2 * k + hf]);
-            cfms_c(h[k], wr, L.V[2 * k + hf]);
+        for (int k = (k0 > K0 ? k0 : K0); k < (k0 + 2 * KB < CH ? k0 + 2 * KB : CH); ++k) {
+            cfms_c(h[k], vr, L.W[Q * k + q]);
+            cfms_c(h[k], wr, L.V[Q * k + q]);
         }
         asm volatile("" ::: "memory");
     }
     // park this row's entry of v_j in the column just annihilated (held by
-    // the lane of parity j % 2, at k = j / 2)
+    // the thread q = j % Q, at k = j / Q)
 #pragma unroll
-    for (int k = (K0 > 0 ? K0 - 1 : 0); k < (K0 + 4 < CH ? K0 + 4 : CH); ++k)
-        if (2 * k + hf == j && r > j + 1) h[k] = vr;
+    for (int k = (K0 > 0 ? K0 - 1 : 0); k < (K0 + KB < CH ? K0 + KB : CH); ++k)
+        if (Q * k + q == j && r > j + 1) h[k] = vr;
     if (r == j + 1) {
 #pragma unroll
-        for (int k = 0; k < CH; ++k) L.R[2 * k + hf] = h[k];
+        for (int k = 0; k < CH; ++k) L.R[Q * k + q] = h[k];
     }
     __syncthreads();
 }
 
-template <int CH, int K0>
-__device__ __forceinline__ void tri_loop(LRows<CH>& L, BRed4& br, cx<float> (&h)[CH], int& j, int n, int r, int hf) {
-    const int jend = (2 * K0 + 7 < n - 1) ? 2 * K0 + 7 : n - 1;
-    for (; j < jend; ++j) tri_step_half<CH, K0>(L, br, h, j, n, r, hf);
-    if constexpr (K0 + 4 < CH) tri_loop<CH, K0 + 4>(L, br, h, j, n, r, hf);
+template <typename T, int Q, int CH, int K0>
+__device__ __forceinline__ void tri_loop(LRows<T, Q, CH>& L, BRedN& br, cx<T> (&h)[CH], int& j, int n, int r, int q) {
+    constexpr int KB = 8 / Q;
+    const int jend = (Q * K0 + 7 < n - 1) ? Q * K0 + 7 : n - 1;
+    for (; j < jend; ++j) tri_step_share<T, Q, CH, K0>(L, br, h, j, n, r, q);
+    if constexpr (K0 + KB < CH) tri_loop<T, Q, CH, K0 + KB>(L, br, h, j, n, r, q);
 }
 
-template <int CH>
-__device__ __forceinline__ void tridiag_half(LRows<CH>& L, BRed4& br, cx<float> (&h)[CH], int n, int r, int hf) {
+template <typename T, int Q, int CH>
+__device__ __forceinline__ void tridiag_share(LRows<T, Q, CH>& L, BRedN& br, cx<T> (&h)[CH], int n, int r, int q) {
     if (r == 0) {
 #pragma unroll
-        for (int k = 0; k < CH; ++k) L.R[2 * k + hf] = h[k];
+        for (int k = 0; k < CH; ++k) L.R[Q * k + q] = h[k];
     }
     __syncthreads();
     int j = 0;
-    tri_loop<CH, 0>(L, br, h, j, n, r, hf);
+    tri_loop<T, Q, CH, 0>(L, br, h, j, n, r, q);
     if (threadIdx.x == 0) L.D[n - 1] = L.R[n - 1].re;
     __syncthreads();
 }
 
 // Sturm count of tridiag(L.D, L.E2) below sig.
-template <int CH> __device__ __forceinline__ int sturm_below(const LRows<CH>& L, int n, float sig, float pivmin) {
-    float q = L.D[0] - sig;
-    if (fabsf(q) < pivmin) q = -pivmin;
-    int cnt = q < 0.0f;
+template <typename T, int Q, int CH>
+__device__ __forceinline__ int sturm_below(const LRows<T, Q, CH>& L, int n, T sig, T pivmin) {
+    T x = L.D[0] - sig;
+    if (fabs(x) < pivmin) x = -pivmin;
+    int cnt = x < T(0);
     for (int i = 1; i < n; ++i) {
-        q = (L.D[i] - sig) - L.E2[i - 1] * fast_rcp(q);
-        if (fabsf(q) < pivmin) q = -pivmin;
-        cnt += q < 0.0f;
+        x = (L.D[i] - sig) - L.E2[i - 1] * fast_rcp(x);
+        if (fabs(x) < pivmin) x = -pivmin;
+        cnt += x < T(0);
     }
     return cnt;
 }
 
-// ---- eigenvalue k (0-based ascending) of tridiag(D, E): 257-section -------
-template <int CH>
-__device__ __forceinline__ float bisect_half(LRows<CH>& L, BRed4& br, int n, int k, int r, int hf, float& tnorm,
-                                             float& pm) {
-    float lo = 3.0e38f, hi = -3.0e38f, em2 = 0.0f, tn = 0.0f;
-    if (hf == 0 && r < n) {
-        const float a = (r > 0) ? fabsf(L.E[r - 1]) : 0.0f;
-        const float b = (r < n - 1) ? fabsf(L.E[r]) : 0.0f;
+// ---- eigenvalue k (0-based ascending) of tridiag(D, E): (128 Q + 1)-section
+template <typename T, int Q, int CH>
+__device__ __forceinline__ T bisect_share(LRows<T, Q, CH>& L, BRedN& br, int n, int k, int r, int q, T& tnorm,
+                                          T& pm) {
+    constexpr int NT = 128 * Q;   // threads
+    T lo = T(3.0e38), hi = T(-3.0e38), em2 = T(0), tn = T(0);
+    if (q == 0 && r < n) {
+        const T a = (r > 0) ? fabs(L.E[r - 1]) : T(0);
+        const T b = (r < n - 1) ? fabs(L.E[r]) : T(0);
         lo = L.D[r] - a - b;
         hi = L.D[r] + a + b;
-        tn = fabsf(L.D[r]) + a + b;
+        tn = fabs(L.D[r]) + a + b;
         if (r < n - 1) {
             L.E2[r] = L.E[r] * L.E[r];
             em2 = L.E2[r];
         }
     }
-    lo = bred4<OpMin>(L, br, lo);
-    hi = bred4<OpMax>(L, br, hi);
-    em2 = bred4<OpMax>(L, br, em2);
-    tn = bred4<OpMax>(L, br, tn);
+    lo = bredn<OpMin>(L, br, lo);
+    hi = bredn<OpMax>(L, br, hi);
+    em2 = bredn<OpMax>(L, br, em2);
+    tn = bredn<OpMax>(L, br, tn);
     tnorm = tn;
-    const float pivmin = Lim<float>::tiny() * max(1.0f, em2);
+    const T pivmin = Lim<T>::tiny() * max(T(1), em2);
     pm = pivmin;
-    const float pad = 4.0f * Lim<float>::eps() * tn + 2.0f * pivmin;
+    const T pad = T(4) * Lim<T>::eps() * tn + T(2) * pivmin;
     lo -= pad;
     hi += pad;
-    // four rounds of 257-section narrow the Gershgorin interval 4.4e9-fold
+    // 257-section, four rounds, narrows the Gershgorin interval 4.4e9-fold
+    // (FP32); 513-section, seven rounds, 9.4e18-fold (FP64)
+    constexpr int rounds = sizeof(T) == 4 ? 4 : 7;
     const int t = threadIdx.x;
-    for (int it = 0; it < 4; ++it) {
-        const float w = (hi - lo) * (1.0f / 257.0f);
-        const float sig = lo + w * (float)(t + 1);
+    for (int it = 0; it < rounds; ++it) {
+        const T w = (hi - lo) * (T(1) / T(NT + 1));
+        const T sig = lo + w * (T)(t + 1);
         const int cnt = sturm_below(L, n, sig, pivmin);
-        const int f = bfirst4(L, br, cnt > k);
-        const float nhi = (f < 256) ? lo + w * (float)(f + 1) : hi;
-        const float nlo = (f > 0) ? lo + w * (float)f : lo;
+        const int f = bfirstn(L, br, cnt > k);
+        const T nhi = (f < NT) ? lo + w * (T)(f + 1) : hi;
+        const T nlo = (f > 0) ? lo + w * (T)f : lo;
         lo = nlo;
         hi = nhi;
     }
-    return 0.5f * (lo + hi);
+    return T(0.5) * (lo + hi);
 }
 
 // ---- eigenvector of tridiag(D, E) for lam: two rounds of inverse iteration
-template <int CH>
-__device__ __forceinline__ float tri_eigvec_half(LRows<CH>& L, BRed4& br, int n, int r, int hf, float lam,
-                                                 float tnorm) {
-    constexpr int C = 2 * CH;
-    const float pert = max(Lim<float>::eps() * tnorm, Lim<float>::tiny());
-    if (hf == 0 && r < C) L.B[r] = 0.5f + (float)((r * 37 + 11) & 63) * (1.0f / 64.0f);
+template <typename T, int Q, int CH>
+__device__ __forceinline__ T tri_eigvec_share(LRows<T, Q, CH>& L, BRedN& br, int n, int r, int q, T lam, T tnorm) {
+    constexpr int C = Q * CH;
+    const T pert = max(Lim<T>::eps() * tnorm, Lim<T>::tiny());
+    if (q == 0 && r < C) L.B[r] = T(0.5) + (T)((r * 37 + 11) & 63) * (T(1) / T(64));
     __syncthreads();
-    float xme = 0.0f;
-    float cd = L.D[0] - lam;
+    T xme = T(0);
+    T cd = L.D[0] - lam;
     for (int round = 0; round < 2; ++round) {
-        float cx_ = L.B[0];
+        T cx_ = L.B[0];
         if (round == 0) {
-            float cu = (n > 1) ? L.E[0] : 0.0f;
+            T cu = (n > 1) ? L.E[0] : T(0);
             for (int i = 0; i < n - 1; ++i) {
-                const float e = L.E[i];
-                const float dn = L.D[i + 1] - lam;
-                const float un = (i + 1 < n - 1) ? L.E[i + 1] : 0.0f;
-                const float bn = L.B[i + 1];
-                const bool keep = fabsf(cd) >= fabsf(e);
-                float pv, qv, sv, xv, ncd, ncu, ncx, f;
+                const T e = L.E[i];
+                const T dn = L.D[i + 1] - lam;
+                const T un = (i + 1 < n - 1) ? L.E[i + 1] : T(0);
+                const T bn = L.B[i + 1];
+                const bool keep = fabs(cd) >= fabs(e);
+                T pv, qv, sv, xv, ncd, ncu, ncx, f;
                 if (keep) {
-                    const float cdp = (fabsf(cd) < pert) ? (cd < 0.0f ? -pert : pert) : cd;
+                    const T cdp = (fabs(cd) < pert) ? (cd < T(0) ? -pert : pert) : cd;
                     f = e * fast_rcp(cdp);
-                    pv = cdp; qv = cu; sv = 0.0f; xv = cx_;
+                    pv = cdp; qv = cu; sv = T(0); xv = cx_;
                     ncd = dn - f * cu; ncu = un; ncx = bn - f * cx_;
                 } else {
                     f = cd * fast_rcp(e);
@@ -320,55 +362,55 @@ __device__ __forceinline__ float tri_eigvec_half(LRows<CH>& L, BRed4& br, int n,
                     L.FS[i] = sv;
                     L.FX[i] = xv;
                     L.FF[i] = f;
-                    L.FK[i] = keep ? 1.0f : 0.0f;
+                    L.FK[i] = keep ? T(1) : T(0);
                 }
                 cd = ncd;
                 cu = ncu;
                 cx_ = ncx;
             }
-            if (fabsf(cd) < pert) cd = (cd < 0.0f) ? -pert : pert;
+            if (fabs(cd) < pert) cd = (cd < T(0)) ? -pert : pert;
         } else {
             for (int i = 0; i < n - 1; ++i) {
-                const float bn = L.B[i + 1];
-                const float f = L.FF[i];
-                const bool keep = L.FK[i] != 0.0f;
-                const float xv = keep ? cx_ : bn;
-                const float ncx = keep ? bn - f * cx_ : cx_ - f * bn;
+                const T bn = L.B[i + 1];
+                const T f = L.FF[i];
+                const bool keep = L.FK[i] != T(0);
+                const T xv = keep ? cx_ : bn;
+                const T ncx = keep ? bn - f * cx_ : cx_ - f * bn;
                 if (threadIdx.x == 0) L.FX[i] = xv;
                 cx_ = ncx;
             }
         }
         __syncthreads();
-        float x1 = cx_ * fast_rcp(cd);
-        float x2 = 0.0f;
+        T x1 = cx_ * fast_rcp(cd);
+        T x2 = T(0);
         if (r == n - 1) xme = x1;
         for (int i = n - 2; i >= 0; --i) {
-            const float xi = (L.FX[i] - L.FQ[i] * x1 - L.FS[i] * x2) * fast_rcp(L.FP[i]);
+            const T xi = (L.FX[i] - L.FQ[i] * x1 - L.FS[i] * x2) * fast_rcp(L.FP[i]);
             if (r == i) xme = xi;
             x2 = x1;
             x1 = xi;
         }
-        const float mx = bred4<OpMax>(L, br, (r < n) ? fabsf(xme) : 0.0f);
-        const float sc = (mx > 0.0f && isfinite(mx)) ? 1.0f / mx : 1.0f;
+        const T mx = bredn<OpMax>(L, br, (r < n) ? fabs(xme) : T(0));
+        const T sc = (mx > T(0) && isfinite(mx)) ? T(1) / mx : T(1);
         xme *= sc;
-        if (hf == 0 && r < C) L.B[r] = xme;
+        if (q == 0 && r < C) L.B[r] = xme;
         __syncthreads();
     }
-    return (r < n) ? xme : 0.0f;
+    return (r < n) ? xme : T(0);
 }
 
 // y <- Q y with the reflectors parked in h (entry r > j+1 of v_j at column j).
-template <int CH>
-__device__ __forceinline__ cx<float> back_transform_half(LRows<CH>& L, BRed4& br, const cx<float> (&h)[CH], int n,
-                                                         int r, int hf, cx<float> y) {
-    constexpr int C = 2 * CH;
+template <typename T, int Q, int CH>
+__device__ __forceinline__ cx<T> back_transform_share(LRows<T, Q, CH>& L, BRedN& br, const cx<T> (&h)[CH], int n,
+                                                      int r, int q, cx<T> y) {
+    constexpr int C = Q * CH;
     // j is a compile-time index into h in every step: two fully unrolled
     // loops (one of 127 steps exceeds the unroller's size limit, and h would
     // then be indexed at run time, in scratch memory)
     auto step = [&](int j) {
-        const cx<float> part = (hf == (j & 1) && r > j + 1 && r < n) ? h[j >> 1] : cx<float>{0.0f, 0.0f};
-        const cx<float> v = (r == j + 1) ? cx<float>{1.0f, 0.0f} : psum(part);
-        const cx<float> s = bsum4(L, br, hf ? cx<float>{0.0f, 0.0f} : cmulc(y, v));
+        const cx<T> part = (q == j % Q && r > j + 1 && r < n) ? h[j / Q] : cx<T>{T(0), T(0)};
+        const cx<T> v = (r == j + 1) ? cx<T>{T(1), T(0)} : qsum<Q>(part);
+        const cx<T> s = bsumn(L, br, q ? cx<T>{T(0), T(0)} : cmulc(y, v));
         y = csub(y, cmul(v, cmul(L.TAU[j], s)));
     };
 #pragma unroll
@@ -380,182 +422,191 @@ __device__ __forceinline__ cx<float> back_transform_half(LRows<CH>& L, BRed4& br
     return y;
 }
 
-// Certification of the eigenpair (criteria at kCertScale).
-template <int CH>
-__device__ __forceinline__ bool eig_uncertain_half(LRows<CH>& L, BRed4& br, int n, int r, int hf, float lam, float tn,
-                                                   float pivmin, bool largest, cx<float> y) {
-    const float a2 = (r < n) ? cabs2(y) : 0.0f;
-    const float s2 = bred4<OpAdd>(L, br, hf ? 0.0f : a2);
-    const float m2 = bred4<OpMin>(L, br, (r < n) ? a2 : 3.0e38f);
-    const float vmin = sqrtf(m2 / s2);
-    const float delta = kCertScale * Lim<float>::eps() * tn / vmin;
+// Certification of an FP32 eigenpair (criteria at kCertScale).
+template <typename T, int Q, int CH>
+__device__ __forceinline__ bool eig_uncertain_share(LRows<T, Q, CH>& L, BRedN& br, int n, int r, int q, T lam, T tn,
+                                                    T pivmin, bool largest, cx<T> y) {
+    const T a2 = (r < n) ? cabs2(y) : T(0);
+    const T s2 = bredn<OpAdd>(L, br, q ? T(0) : a2);
+    const T m2 = bredn<OpMin>(L, br, (r < n) ? a2 : T(3.0e38));
+    const T vmin = sqrt(m2 / s2);
+    const T delta = T(kCertScale) * Lim<T>::eps() * tn / vmin;
     if (!(delta <= tn)) return true;
     const int c = sturm_below(L, n, largest ? lam - delta : lam + delta, pivmin);
     return largest ? (c <= n - 2) : (c >= 2);
 }
 
-template <int CH>
-__device__ __forceinline__ bool load_half(LRows<CH>& L, BRed4& br, cx<float> (&h)[CH], const cx<float>* grow, int n,
-                                          int r, int hf, bool act, int band) {
+template <typename T, int Q, int CH, typename Tg>
+__device__ __forceinline__ bool load_share(LRows<T, Q, CH>& L, BRedN& br, cx<T> (&h)[CH], const cx<Tg>* grow, int n,
+                                           int r, int q, bool act, int band) {
     bool fin = true;
 #pragma unroll
     for (int k0 = 0; k0 < CH; k0 += 16) {
 #pragma unroll
         for (int k = k0; k < (k0 + 16 < CH ? k0 + 16 : CH); ++k) {
-            const int c = 2 * k + hf;
-            cx<float> v = {0.0f, 0.0f};
+            const int c = Q * k + q;
+            cx<T> v = {T(0), T(0)};
             if (r < n && c < n) {
-                if (act) v = grow[c];
-                else if (c == r) v = cx<float>{1.0f, 0.0f};
-                if (band >= 0 && (c - r > band || r - c > band)) v = cx<float>{0.0f, 0.0f};
+                if (act) {
+                    const cx<Tg> g = grow[c];
+                    v = cx<T>{(T)g.re, (T)g.im};
+                } else if (c == r) {
+                    v = cx<T>{T(1), T(0)};
+                }
+                if (band >= 0 && (c - r > band || r - c > band)) v = cx<T>{T(0), T(0)};
             }
             fin = fin && isfinite(v.re) && isfinite(v.im);
             h[k] = v;
         }
         asm volatile("" ::: "memory");
     }
-    return bred4<OpMin>(L, br, fin ? 1.0f : 0.0f) > 0.5f;
+    return bredn<OpMin>(L, br, fin ? 1.0f : 0.0f) > 0.5f;
 }
 
-template <int CH>
-__device__ __forceinline__ bool hadamard_half(LRows<CH>& L, BRed4& br, cx<float> (&h)[CH], const double (&m)[CH],
-                                              const cx<float>* grow, int n, int r, int hf, bool act) {
+template <typename T, int Q, int CH, typename Tg>
+__device__ __forceinline__ bool hadamard_share(LRows<T, Q, CH>& L, BRedN& br, cx<T> (&h)[CH], const double (&m)[CH],
+                                               const cx<Tg>* grow, int n, int r, int q, bool act) {
     bool fin = true;
 #pragma unroll
     for (int k0 = 0; k0 < CH; k0 += 16) {
 #pragma unroll
         for (int k = k0; k < (k0 + 16 < CH ? k0 + 16 : CH); ++k) {
-            const int c = 2 * k + hf;
-            cx<float> v = {0.0f, 0.0f};
+            const int c = Q * k + q;
+            cx<T> v = {T(0), T(0)};
             if (r < n && c < n) {
-                cx<float> g = (c == r) ? cx<float>{1.0f, 0.0f} : cx<float>{0.0f, 0.0f};
-                if (act && c != r) g = grow[c];
-                const float mm = (float)(-m[k]);   // m holds -|Gamma|^-1
-                v = cx<float>{mm * g.re, mm * g.im};
+                cx<T> g = (c == r) ? cx<T>{T(1), T(0)} : cx<T>{T(0), T(0)};
+                if (act && c != r) {
+                    const cx<Tg> gg = grow[c];
+                    g = cx<T>{(T)gg.re, (T)gg.im};
+                }
+                const T mm = (T)(-m[k]);   // m holds -|Gamma|^-1
+                v = cx<T>{mm * g.re, mm * g.im};
                 fin = fin && isfinite(v.re + v.im);
             }
             h[k] = v;
         }
         asm volatile("" ::: "memory");
     }
-    return bred4<OpMin>(L, br, fin ? 1.0f : 0.0f) > 0.5f;
+    return bredn<OpMin>(L, br, fin ? 1.0f : 0.0f) > 0.5f;
 }
 
-// One pixel per 256-thread workgroup; slot s of the round is pixel first + s.
-// The FP32 result is certified; uncertified pixels go to the FP64 re-solve.
-template <int CH, int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CH <= 48 ? 3 : 2)))
-void solve_rows_kernel(SolveArgs a) {
-    constexpr int C = 2 * CH;
-    __shared__ LRows<CH> L;
-    BRed4 br;
-    const int r = threadIdx.x >> 1, hf = threadIdx.x & 1;
-    const int slot = (int)(gridDim.x - 1 - blockIdx.x);   // last-first, as the other solvers
-    if (slot >= a.slots) return;                            // the whole workgroup
+// The estimator chain of one pixel (every thread of the workgroup).  CERT:
+// the FP32 result is certified and an uncertified pixel is listed.
+template <typename T, int Q, int CH, int MODE, bool CERT, typename Tg>
+__device__ __forceinline__ void solve_rows_pixel(const SolveArgs& a, LRows<T, Q, CH>& L, const cx<Tg>* gam, int np,
+                                                 long long pix) {
+    constexpr int C = Q * CH;
+    BRedN br;
+    const int r = threadIdx.x / Q, q = threadIdx.x % Q;
     const int n = a.n;
-    const int np = a.npix[slot];
     const bool act = np >= 0;
-    const cx<float>* gam = (const cx<float>*)a.gamma + (size_t)slot * n * n;   // the MFMA Gram's complex64
-    const cx<float>* grow = gam + (size_t)min(r, n - 1) * n;
-    const long long pix = a.first + slot;
+    const cx<Tg>* grow = gam + (size_t)min(r, n - 1) * n;
 
     float code = 0.0f;
     bool failed = false;
-    cx<float> h[CH];
-    cx<float> y = {0.0f, 0.0f};
-    float tn, pm;
+    cx<T> h[CH];
+    cx<T> y = {T(0), T(0)};
+    T tn, pm;
     bool unc = false;
+    const T thr_margin = T(kCertThrMargin) * Lim<T>::eps();
 
     if constexpr (MODE == MODE_MLE) {
-        const bool fin = load_half(L, br, h, grow, n, r, hf, act, -1);
-        tridiag_half(L, br, h, n, r, hf);
-        const float lmin = bisect_half(L, br, n, 0, r, hf, tn, pm);
+        const bool fin = load_share(L, br, h, grow, n, r, q, act, -1);
+        tridiag_share(L, br, h, n, r, q);
+        const T lmin = bisect_share(L, br, n, 0, r, q, tn, pm);
         if (!fin) { code = -1.0f; failed = true; }
-        else if (!(lmin >= 1.0e-6f)) { code = -2.0f; failed = true; }
-        unc = !(lmin >= 1.0e-6f + kCertThrMargin * Lim<float>::eps() * tn);
+        else if (!(lmin >= T(1.0e-6))) { code = -2.0f; failed = true; }
+        if constexpr (CERT) unc = !(lmin >= T(1.0e-6) + thr_margin * tn);
         double m[CH];
         double pmin;
-        abs_gamma_half(m, grow, n, r, hf, act, 1.0e-6);
-        const bool pd_shift = sweep_half(L, m, n, r, hf, pmin);
+        abs_gamma_share<Q, CH>(m, grow, n, r, q, act, 1.0e-6);
+        const bool pd_shift = sweep_share(L, m, n, r, q, pmin);
         if (!failed && !pd_shift) { code = -4.0f; failed = true; }
-        unc = unc || !(pmin >= kCertPivotMin);
-        abs_gamma_half(m, grow, n, r, hf, act, 0.0);
-        const bool pd = sweep_half(L, m, n, r, hf, pmin);
+        if constexpr (CERT) unc = unc || !(pmin >= kCertPivotMin);
+        abs_gamma_share<Q, CH>(m, grow, n, r, q, act, 0.0);
+        const bool pd = sweep_share(L, m, n, r, q, pmin);
         if (!failed && !pd) { code = -5.0f; failed = true; }
-        unc = unc || !(pmin >= kCertPivotMin);
-        const bool hfin = hadamard_half(L, br, h, m, grow, n, r, hf, act);
-        tridiag_half(L, br, h, n, r, hf);
-        const float lam = bisect_half(L, br, n, 0, r, hf, tn, pm);
-        const float yr = tri_eigvec_half(L, br, n, r, hf, lam, tn);
-        y = back_transform_half(L, br, h, n, r, hf, cx<float>{yr, 0.0f});
+        if constexpr (CERT) unc = unc || !(pmin >= kCertPivotMin);
+        const bool hfin = hadamard_share(L, br, h, m, grow, n, r, q, act);
+        tridiag_share(L, br, h, n, r, q);
+        const T lam = bisect_share(L, br, n, 0, r, q, tn, pm);
+        const T yr = tri_eigvec_share(L, br, n, r, q, lam, tn);
+        y = back_transform_share(L, br, h, n, r, q, cx<T>{yr, T(0)});
         if (!failed && !hfin) { code = -6.0f; failed = true; }
-        if (!failed && !(lam >= 1.0e-6f)) { code = -7.0f; failed = true; }
-        unc = unc || !(lam >= 1.0e-6f + kCertThrMargin * Lim<float>::eps() * tn);
-        const bool ue = eig_uncertain_half(L, br, n, r, hf, lam, tn, pm, false, y);
-        unc = fin && (unc || (!failed && ue));
+        if (!failed && !(lam >= T(1.0e-6))) { code = -7.0f; failed = true; }
+        if constexpr (CERT) {
+            unc = unc || !(lam >= T(1.0e-6) + thr_margin * tn);
+            const bool ue = eig_uncertain_share(L, br, n, r, q, lam, tn, pm, false, y);
+            unc = fin && (unc || (!failed && ue));
+        }
     } else if constexpr (MODE == MODE_EVD) {
         const int band = (a.method == FRINGE_METHOD_STBAS) ? a.bw : -1;
-        const bool fin = load_half(L, br, h, grow, n, r, hf, act, band);
-        tridiag_half(L, br, h, n, r, hf);
-        const float lam = bisect_half(L, br, n, n - 1, r, hf, tn, pm);
-        const float yr = tri_eigvec_half(L, br, n, r, hf, lam, tn);
-        y = back_transform_half(L, br, h, n, r, hf, cx<float>{yr, 0.0f});
+        const bool fin = load_share(L, br, h, grow, n, r, q, act, band);
+        tridiag_share(L, br, h, n, r, q);
+        const T lam = bisect_share(L, br, n, n - 1, r, q, tn, pm);
+        const T yr = tri_eigvec_share(L, br, n, r, q, lam, tn);
+        y = back_transform_share(L, br, h, n, r, q, cx<T>{yr, T(0)});
         if (!fin) { code = -6.0f; failed = true; }
-        else if (!(lam >= 1.0e-6f)) { code = -7.0f; failed = true; }
-        const bool ue = eig_uncertain_half(L, br, n, r, hf, lam, tn, pm, true, y);
-        unc = fin && (ue || !(lam >= 1.0e-6f + kCertThrMargin * Lim<float>::eps() * tn));
+        else if (!(lam >= T(1.0e-6))) { code = -7.0f; failed = true; }
+        if constexpr (CERT) {
+            const bool ue = eig_uncertain_share(L, br, n, r, q, lam, tn, pm, true, y);
+            unc = fin && (ue || !(lam >= T(1.0e-6) + thr_margin * tn));
+        }
     } else {
         double m[CH];
         double pmin;
-        bool fin = abs_gamma_half(m, grow, n, r, hf, act, 0.0);
-        fin = bred4<OpMin>(L, br, fin ? 1.0f : 0.0f) > 0.5f;
-        const bool pd = sweep_half(L, m, n, r, hf, pmin);
-        const bool hfin = hadamard_half(L, br, h, m, grow, n, r, hf, act);
+        bool fin = abs_gamma_share<Q, CH>(m, grow, n, r, q, act, 0.0);
+        fin = bredn<OpMin>(L, br, fin ? 1.0f : 0.0f) > 0.5f;
+        const bool pd = sweep_share(L, m, n, r, q, pmin);
+        const bool hfin = hadamard_share(L, br, h, m, grow, n, r, q, act);
         const bool run_evd = !pd || !hfin;   // uniform over the workgroup
-        tridiag_half(L, br, h, n, r, hf);
-        const float lam = bisect_half(L, br, n, 0, r, hf, tn, pm);
-        const float yr = tri_eigvec_half(L, br, n, r, hf, lam, tn);
-        y = back_transform_half(L, br, h, n, r, hf, cx<float>{yr, 0.0f});
-        const bool ue = eig_uncertain_half(L, br, n, r, hf, lam, tn, pm, false, y);
-        unc = !(pmin >= kCertPivotMin) || (!run_evd && ue);
+        tridiag_share(L, br, h, n, r, q);
+        const T lam = bisect_share(L, br, n, 0, r, q, tn, pm);
+        const T yr = tri_eigvec_share(L, br, n, r, q, lam, tn);
+        y = back_transform_share(L, br, h, n, r, q, cx<T>{yr, T(0)});
+        if constexpr (CERT) {
+            const bool ue = eig_uncertain_share(L, br, n, r, q, lam, tn, pm, false, y);
+            unc = !(pmin >= kCertPivotMin) || (!run_evd && ue);
+        }
         if (act && fin && run_evd) {   // uniform: the whole workgroup is one pixel
-            load_half(L, br, h, grow, n, r, hf, act, -1);
-            tridiag_half(L, br, h, n, r, hf);
-            const float le = bisect_half(L, br, n, n - 1, r, hf, tn, pm);
-            const float ye = tri_eigvec_half(L, br, n, r, hf, le, tn);
-            y = back_transform_half(L, br, h, n, r, hf, cx<float>{ye, 0.0f});
-            unc = unc || eig_uncertain_half(L, br, n, r, hf, le, tn, pm, true, y);
+            load_share(L, br, h, grow, n, r, q, act, -1);
+            tridiag_share(L, br, h, n, r, q);
+            const T le = bisect_share(L, br, n, n - 1, r, q, tn, pm);
+            const T ye = tri_eigvec_share(L, br, n, r, q, le, tn);
+            y = back_transform_share(L, br, h, n, r, q, cx<T>{ye, T(0)});
+            if constexpr (CERT) unc = unc || eig_uncertain_share(L, br, n, r, q, le, tn, pm, true, y);
         }
         if (!fin) { code = -1.0f; failed = true; }
         unc = unc && fin;
     }
 
     // ---- referenced phases (evd.cpp:738-749) ----
-    if (hf == 0 && r == a.ref) L.YREF = y;
+    if (q == 0 && r == a.ref) L.YREF = y;
     __syncthreads();
-    const cx<float> vref = L.YREF;
+    const cx<T> vref = L.YREF;
     bool okr, oku, okv;
-    const cx<float> cref = unit_or(cconj(vref), cx<float>{1.0f, 0.0f}, okr);
-    const cx<float> u = unit_or(cmulc(y, vref), cx<float>{0.0f, 0.0f}, oku);
-    cx<float> o = oku ? u : (okr ? cref : unit_or(y, cx<float>{1.0f, 0.0f}, okv));
+    const cx<T> cref = unit_or(cconj(vref), cx<T>{T(1), T(0)}, okr);
+    const cx<T> u = unit_or(cmulc(y, vref), cx<T>{T(0), T(0)}, oku);
+    const cx<T> ot = oku ? u : (okr ? cref : unit_or(y, cx<T>{T(1), T(0)}, okv));
+    cx<float> o = {(float)ot.re, (float)ot.im};
     if (r == a.ref) o = cx<float>{1.0f, 0.0f};
-    if (hf == 0 && r < C) L.OUT[r] = o;
+    if (q == 0 && r < C) L.OUT[r] = o;
     __syncthreads();
 
     const bool write_ok = act && !failed;
     // ---- compressed SLC (evd.cpp:755-762) ----
     cx<double> cs = {0.0, 0.0};
-    if (hf == 0 && r >= a.ref && r < n && write_ok) {
+    if (q == 0 && r >= a.ref && r < n && write_ok) {
         const float2 z = a.slc[(long long)r * a.npx + pix];
         cs.re = (double)(z.x * o.re + z.y * o.im);
         cs.im = (double)(z.y * o.re - z.x * o.im);
     }
-    cs.re = bred4<OpAdd>(L, br, cs.re);
-    cs.im = bred4<OpAdd>(L, br, cs.im);
+    cs.re = bredn<OpAdd>(L, br, cs.re);
+    cs.im = bredn<OpAdd>(L, br, cs.im);
     // ---- temporal coherence (evd.cpp:770-786), pairs dealt out circulantly ----
     const bool stbas = (a.method == FRINGE_METHOD_STBAS);
     cx<float> tc = {0.0f, 0.0f};
-    if (hf == 0 && r < n && write_ok) {
+    if (q == 0 && r < n && write_ok) {
         bool okx;
         const cx<float> ci = unit_or(cx<float>{o.re, -o.im}, cx<float>{1.0f, 0.0f}, okx);
         const int half = (n - 1) / 2;
@@ -565,17 +616,17 @@ void solve_rows_kernel(SolveArgs a) {
             jd = jd + 1 == n ? 0 : jd + 1;
             const int lo = min(r, jd), hi = max(r, jd);
             if (stbas && hi - lo > a.bw) continue;
-            const cx<float> gv = grow[jd];
-            const cx<float> ph = unit_or(cx<float>{gv.re, gv.im}, cx<float>{1.0f, 0.0f}, okx);
+            const cx<Tg> gv = grow[jd];
+            const cx<float> ph = unit_or(cx<float>{(float)gv.re, (float)gv.im}, cx<float>{1.0f, 0.0f}, okx);
             const cx<float> oj = L.OUT[jd];
             cx<float> t = cmul(cmul(ph, ci), oj);
             if (r > jd) t = cconj(t);
             tc = cadd(tc, t);
         }
     }
-    tc = bsum4(L, br, tc);
+    tc = bsumn(L, br, tc);
 
-    if (hf == 0 && r < n)
+    if (q == 0 && r < n)
         a.phase[(long long)r * a.npx + pix] = write_ok ? make_float2(o.re, o.im) : make_float2(0.0f, 0.0f);
     if (threadIdx.x == 0) {
         long long counter = 0;
@@ -596,11 +647,66 @@ void solve_rows_kernel(SolveArgs a) {
         }
         a.tcorr[pix] = tco;
         a.comp[pix] = co;
-        if (act && unc) {
-            const int e = atomicAdd(a.cert_count, 1);
-            if (e < a.cert_cap) a.cert_list[e] = (int)pix;
+        if constexpr (CERT) {
+            if (act && unc) {
+                const int e = atomicAdd(a.cert_count, 1);
+                if (e < a.cert_cap) a.cert_list[e] = (int)pix;
+            }
         }
     }
 }
+
+// Waves per SIMD the register budget is held to: FP32 shares of 40 or 48
+// columns three (a few registers spilled), of 56 and 64 two; FP64 two.
+template <typename T, int CH> constexpr int rows_waves_per_eu() { return sizeof(T) == 4 && CH <= 48 ? 3 : 2; }
+
+// One pixel per workgroup of 128 Q threads; slot s of the round is pixel
+// first + s, its coherence complex Tg.
+template <typename T, int Q, int CH, int MODE, typename Tg>
+__global__ __launch_bounds__(128 * Q) __attribute__((amdgpu_waves_per_eu(rows_waves_per_eu<T, CH>())))
+void solve_rows_kernel(SolveArgs a) {
+    __shared__ LRows<T, Q, CH> L;
+    const int slot = (int)(gridDim.x - 1 - blockIdx.x);   // last-first, as the other solvers
+    if (slot >= a.slots) return;                            // the whole workgroup
+    const cx<Tg>* gam = (const cx<Tg>*)a.gamma + (size_t)slot * a.n * a.n;
+    solve_rows_pixel<T, Q, CH, MODE, sizeof(T) == 4, Tg>(a, L, gam, a.npix[slot], a.first + slot);
+}
+
+// FP64 re-solve of the listed pixels: each workgroup loops over its share of
+// the list.  Every wave computes the listed pixel's FP64 coherence with the
+// plain path's Gram (gram_f64_pixel, 64 lanes, the same values in every
+// wave; its barriers need every wave to run the same positions) into the
+// workgroup's global area, then the whole workgroup runs the FP64 chain on
+// it.
+template <int Q, int CH, int MODE>
+__global__ __launch_bounds__(128 * Q) __attribute__((amdgpu_waves_per_eu(rows_waves_per_eu<double, CH>())))
+void resolve_rows_kernel(SolveArgs a) {
+    __shared__ LRows<double, Q, CH> L;
+    extern __shared__ __align__(16) char smem[];
+    const int n = a.n;
+    cx<double>* gam = (cx<double>*)((char*)a.awork + a.awork_stride * (size_t)blockIdx.x);
+    const size_t gbytes = gram_f64_group_bytes(n, a.g64.Nx, a.g64.Ny, 16);
+    char* gscratch = smem + (threadIdx.x / 64) * gbytes;
+    const int cnt = min(*a.list_count, a.list_cap);
+    for (int e = blockIdx.x; e < cnt; e += gridDim.x) {
+        const long long pix = a.list[e];
+        const int np = gram_f64_pixel<64, 2, 16>(a.g64, true, pix, gam, gscratch);
+        __threadfence();
+        __syncthreads();
+        solve_rows_pixel<double, Q, CH, MODE, false, double>(a, L, gam, np, pix);
+        __syncthreads();
+    }
+}
+
+// Launchers (rows32.hip: T = float, Q = 2; rows64.hip: T = double, Q = 4),
+// Q CH columns per row, the fewest that hold n.
+void launch_rows_solver_f32(const SolveArgs& a, hipStream_t s);
+void launch_rows_solver_f64(const SolveArgs& a, hipStream_t s);
+// a.awork / a.awork_stride: the per-workgroup areas (at least n^2 complex
+// doubles each), blocks of them; a.g64, a.list*: the FP64 Gram and the list.
+void launch_resolve_rows(const SolveArgs& a, unsigned blocks, hipStream_t s);
+// LDS of the re-solve's workgroup (static chain state plus eight FP64 Gram
+// scratch areas), for the launch checks.
+size_t resolve_rows_lds_bytes(int n, int Nx, int Ny);
 
 }  // namespace fringe

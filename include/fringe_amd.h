@@ -196,8 +196,8 @@ int fringe_link_batch_run(const fringe_link_batch_item* items, int count, void* 
 #define FRINGE_SCHED_GRAM_VARIANT 0
 #define FRINGE_SCHED_SOLVE 1
 #define FRINGE_SCHED_GENERIC_A 2    /* (nbands class 96/128/256, precision): the solver above 64 dates keeps its
-                                       work matrix in LDS (0) or global memory (1), or (mixed precisions, <= 128
-                                       dates; the default there) each row in registers, half a row per thread (2) */
+                                       work matrix in LDS (0) or global memory (1), or (<= 128 dates, the default
+                                       there) each row in registers, two threads per row in FP32, four in FP64 (2) */
 #define FRINGE_GRAM_DEFAULT 0
 #define FRINGE_GRAM_STAGED12 1      /* 12 waves, 8 x 16-pixel tiles staged in LDS */
 #define FRINGE_GRAM_STAGED8 2       /* 8 waves, 4 x 16 */

@@ -371,12 +371,12 @@ def test_schedule_table_loads():
         assert L.fringe_schedule_get(_lib.FRINGE_SCHED_SOLVE, e["n"], e["mode"], e["precision"]) == want
     assert L.fringe_schedule_set(_lib.FRINGE_SCHED_GRAM_VARIANT, 30, 7, 5, 9) == 201
     assert L.fringe_schedule_set(7, 30, 7, 5, 0) == 201
-    # rows in registers: FP32 (mixed precisions) and at most 128 dates only
+    # rows in registers: at most 128 dates, either precision
     G = _lib.FRINGE_SCHED_GENERIC_A
-    assert L.fringe_schedule_set(G, 160, 1, 0, 2) == 201 and L.fringe_schedule_set(G, 96, 0, 0, 2) == 201
-    assert L.fringe_schedule_set(G, 96, 1, 0, 3) == 201
+    assert L.fringe_schedule_set(G, 160, 1, 0, 2) == 201 and L.fringe_schedule_set(G, 160, 0, 0, 2) == 201
+    assert L.fringe_schedule_set(G, 96, 1, 0, 3) == 201 and L.fringe_schedule_set(G, 96, 0, 0, 2) == 0
     L.fringe_schedule_clear()
-    assert L.fringe_schedule_get(G, 80, 1, 0) == 2 and L.fringe_schedule_get(G, 80, 0, 0) == 0
+    assert L.fringe_schedule_get(G, 80, 1, 0) == 2 and L.fringe_schedule_get(G, 80, 0, 0) == 2
     assert L.fringe_schedule_get(G, 128, 1, 0) == 2 and L.fringe_schedule_get(G, 160, 1, 0) == 0
     assert L.fringe_schedule_set(G, 72, 1, 0, 0) == 0 and L.fringe_schedule_get(G, 96, 1, 0) == 0
     L.fringe_schedule_clear()

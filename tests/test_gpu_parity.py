@@ -233,6 +233,25 @@ def test_link_generic_solver_modes(n, module, method, precision, place):
     assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL and st["comp_err"] <= PHASE_TOL
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("n", [65, 81, 97, 113, 128])
+def test_register_rows_column_counts(n, precision):
+    """Date counts just past each column count of the register-row solvers
+    (80, 96, 112, 128 columns in FP32; 20..32 per thread in FP64) and at
+    the last one: most of the last block of columns is padding."""
+    rows, cols, Nx, Ny = 16, 20, 7, 5
+    print(f"case register rows n={n} {precision}", flush=True)
+    slc = synthetic.stack(n, rows, cols, seed=800 + n)
+    wts = synthetic.full_mask(rows, cols, Nx, Ny)
+    for module, method in (("phase_link", "MLE"), ("evd", "EVD")):
+        g, r = _run_both(slc, wts, Nx, Ny, module, method, precision)
+        st = compare_link(g, r, r["gap"], stack=slc)
+        print(module, method, st)
+        assert st["untouched_equal"] and st["codes_equal"]
+        assert st["n_well"] >= 4
+        assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL and st["comp_err"] <= PHASE_TOL
+
+
 @pytest.mark.parametrize("n,Nx,Ny,module,method,place", [
     (10, 29, 9, "evd", "MLE", None), (30, 7, 5, "phase_link", "MLE", None), (64, 5, 4, "evd", "MLE", None),
     (64, 5, 4, "evd", "EVD", None), (48, 5, 4, "phase_link", "MLE", None), (96, 7, 5, "phase_link", "MLE", "lds"),

@@ -624,11 +624,16 @@ __device__ __forceinline__ void solve_generic_pixel(const SolveArgs& a, SolveLds
         const bool hfin = form_hadamard<T, Tg, G, R>(L.A, S, M, MS, gam, n, act);
         // run_evd is uniform over the group: the barriers below are block-wide
         bool run_evd = !pd || !hfin;
-        tridiagonalize<T, G, R>(L, S, n);
-        const T lam = extreme_eigenpair<T, G, R>(L, S, n, 0, y, tn, pm);
-        if constexpr (CERT) {
-            const bool ue = eig_uncertain_g<T, G, R>(L, n, lam, tn, pm, false, y);
-            unc = !(pmin >= kCertPivotMin) || (!run_evd && ue);
+        if constexpr (CERT) unc = !(pmin >= kCertPivotMin);
+        // the EMI eigenpair, unless the pixel falls back to EVD (the test is
+        // over the whole workgroup, so the barriers inside stay uniform)
+        if (__syncthreads_or(!run_evd)) {
+            tridiagonalize<T, G, R>(L, S, n);
+            const T lam = extreme_eigenpair<T, G, R>(L, S, n, 0, y, tn, pm);
+            if constexpr (CERT) {
+                const bool ue = eig_uncertain_g<T, G, R>(L, n, lam, tn, pm, false, y);
+                unc = unc || (!run_evd && ue);
+            }
         }
         if (__any(act && fin && run_evd)) {
             // EVD fallback: largest eigenvector of Gamma

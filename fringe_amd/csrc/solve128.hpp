@@ -560,13 +560,13 @@ __device__ __forceinline__ void solve_rows_pixel(const SolveArgs& a, LRows<T, Q,
         const bool pd = sweep_share(L, m, n, r, q, pmin);
         const bool hfin = hadamard_share(L, br, h, m, grow, n, r, q, act);
         const bool run_evd = !pd || !hfin;   // uniform over the workgroup
-        tridiag_share(L, br, h, n, r, q);
-        const T lam = bisect_share(L, br, n, 0, r, q, tn, pm);
-        const T yr = tri_eigvec_share(L, br, n, r, q, lam, tn);
-        y = back_transform_share(L, br, h, n, r, q, cx<T>{yr, T(0)});
-        if constexpr (CERT) {
-            const bool ue = eig_uncertain_share(L, br, n, r, q, lam, tn, pm, false, y);
-            unc = !(pmin >= kCertPivotMin) || (!run_evd && ue);
+        if constexpr (CERT) unc = !(pmin >= kCertPivotMin);
+        if (!run_evd) {   // the EMI eigenpair; a pixel that falls back to EVD skips it
+            tridiag_share(L, br, h, n, r, q);
+            const T lam = bisect_share(L, br, n, 0, r, q, tn, pm);
+            const T yr = tri_eigvec_share(L, br, n, r, q, lam, tn);
+            y = back_transform_share(L, br, h, n, r, q, cx<T>{yr, T(0)});
+            if constexpr (CERT) unc = unc || eig_uncertain_share(L, br, n, r, q, lam, tn, pm, false, y);
         }
         if (act && fin && run_evd) {   // uniform: the whole workgroup is one pixel
             load_share(L, br, h, grow, n, r, q, act, -1);

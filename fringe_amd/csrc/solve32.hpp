@@ -624,14 +624,19 @@ __device__ __forceinline__ void solve_reg_pixel(const SolveArgs& a, LdsR<T, NR>&
         blocks_to_rows<T, NR>(L, M, m, r);
         const bool hfin = hadamard_rows<T, Tg, NR>(h, m, grow, n, r, act);
         const bool run_evd = !pd || !hfin;
-        tridiag<T, NR>(L, h, n, r);
-        const T lam = bisect<T, NR>(L, n, 0, r, tn, pm);
-        const T yr = tri_eigvec<T, NR>(L, n, r, lam, tn);
-        y = back_transform<T, NR>(L, h, n, r, cx<T>{yr, T(0)});
-        if constexpr (CERT) {
-            // every lane runs the check (it holds cross-lane reductions)
-            const bool ue = eig_uncertain<T, NR>(L, n, r, lam, tn, pm, false, y);
-            unc = !(pmin >= kCertPivotMin) || (!run_evd && ue);
+        if constexpr (CERT) unc = !(pmin >= kCertPivotMin);
+        // the EMI eigenpair, unless every pixel of the wave falls back to EVD
+        // (a pixel's result never depends on whether this ran for it)
+        if (__any(!run_evd)) {
+            tridiag<T, NR>(L, h, n, r);
+            const T lam = bisect<T, NR>(L, n, 0, r, tn, pm);
+            const T yr = tri_eigvec<T, NR>(L, n, r, lam, tn);
+            y = back_transform<T, NR>(L, h, n, r, cx<T>{yr, T(0)});
+            if constexpr (CERT) {
+                // every lane runs the check (it holds cross-lane reductions)
+                const bool ue = eig_uncertain<T, NR>(L, n, r, lam, tn, pm, false, y);
+                unc = unc || (!run_evd && ue);
+            }
         }
         if (__any(act && fin && run_evd)) {
             load_rows<T, Tg, NR>(h, grow, n, r, act, -1);

@@ -235,10 +235,15 @@ static LinkLayout link_layout(const fringe_link_tile* t, int chunk, size_t area_
         L.awork_stride = align256((size_t)n * (n | 1) * el);
         b += (size_t)chunk * L.awork_stride;
     }
-    L.rwork = b;
-    if (n > 64 && link_certifies(n, t->precision)) {
-        L.resolve_blocks = resolve_generic_blocks(resolve_generic_area_bytes(n), area_budget);
-        b += (size_t)L.resolve_blocks * resolve_generic_area_bytes(n);
+    L.rwork = b;   // the FP64 re-solve's per-workgroup areas
+    if (link_certifies(n, t->precision)) {
+        if (n > 64) {
+            L.resolve_blocks = resolve_generic_blocks(resolve_generic_area_bytes(n), area_budget);
+            b += (size_t)L.resolve_blocks * resolve_generic_area_bytes(n);
+        } else {
+            L.resolve_blocks = resolve_small_blocks(n, area_budget);
+            b += (size_t)L.resolve_blocks * (n <= 32 ? 2 : 1) * resolve_small_area_bytes(n);
+        }
     }
     L.total = b;
     return L;
@@ -302,7 +307,7 @@ int link_run_chunked(const fringe_link_tile* t, const void* d_slc, const uint32_
         P.d_awork = (char*)d_work + LL.awork;
         P.awork_stride = LL.awork_stride;
     }
-    if (P.n > 64 && link_certifies(P.n, P.precision)) {
+    if (link_certifies(P.n, P.precision)) {
         P.d_rwork = (char*)d_work + LL.rwork;
         P.resolve_blocks = LL.resolve_blocks;
     }
@@ -394,9 +399,11 @@ static BatchLayout batch_layout(const fringe_link_batch_item* items, int count) 
         need += (size_t)out * per_px + 512;
         max_row = std::max(max_row, (size_t)t->cols * per_px + 1024);
         pixels += (size_t)out;
-        if (n > 64 && link_certifies(n, t->precision))
-            max_rwork = std::max(max_rwork, (size_t)resolve_generic_blocks(resolve_generic_area_bytes(n)) *
-                                                resolve_generic_area_bytes(n));
+        if (link_certifies(n, t->precision))
+            max_rwork = std::max(max_rwork, n > 64 ? (size_t)resolve_generic_blocks(resolve_generic_area_bytes(n)) *
+                                                         resolve_generic_area_bytes(n)
+                                                   : (size_t)resolve_small_blocks(n, kResolveAreaBudget) *
+                                                         (n <= 32 ? 2 : 1) * resolve_small_area_bytes(n));
     }
     L.round = b;
     L.round_bytes = std::max(std::min(need, (size_t)2048 << 20), max_row);

@@ -102,6 +102,16 @@ int link_batch_launch(const BatchPlan& B, hipStream_t s);
 // group (its coherence plus the FP64 Gram's scratch), and the whole
 // workgroup's LDS including the solver's static part.
 size_t resolve_group_bytes(int n, int Nx, int Ny);
+// Re-solve up to 64 dates: one lane group's coherence area in the workspace,
+// and the workgroups that a quarter of `budget` of such areas allows
+// (16..1024; two groups per workgroup up to 32 dates).
+size_t resolve_small_area_bytes(int n);
+int resolve_small_blocks(int n, size_t budget);
+// Column block of that re-solve's FP64 Gram: each coherence entry sums the
+// same products in the same order whatever the block (bit-identical to the
+// plain path's 32); 8 keeps the accumulators, and the kernel, within the
+// plain FP64 solver's registers.
+constexpr int kResolveSmallCB = 8;
 size_t resolve_lds_bytes(int n, int Nx, int Ny);
 // Whether a tile of this precision and date count certifies its FP32 results.
 bool link_certifies(int n, int precision);

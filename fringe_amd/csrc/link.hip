@@ -145,11 +145,10 @@ static void dispatch_reg(const SolveArgs& a, hipStream_t s) {
     }
 }
 
-// Workgroups of the FP64 re-solve; each loops over its share of the list.
-constexpr int kResolveBlocks = 1024;
-
 // FP64 re-solve (Gram and estimator chain of the plain FP64 path) of the
 // pixels the FP32 solver of this tile listed as not certified (n <= 64).
+// Each lane group's coherence goes to its own global area at P.d_rwork (in
+// LDS it allowed two workgroups per CU); the workgroups loop over the list.
 static void launch_resolve(const LinkPlan& P, const uint32_t* wts, SolveArgs a, hipStream_t s) {
     GramArgs g{};
     g.slc = a.slc;
@@ -170,22 +169,25 @@ static void launch_resolve(const LinkPlan& P, const uint32_t* wts, SolveArgs a, 
     a.cert_count = nullptr;
     a.cert_cap = 0;
     a.group_bytes = resolve_group_bytes(P.n, P.Nx, P.Ny);
+    a.awork = P.d_rwork;
+    a.awork_stride = resolve_small_area_bytes(P.n);
     const int gpw = P.n <= 32 ? 2 : 1;
     const size_t lds = a.group_bytes * gpw;
+    const dim3 grid((unsigned)(P.resolve_blocks ? P.resolve_blocks : resolve_small_blocks(P.n, kResolveAreaBudget)));
     if (P.n <= 32) {
         if (a.module == FRINGE_MODULE_PHASE_LINK)
-            hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_EMI, 32, true>), dim3(kResolveBlocks), dim3(64), lds, s, a);
+            hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_EMI, 32, true>), grid, dim3(64), lds, s, a);
         else if (a.method == FRINGE_METHOD_MLE)
-            hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_MLE, 32, true>), dim3(kResolveBlocks), dim3(64), lds, s, a);
+            hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_MLE, 32, true>), grid, dim3(64), lds, s, a);
         else
-            hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_EVD, 32, true>), dim3(kResolveBlocks), dim3(64), lds, s, a);
+            hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_EVD, 32, true>), grid, dim3(64), lds, s, a);
     } else {
         if (a.module == FRINGE_MODULE_PHASE_LINK)
-            hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_EMI, 64, true>), dim3(kResolveBlocks), dim3(64), lds, s, a);
+            hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_EMI, 64, true>), grid, dim3(64), lds, s, a);
         else if (a.method == FRINGE_METHOD_MLE)
-            hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_MLE, 64, true>), dim3(kResolveBlocks), dim3(64), lds, s, a);
+            hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_MLE, 64, true>), grid, dim3(64), lds, s, a);
         else
-            hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_EVD, 64, true>), dim3(kResolveBlocks), dim3(64), lds, s, a);
+            hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_EVD, 64, true>), grid, dim3(64), lds, s, a);
     }
 }
 
@@ -226,8 +228,15 @@ static void launch_resolve_generic(const LinkPlan& P, const uint32_t* wts, Solve
         hipLaunchKernelGGL((solve_kernel<double, double, 128, 2, true>), grid, dim3(128), lds, s, a);
 }
 
-size_t resolve_group_bytes(int n, int Nx, int Ny) {
-    return (((size_t)n * n * sizeof(cx<double>) + 255) & ~(size_t)255) + gram_f64_group_bytes(n, Nx, Ny, 32);
+// LDS of one lane group of the re-solve up to 64 dates: the Gram's scratch.
+size_t resolve_group_bytes(int n, int Nx, int Ny) { return gram_f64_group_bytes(n, Nx, Ny, kResolveSmallCB); }
+
+size_t resolve_small_area_bytes(int n) { return align256((size_t)n * n * sizeof(cx<double>)); }
+
+int resolve_small_blocks(int n, size_t budget) {
+    const int gpw = n <= 32 ? 2 : 1;
+    const size_t b = (budget / 4) / ((size_t)gpw * resolve_small_area_bytes(n));
+    return (int)(b < 16 ? 16 : (b > 1024 ? 1024 : b));
 }
 
 size_t resolve_lds_bytes(int n, int Nx, int Ny) {

@@ -56,8 +56,9 @@ extern "C" int fringe_link_run_noalloc(const fringe_link_tile* t, const void* d_
     if (st != 0) return st;
     if (!d_slc || !d_wts || !d_phase || !d_tcorr || !d_comp) return FRINGE_ERR_ARGUMENT;
     // the largest round (whole rows) whose workspace fits the fixed scratch;
-    // above 64 dates the FP64 re-solve keeps its minimum of 256 areas
-    constexpr size_t kAreaBudget = kScratchBytes / 4;
+    // the FP64 re-solve keeps few areas (its minimum of 256 above 64 dates,
+    // about 1 MiB of them up to 64)
+    constexpr size_t kAreaBudget = kScratchBytes / 64;
     const int chunk = link_chunk_for_bytes(t, kScratchBytes, kAreaBudget);
     if (chunk <= 0) {
         set_last_error("fringe_link_run_noalloc: the tile needs more than the 256 MiB fixed scratch "

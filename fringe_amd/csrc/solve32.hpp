@@ -22,6 +22,7 @@
 //     broadcast operands), back-transformation by the stored reflectors;
 //   * referenced phases, compressed SLC and temporal coherence.
 #pragma once
+#include "engine_internal.hpp"
 #include "link_solve.hpp"
 
 namespace fringe {
@@ -770,16 +771,17 @@ void solve_reg_kernel(SolveArgs a) {
         static_assert(sizeof(T) == 8 && sizeof(Tg) == 8, "the re-solve runs in FP64");
         constexpr int GPW = 64 / NR;
         extern __shared__ __align__(16) char dyn[];
-        const int n = a.n;
-        char* base = dyn + grp * a.group_bytes;
-        cx<double>* gl = (cx<double>*)base;
-        char* gscratch = base + (((size_t)n * n * sizeof(cx<double>) + 255) & ~(size_t)255);
+        // the group's coherence in its global area (a.awork), the Gram's
+        // scratch in LDS
+        cx<double>* gl = (cx<double>*)((char*)a.awork + a.awork_stride * ((size_t)blockIdx.x * GPW + grp));
+        char* gscratch = dyn + grp * a.group_bytes;
         const int cnt = min(*a.list_count, a.list_cap);
         for (int b0 = blockIdx.x * GPW; b0 < cnt; b0 += gridDim.x * GPW) {
             const int e = b0 + grp;
             const bool valid = e < cnt;
             const long long pix = valid ? (long long)a.list[e] : 0;
-            const int np = gram_f64_pixel<NR, 1, 32>(a.g64, valid, pix, gl, gscratch);
+            const int np = gram_f64_pixel<NR, 1, kResolveSmallCB>(a.g64, valid, pix, gl, gscratch);
+            __threadfence();
             __syncthreads();
             solve_reg_pixel<T, Tg, MODE, NR, false>(a, L, r, valid, np, (const cx<Tg>*)gl, pix);
             __syncthreads();

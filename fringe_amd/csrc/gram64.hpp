@@ -65,14 +65,28 @@ __host__ __device__ inline size_t gram_f64_group_bytes(int n, int Nx, int Ny, in
     return (b + 15) & ~(size_t)15;
 }
 
+// Barrier of the Gram's lane groups: the workgroup's, or (WSYNC, a single
+// 64-lane group called by one wave while the others wait) the wave's.
+template <bool WSYNC> __device__ __forceinline__ void gram_sync() {
+    if constexpr (WSYNC) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // Coherence of pixel `pix` into gam[n][n] (both triangles, unit diagonal);
 // returns the neighbour count, or -1 when the pixel is not linked (centre
 // bit clear, or fewer than min_nb neighbours; gam is then not written).
 // Every group of the workgroup calls it together (it holds barriers);
-// `valid` false makes a group take part without a pixel.
-template <int G, int R, int CB>
+// `valid` false makes a group take part without a pixel.  WSYNC (G = 64):
+// one wave calls it alone and only the wave synchronises.
+template <int G, int R, int CB, bool WSYNC = false>
 __device__ __forceinline__ int gram_f64_pixel(const GramArgs& a, bool valid, long long pix, cx<double>* gam,
                                               char* base) {
+    static_assert(!WSYNC || G == 64, "a wave-synchronised Gram is one 64-lane group");
     constexpr int GPW = 64 / G;
     const int g = threadIdx.x % G;
     const int n = a.n;
@@ -91,7 +105,7 @@ __device__ __forceinline__ int gram_f64_pixel(const GramArgs& a, bool valid, lon
     int Kw = K;
     if (GPW > 1) Kw = max(Kw, __shfl_xor(Kw, 32));
     for (int c = g; c < NC * CB; c += G) zb[c] = cx<double>{0.0, 0.0};
-    __syncthreads();
+    gram_sync<WSYNC>();
 
     for (int cb = 0; cb < NC; ++cb) {
         cx<double> acc[R][CB];
@@ -115,14 +129,14 @@ __device__ __forceinline__ int gram_f64_pixel(const GramArgs& a, bool valid, lon
                 }
                 if (r < n) zb[r] = z[q];
             }
-            __syncthreads();
+            gram_sync<WSYNC>();
 #pragma unroll
             for (int q = 0; q < R; ++q) {
 #pragma unroll
                 for (int c = 0; c < CB; ++c) cfma_c(acc[q][c], z[q], zb[cb * CB + c]);
                 if (cb == 0) d[q] += cabs2(z[q]);
             }
-            __syncthreads();
+            gram_sync<WSYNC>();
         }
         if (cb == 0) {
 #pragma unroll
@@ -130,7 +144,7 @@ __device__ __forceinline__ int gram_f64_pixel(const GramArgs& a, bool valid, lon
                 const int r = g + G * q;
                 if (r < n) dg[r] = d[q];
             }
-            __syncthreads();
+            gram_sync<WSYNC>();
         }
         if (go) {
 #pragma unroll
@@ -153,7 +167,7 @@ __device__ __forceinline__ int gram_f64_pixel(const GramArgs& a, bool valid, lon
                 }
             }
         }
-        __syncthreads();
+        gram_sync<WSYNC>();
     }
     return go ? cnt : -1;
 }

@@ -82,10 +82,13 @@ template <bool WSYNC> __device__ __forceinline__ void gram_sync() {
 // bit clear, or fewer than min_nb neighbours; gam is then not written).
 // Every group of the workgroup calls it together (it holds barriers);
 // `valid` false makes a group take part without a pixel.  WSYNC (G = 64):
-// one wave calls it alone and only the wave synchronises.
+// the calling wave synchronises only itself, and computes the column blocks
+// cb_first, cb_first + cb_step, ... (several waves can share a pixel's
+// blocks; each entry is the same sum in the same order whatever the split,
+// and every wave accumulates the diagonal in its first block).
 template <int G, int R, int CB, bool WSYNC = false>
 __device__ __forceinline__ int gram_f64_pixel(const GramArgs& a, bool valid, long long pix, cx<double>* gam,
-                                              char* base) {
+                                              char* base, int cb_first = 0, int cb_step = 1) {
     static_assert(!WSYNC || G == 64, "a wave-synchronised Gram is one 64-lane group");
     constexpr int GPW = 64 / G;
     const int g = threadIdx.x % G;
@@ -107,7 +110,7 @@ __device__ __forceinline__ int gram_f64_pixel(const GramArgs& a, bool valid, lon
     for (int c = g; c < NC * CB; c += G) zb[c] = cx<double>{0.0, 0.0};
     gram_sync<WSYNC>();
 
-    for (int cb = 0; cb < NC; ++cb) {
+    for (int cb = cb_first; cb < NC; cb += cb_step) {
         cx<double> acc[R][CB];
         double d[R];
 #pragma unroll
@@ -134,11 +137,11 @@ __device__ __forceinline__ int gram_f64_pixel(const GramArgs& a, bool valid, lon
             for (int q = 0; q < R; ++q) {
 #pragma unroll
                 for (int c = 0; c < CB; ++c) cfma_c(acc[q][c], z[q], zb[cb * CB + c]);
-                if (cb == 0) d[q] += cabs2(z[q]);
+                if (cb == cb_first) d[q] += cabs2(z[q]);
             }
             gram_sync<WSYNC>();
         }
-        if (cb == 0) {
+        if (cb == cb_first) {
 #pragma unroll
             for (int q = 0; q < R; ++q) {
                 const int r = g + G * q;

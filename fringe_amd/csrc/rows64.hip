@@ -28,12 +28,12 @@ void launch_rows_solver_f64(const SolveArgs& a, hipStream_t s) {
 size_t resolve_rows_lds_bytes(int n, int Nx, int Ny) {
     const size_t stat = n <= 80 ? sizeof(LRows<double, 4, 20>) : n <= 96 ? sizeof(LRows<double, 4, 24>)
                         : n <= 112 ? sizeof(LRows<double, 4, 28>) : sizeof(LRows<double, 4, 32>);
-    return stat + gram_f64_group_bytes(n, Nx, Ny, 16);
+    return stat + 8 * gram_f64_group_bytes(n, Nx, Ny, 16);
 }
 
 template <int CH> static void resolve_f64(const SolveArgs& a, unsigned blocks, hipStream_t s) {
     const dim3 grid(blocks), blk(512);
-    const size_t lds = gram_f64_group_bytes(a.n, a.g64.Nx, a.g64.Ny, 16);   // wave 0's Gram scratch
+    const size_t lds = 8 * gram_f64_group_bytes(a.n, a.g64.Nx, a.g64.Ny, 16);   // one Gram scratch per wave
     if (a.module == FRINGE_MODULE_PHASE_LINK)
         hipLaunchKernelGGL((resolve_rows_kernel<4, CH, MODE_EMI>), grid, blk, lds, s, a);
     else if (a.method == FRINGE_METHOD_MLE)

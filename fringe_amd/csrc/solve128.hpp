@@ -673,11 +673,12 @@ void solve_rows_kernel(SolveArgs a) {
 }
 
 // FP64 re-solve of the listed pixels: each workgroup loops over its share of
-// the list.  Wave 0 computes the listed pixel's FP64 coherence with the
-// plain path's Gram (gram_f64_pixel, one 64-lane group synchronised by the
-// wave alone: with every wave taking part, its two barriers per position
-// over eight waves cost more than the chain) into the workgroup's global
-// area, then the whole workgroup runs the FP64 chain on it.
+// the list.  The waves compute the listed pixel's FP64 coherence with the
+// plain path's Gram (gram_f64_pixel), wave w the column blocks w, w + 8, ...
+// each synchronising only itself (with the whole workgroup taking part in
+// every block, its two barriers per position over eight waves cost more
+// than the chain), into the workgroup's global area; then the whole
+// workgroup runs the FP64 chain on it.
 template <int Q, int CH, int MODE>
 __global__ __launch_bounds__(128 * Q) __attribute__((amdgpu_waves_per_eu(rows_waves_per_eu<double, CH>())))
 void resolve_rows_kernel(SolveArgs a) {
@@ -688,14 +689,12 @@ void resolve_rows_kernel(SolveArgs a) {
     const int cnt = min(*a.list_count, a.list_cap);
     for (int e = blockIdx.x; e < cnt; e += gridDim.x) {
         const long long pix = a.list[e];
-        if (threadIdx.x < 64) {   // wave-uniform
-            const int np = gram_f64_pixel<64, 2, 16, true>(a.g64, true, pix, gam, smem);
-            if (threadIdx.x == 0) L.PV = (double)np;
-            __threadfence();
-        }
+        const int w = threadIdx.x >> 6;
+        const int np = gram_f64_pixel<64, 2, 16, true>(a.g64, true, pix, gam,
+                                                       smem + w * gram_f64_group_bytes(n, a.g64.Nx, a.g64.Ny, 16),
+                                                       w, 2 * Q);
+        __threadfence();
         __syncthreads();
-        const int np = (int)L.PV;
-        __syncthreads();   // every thread has read the count before the chain reuses PV
         solve_rows_pixel<double, Q, CH, MODE, false, double>(a, L, gam, np, pix);
         __syncthreads();
     }
@@ -708,8 +707,8 @@ void launch_rows_solver_f64(const SolveArgs& a, hipStream_t s);
 // a.awork / a.awork_stride: the per-workgroup areas (at least n^2 complex
 // doubles each), blocks of them; a.g64, a.list*: the FP64 Gram and the list.
 void launch_resolve_rows(const SolveArgs& a, unsigned blocks, hipStream_t s);
-// LDS of the re-solve's workgroup (static chain state plus wave 0's FP64
-// Gram scratch), for the launch checks.
+// LDS of the re-solve's workgroup (static chain state plus one FP64 Gram
+// scratch per wave), for the launch checks.
 size_t resolve_rows_lds_bytes(int n, int Nx, int Ny);
 
 }  // namespace fringe

@@ -86,9 +86,10 @@ class BatchItem(ctypes.Structure):
 
 FRINGE_SCHED_GRAM_VARIANT, FRINGE_SCHED_SOLVE, FRINGE_SCHED_GENERIC_A = 0, 1, 2
 GRAM_VARIANTS = {"default": 0, "staged12": 1, "staged8": 2, "staged4": 3, "unstaged": 4}
-# FRINGE_SCHED_GENERIC_A values: the solver's work matrix above 64 dates in
-# LDS or in global memory, or (mixed precisions, <= 128 dates) rows in registers
-GENERIC_A_PLACES = {"lds": 0, "global": 1, "registers": 2}
+# FRINGE_SCHED_GENERIC_A values: above 32 dates the solver's rows in
+# registers (up to 128 dates), or one row per lane (up to 64 dates), or its
+# work matrix in LDS or in global memory (above 64 dates)
+GENERIC_A_PLACES = {"lanes": 0, "lds": 0, "global": 1, "registers": 2}
 SCHEDULE_FILE = os.path.join(_HERE, "schedules.json")
 
 

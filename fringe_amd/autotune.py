@@ -100,7 +100,6 @@ def tune(quick=False, log=sys.stderr):
              (48, 0, "mixed"), (64, 1, "mixed"), (96, 0, "mixed")]
     if quick:
         cases = cases[:3]
-    generic_a_default = {pk: L.fringe_schedule_get(_lib.FRINGE_SCHED_GENERIC_A, 96, pk, 0) for pk in (0, 1)}
     for n, mode, prec in cases:
         module, method = MODES[mode]
         Nx, Ny = (10, 5) if n == 64 else (7, 5)
@@ -110,7 +109,10 @@ def tune(quick=False, log=sys.stderr):
         if n > 64:
             slc, w = slc[:, :16].contiguous(), w[:16].contiguous()
         key = (n, mode, 0 if prec == "fp64" else 1)
-        if n > 64:   # the launch schedule serves the solver with a work matrix
+        # the launch schedule serves the one-row-per-lane solver and the
+        # solver with a work matrix, not the register rows
+        ga_default = L.fringe_schedule_get(_lib.FRINGE_SCHED_GENERIC_A, n, key[2], 0)
+        if n > 32:
             L.fringe_schedule_set(_lib.FRINGE_SCHED_GENERIC_A, n, key[2], 0, 0)
         times = {}
         for launch, wv, g in cands:
@@ -119,26 +121,29 @@ def tune(quick=False, log=sys.stderr):
             times[f"{launch}:{wv}:{g}"] = _time_stage(L, lambda: r.run(slc, w), "solve", reps)
             del r
         L.fringe_schedule_set(_lib.FRINGE_SCHED_SOLVE, *key, 0)
-        if n > 64:
-            L.fringe_schedule_set(_lib.FRINGE_SCHED_GENERIC_A, n, key[2], 0, generic_a_default[key[2]])
+        if n > 32:
+            L.fringe_schedule_set(_lib.FRINGE_SCHED_GENERIC_A, n, key[2], 0, ga_default)
         best = min(times, key=times.get)
         launch, wv, g = best.split(":")
         solve.append({"n": n, "mode": mode, "precision": key[2], "estimator": f"{module}/{method}",
                       "launch": launch, "waves_per_cu": int(wv), "grab": int(g), "ms": times})
         print(f"solve n={n} {module}/{method} {prec}: {best} ({times[best]:.2f} ms, grid {times['grid:0:0']:.2f})",
               file=log, flush=True)
-    # ---- above 64 dates: work matrix in LDS or global memory, or registers --
+    # ---- above 32 dates: rows in registers, or one row per lane (up to 64
+    # dates), or a work matrix in LDS or global memory (above) --------------
     generic_a = []
-    for n, prec in ((72, "mixed"), (96, "mixed"), (72, "fp64"), (96, "fp64"), (128, "mixed"), (128, "fp64")):
+    for n, prec in ((48, "mixed"), (64, "mixed"), (48, "fp64"), (64, "fp64"), (72, "mixed"), (96, "mixed"),
+                    (72, "fp64"), (96, "fp64"), (128, "mixed"), (128, "fp64")):
         if quick and n != 96:
             continue
+        prow = 16 if n > 64 else 256
         slc, w = data(n, 7, 5, fill=0.9)
-        slc, w = slc[:, :16].contiguous(), w[:16].contiguous()
-        p = tile.link_params(16, cols, n, 7, 5, min_neighbors=5, precision=prec)
+        slc, w = slc[:, :prow].contiguous(), w[:prow].contiguous()
+        p = tile.link_params(prow, cols, n, 7, 5, min_neighbors=5, precision=prec)
         pk = 0 if prec == "fp64" else 1
         default = L.fringe_schedule_get(_lib.FRINGE_SCHED_GENERIC_A, n, pk, 0)
         times = {}
-        places = ["lds", "global"] + (["registers"] if n <= 128 else [])
+        places = ["lanes", "registers"] if n <= 64 else ["lds", "global"] + (["registers"] if n <= 128 else [])
         for place in places:
             v = _lib.GENERIC_A_PLACES[place]
             L.fringe_schedule_set(_lib.FRINGE_SCHED_GENERIC_A, n, pk, 0, v)
@@ -148,7 +153,7 @@ def tune(quick=False, log=sys.stderr):
         L.fringe_schedule_set(_lib.FRINGE_SCHED_GENERIC_A, n, pk, 0, default)
         best = min(times, key=times.get)
         generic_a.append({"n": n, "precision": pk, "work_matrix": best, "ms": times})
-        print(f"work matrix n={n} {prec}: {best} {times}", file=log, flush=True)
+        print(f"solver layout n={n} {prec}: {best} {times}", file=log, flush=True)
     return {"device": torch.cuda.get_device_name(0), "tile": f"{rows} x {cols}",
             "note": "stage times in ms per call (median); the library installs the chosen entries",
             "gram": gram, "solve": solve, "generic_a": generic_a}

@@ -174,7 +174,9 @@ int link_check(const fringe_link_tile* t) {
     // global memory when it does not fit, the rest of a pixel's group must
     const size_t lds = (size_t)device_lds_limit();
     if (t->nbands <= 64) {
-        if (link_certifies(t->nbands, t->precision) && resolve_lds_bytes(t->nbands, t->Nx, t->Ny) > lds)
+        if (link_certifies(t->nbands, t->precision) &&
+            (rows_solver_selected(t->nbands, FRINGE_PREC_FP64) ? resolve_rows_lds_bytes(t->nbands, t->Nx, t->Ny)
+                                                               : resolve_lds_bytes(t->nbands, t->Nx, t->Ny)) > lds)
             return FRINGE_ERR_UNSUPPORTED;
     } else {
         const size_t rest = (t->precision == FRINGE_PREC_FP64 ? solve_group_bytes<double>(t->nbands, false)

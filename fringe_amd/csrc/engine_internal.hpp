@@ -125,10 +125,11 @@ inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 // Value of a schedule key (fringe_schedule_set), or fallback when unset.
 int schedule_lookup(int kind, int k0, int k1, int k2, int fallback);
 // FRINGE_SCHED_GENERIC_A when unset: the register-row solver (2) at
-// 65..128 dates, the LDS work matrix (0) otherwise.
+// 33..128 dates (FP32: 33..48 and 65..128), otherwise the one-row-per-lane
+// solver up to 64 dates and the LDS work matrix above (0).
 constexpr int kGenericARowsDefault = 2;
 int generic_a_default(int n, int precision_key);
-// 65..128 dates under FRINGE_SCHED_GENERIC_A = 2 (either precision; the
+// 33..128 dates under FRINGE_SCHED_GENERIC_A = 2 (either precision; the
 // FP64 re-solve follows the FP64 choice, whose outputs it must equal).
 bool rows_solver_selected(int n, int precision);
 size_t resolve_rows_lds_bytes(int n, int Nx, int Ny);

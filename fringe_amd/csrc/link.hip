@@ -61,11 +61,13 @@ static void launch_solve_generic(SolveArgs a, hipStream_t s) {
                        a);
 }
 
-// 64 < n <= 128: the register-row solver (FP32 with two threads per row in
-// the mixed precisions, FP64 with four in the plain path and the re-solve),
-// unless the schedule FRINGE_SCHED_GENERIC_A places a work matrix instead.
+// 32 < n <= 128: the register-row solver (FP32 with two threads per row in
+// the mixed precisions, FP64 with four in the plain path and the re-solve)
+// where the schedule FRINGE_SCHED_GENERIC_A (or its default) selects it;
+// otherwise the one-row-per-lane solver up to 64 dates and a work matrix
+// above.
 bool rows_solver_selected(int n, int precision) {
-    if (n <= 64 || n > kMaxBandsMfma) return false;
+    if (n <= 32 || n > kMaxBandsMfma) return false;
     const int pk = precision == FRINGE_PREC_FP64 ? 0 : 1;
     return schedule_lookup(FRINGE_SCHED_GENERIC_A, n, pk, 0, generic_a_default(n, pk)) == 2;
 }
@@ -213,10 +215,12 @@ static void launch_resolve_generic(const LinkPlan& P, const uint32_t* wts, Solve
     a.cert_count = nullptr;
     a.cert_cap = 0;
     a.awork = P.d_rwork;
-    a.awork_stride = resolve_generic_area_bytes(P.n);
+    a.awork_stride = P.n <= 64 ? resolve_small_area_bytes(P.n) : resolve_generic_area_bytes(P.n);
     a.group_bytes = solve_group_bytes<double>(P.n, false);
     const size_t lds = a.group_bytes + 2 * gram_f64_group_bytes(P.n, P.Nx, P.Ny, kResolveCB);
-    const unsigned blocks = (unsigned)(P.resolve_blocks ? P.resolve_blocks : resolve_generic_blocks(a.awork_stride));
+    const unsigned blocks = (unsigned)(P.resolve_blocks ? P.resolve_blocks
+                                       : P.n <= 64 ? resolve_small_blocks(P.n, kResolveAreaBudget)
+                                                   : resolve_generic_blocks(a.awork_stride));
     const dim3 grid(blocks);
     if (rows_solver_selected(P.n, FRINGE_PREC_FP64)) {   // the plain FP64 path's solver
         launch_resolve_rows(a, blocks, s);
@@ -459,7 +463,7 @@ static void resolve_tile(const LinkPlan& P, const float2* slc, const uint32_t* w
     a.phase = phase;
     a.tcorr = tcorr;
     a.comp = comp;
-    if (P.n <= 64) launch_resolve(P, wts, a, s);
+    if (P.n <= 64 && !rows_solver_selected(P.n, FRINGE_PREC_FP64)) launch_resolve(P, wts, a, s);
     else launch_resolve_generic(P, wts, a, s);
 }
 
@@ -517,7 +521,7 @@ int link_launch(const LinkPlan& P, const float2* slc, const uint32_t* wts, float
             a.phase = phase;
             a.tcorr = tcorr;
             a.comp = comp;
-            if (P.n <= 64) {
+            if (P.n <= 64 && !rows_solver_selected(P.n, P.precision)) {
                 if (fp64) dispatch_reg<double, double>(a, s);
                 else dispatch_reg<float, float>(a, s);
             } else {

@@ -8,10 +8,11 @@
 //   Solver launch, keyed by (date class, estimator, precision): a grid with
 //   one wave per unit, or a persistent grid of `waves per CU` x CUs waves
 //   that take `grab` units at a time from a queue.
-//   Work matrix above 64 dates, keyed by (date class 96 / 128 / 256,
-//   precision): in LDS where it fits (0), in global memory (1), or, up to
-//   128 dates, no work matrix at all: the solver that keeps each row in
-//   registers, two (FP32) or four (FP64) threads per row (2).
+//   Solver layout above 32 dates, keyed by (date class 48 / 64 / 96 / 128 /
+//   256, precision): the solver that keeps each row in registers, two
+//   (FP32) or four (FP64) threads per row (2, up to 128 dates); otherwise,
+//   up to 64 dates, the solver with one row per lane (0), and above 64 dates
+//   a work matrix in LDS where it fits (0) or in global memory (1).
 //
 // Unset keys use the built-in defaults (the heuristics the table replaces).
 #include <map>
@@ -30,15 +31,19 @@ int date_class(int n) { return n <= 32 ? 32 : (n <= 64 ? 64 : 256); }
 
 std::tuple<int, int, int, int> key_of(int kind, int k0, int k1, int k2) {
     if (kind == FRINGE_SCHED_GRAM_VARIANT || kind == FRINGE_SCHED_SOLVE) k0 = date_class(k0);
-    if (kind == FRINGE_SCHED_GENERIC_A) k0 = k0 <= 96 ? 96 : (k0 <= 128 ? 128 : 256);
+    if (kind == FRINGE_SCHED_GENERIC_A)
+        k0 = k0 <= 48 ? 48 : (k0 <= 64 ? 64 : (k0 <= 96 ? 96 : (k0 <= 128 ? 128 : 256)));
     return std::make_tuple(kind, k0, k1, k2);
 }
 
 }  // namespace
 
 int generic_a_default(int n, int precision_key) {
-    (void)precision_key;
-    return (n > 64 && n <= 128) ? kGenericARowsDefault : 0;
+    if (n <= 32 || n > 128) return 0;
+    // 49..64 dates in FP32: the one-row-per-lane solver (rows of two threads
+    // measured 1.35x slower there, 1.05x faster at 33..48)
+    if (precision_key != 0 && n > 48 && n <= 64) return 0;
+    return kGenericARowsDefault;
 }
 
 int schedule_lookup(int kind, int k0, int k1, int k2, int fallback) {
@@ -62,6 +67,7 @@ int fringe_schedule_set(int kind, int key0, int key1, int key2, int value) {
     } else if (kind == FRINGE_SCHED_GENERIC_A) {
         if (value < 0 || value > 2) return FRINGE_ERR_ARGUMENT;
         if (value == 2 && key0 > 128) return FRINGE_ERR_ARGUMENT;   // rows in registers: <= 128 dates
+        if (value == 1 && key0 <= 64) return FRINGE_ERR_ARGUMENT;   // no work matrix up to 64 dates
     } else {
         return FRINGE_ERR_ARGUMENT;
     }

@@ -1,5 +1,6 @@
-// solve128.hpp -- register-resident estimator chain for 64 < n <= 128 dates,
-// one pixel per workgroup of Q threads per row.
+// solve128.hpp -- register-resident estimator chain for 32 < n <= 128 dates,
+// one pixel per workgroup of RW rows (64, 96 or 128: the fewest that hold n)
+// of Q threads each.
 //
 // Thread t holds a 1/Q share of row r = t / Q of its pixel's matrices: the
 // columns c = Q k + q (k < CH, q = t % Q), interleaved so that the live
@@ -14,22 +15,22 @@
 // registers and applies the symmetric sweep operator one published pivot row
 // at a time.
 //
-//   T = float, Q = 2 (four waves): the mixed precisions, coherence from the
-//   MFMA Gram, FP32 results certified (uncertified pixels are listed for the
-//   FP64 re-solve).  CH = 40, 48: three waves per SIMD; 56, 64: two.
-//   T = double, Q = 4 (eight waves): the plain FP64 path, and the FP64
-//   re-solve of the listed pixels (each workgroup computes the listed pixel's
-//   FP64 coherence into its global area first), so that re-solved pixels
-//   equal the FP64 path bit for bit.  CH = 20 .. 32.
+//   T = float, Q = 2 (two to four waves): the mixed precisions, coherence
+//   from the MFMA Gram, FP32 results certified (uncertified pixels are listed
+//   for the FP64 re-solve).  CH = 24 .. 64.
+//   T = double, Q = 4 (four to eight waves): the plain FP64 path, and the
+//   FP64 re-solve of the listed pixels (each workgroup computes the listed
+//   pixel's FP64 coherence into its global area first), so that re-solved
+//   pixels equal the FP64 path bit for bit.  CH = 12 .. 32.
 #pragma once
 #include "gram64.hpp"
 #include "solve32.hpp"
 
 namespace fringe {
 
-template <typename T, int Q, int CH> struct LRows {
-    static constexpr int C = Q * CH;   // columns (dates) held
-    static constexpr int NW = 2 * Q;   // waves (128 rows of Q threads)
+template <typename T, int Q, int CH, int RW> struct LRows {
+    static constexpr int C = Q * CH;        // columns (dates) held
+    static constexpr int NW = RW * Q / 64;  // waves (RW rows of Q threads)
     cx<T> R[C];         // published row
     cx<T> V[C];
     cx<T> W[C];
@@ -61,9 +62,9 @@ template <int Q, typename T> __device__ __forceinline__ cx<T> qsum(cx<T> x) {
     return cx<T>{qsum<Q>(x.re), qsum<Q>(x.im)};
 }
 
-template <typename Op, typename T, typename S, int Q, int CH>
-__device__ __forceinline__ S bredn(LRows<T, Q, CH>& L, BRedN& br, S x) {
-    constexpr int NW = 2 * Q;
+template <typename Op, typename T, typename S, int Q, int CH, int RW>
+__device__ __forceinline__ S bredn(LRows<T, Q, CH, RW>& L, BRedN& br, S x) {
+    constexpr int NW = LRows<T, Q, CH, RW>::NW;
     x = redR<Op, 64>(x);
     const int s = br.seq++ & 1, w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) L.red[s][w][0] = (double)x;
@@ -73,9 +74,9 @@ __device__ __forceinline__ S bredn(LRows<T, Q, CH>& L, BRedN& br, S x) {
     for (int q = 1; q < NW; ++q) a = Op::f(a, (S)L.red[s][q][0]);
     return a;
 }
-template <typename T, typename S, int Q, int CH>
-__device__ __forceinline__ cx<S> bsumn(LRows<T, Q, CH>& L, BRedN& br, cx<S> x) {
-    constexpr int NW = 2 * Q;
+template <typename T, typename S, int Q, int CH, int RW>
+__device__ __forceinline__ cx<S> bsumn(LRows<T, Q, CH, RW>& L, BRedN& br, cx<S> x) {
+    constexpr int NW = LRows<T, Q, CH, RW>::NW;
     x.re = redR<OpAdd, 64>(x.re);
     x.im = redR<OpAdd, 64>(x.im);
     const int s = br.seq++ & 1, w = threadIdx.x >> 6;
@@ -93,8 +94,9 @@ __device__ __forceinline__ cx<S> bsumn(LRows<T, Q, CH>& L, BRedN& br, cx<S> x) {
     return cx<S>{(S)re, (S)im};
 }
 // Index of the first thread whose predicate holds, or 64 NW.
-template <typename T, int Q, int CH> __device__ __forceinline__ int bfirstn(LRows<T, Q, CH>& L, BRedN& br, bool p) {
-    constexpr int NW = 2 * Q;
+template <typename T, int Q, int CH, int RW>
+__device__ __forceinline__ int bfirstn(LRows<T, Q, CH, RW>& L, BRedN& br, bool p) {
+    constexpr int NW = LRows<T, Q, CH, RW>::NW;
     const uint64_t b = __ballot(p);
     const int s = br.seq++ & 1, w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) L.red[s][w][0] = __builtin_bit_cast(double, b);
@@ -143,8 +145,8 @@ __device__ __forceinline__ bool abs_gamma_share(double (&m)[CH], const cx<Tg>* g
 // read it, so subtracting 2 from each swept diagonal at the end gives
 // -A^-1.  Returns whether every pivot was positive; pmin receives the
 // smallest |pivot| up to the first one that is not.
-template <typename T, int Q, int CH>
-__device__ __forceinline__ bool sweep_share(LRows<T, Q, CH>& L, double (&m)[CH], int n, int r, int q, double& pmin) {
+template <typename T, int Q, int CH, int RW>
+__device__ __forceinline__ bool sweep_share(LRows<T, Q, CH, RW>& L, double (&m)[CH], int n, int r, int q, double& pmin) {
     constexpr int C = Q * CH;
     bool ok = true;
     pmin = 1.0;
@@ -183,8 +185,8 @@ __device__ __forceinline__ bool sweep_share(LRows<T, Q, CH>& L, double (&m)[CH],
 // Step j, for Q K0 - 1 <= j <= Q K0 + 6: local columns k < K0 (global columns
 // below Q K0, a multiple of 8) hold only finished entries and parked
 // reflectors.
-template <typename T, int Q, int CH, int K0>
-__device__ __forceinline__ void tri_step_share(LRows<T, Q, CH>& L, BRedN& br, cx<T> (&h)[CH], int j, int n, int r,
+template <typename T, int Q, int CH, int RW, int K0>
+__device__ __forceinline__ void tri_step_share(LRows<T, Q, CH, RW>& L, BRedN& br, cx<T> (&h)[CH], int j, int n, int r,
                                                int q) {
     constexpr int C = Q * CH;
     constexpr int KB = 8 / Q;   // local columns per 8 global ones
@@ -247,30 +249,30 @@ __device__ __forceinline__ void tri_step_share(LRows<T, Q, CH>& L, BRedN& br, cx
     __syncthreads();
 }
 
-template <typename T, int Q, int CH, int K0>
-__device__ __forceinline__ void tri_loop(LRows<T, Q, CH>& L, BRedN& br, cx<T> (&h)[CH], int& j, int n, int r, int q) {
+template <typename T, int Q, int CH, int RW, int K0>
+__device__ __forceinline__ void tri_loop(LRows<T, Q, CH, RW>& L, BRedN& br, cx<T> (&h)[CH], int& j, int n, int r, int q) {
     constexpr int KB = 8 / Q;
     const int jend = (Q * K0 + 7 < n - 1) ? Q * K0 + 7 : n - 1;
-    for (; j < jend; ++j) tri_step_share<T, Q, CH, K0>(L, br, h, j, n, r, q);
-    if constexpr (K0 + KB < CH) tri_loop<T, Q, CH, K0 + KB>(L, br, h, j, n, r, q);
+    for (; j < jend; ++j) tri_step_share<T, Q, CH, RW, K0>(L, br, h, j, n, r, q);
+    if constexpr (K0 + KB < CH) tri_loop<T, Q, CH, RW, K0 + KB>(L, br, h, j, n, r, q);
 }
 
-template <typename T, int Q, int CH>
-__device__ __forceinline__ void tridiag_share(LRows<T, Q, CH>& L, BRedN& br, cx<T> (&h)[CH], int n, int r, int q) {
+template <typename T, int Q, int CH, int RW>
+__device__ __forceinline__ void tridiag_share(LRows<T, Q, CH, RW>& L, BRedN& br, cx<T> (&h)[CH], int n, int r, int q) {
     if (r == 0) {
 #pragma unroll
         for (int k = 0; k < CH; ++k) L.R[Q * k + q] = h[k];
     }
     __syncthreads();
     int j = 0;
-    tri_loop<T, Q, CH, 0>(L, br, h, j, n, r, q);
+    tri_loop<T, Q, CH, RW, 0>(L, br, h, j, n, r, q);
     if (threadIdx.x == 0) L.D[n - 1] = L.R[n - 1].re;
     __syncthreads();
 }
 
 // Sturm count of tridiag(L.D, L.E2) below sig.
-template <typename T, int Q, int CH>
-__device__ __forceinline__ int sturm_below(const LRows<T, Q, CH>& L, int n, T sig, T pivmin) {
+template <typename T, int Q, int CH, int RW>
+__device__ __forceinline__ int sturm_below(const LRows<T, Q, CH, RW>& L, int n, T sig, T pivmin) {
     T x = L.D[0] - sig;
     if (fabs(x) < pivmin) x = -pivmin;
     int cnt = x < T(0);
@@ -282,11 +284,11 @@ __device__ __forceinline__ int sturm_below(const LRows<T, Q, CH>& L, int n, T si
     return cnt;
 }
 
-// ---- eigenvalue k (0-based ascending) of tridiag(D, E): (128 Q + 1)-section
-template <typename T, int Q, int CH>
-__device__ __forceinline__ T bisect_share(LRows<T, Q, CH>& L, BRedN& br, int n, int k, int r, int q, T& tnorm,
+// ---- eigenvalue k (0-based ascending) of tridiag(D, E): (RW Q + 1)-section
+template <typename T, int Q, int CH, int RW>
+__device__ __forceinline__ T bisect_share(LRows<T, Q, CH, RW>& L, BRedN& br, int n, int k, int r, int q, T& tnorm,
                                           T& pm) {
-    constexpr int NT = 128 * Q;   // threads
+    constexpr int NT = RW * Q;   // threads
     T lo = T(3.0e38), hi = T(-3.0e38), em2 = T(0), tn = T(0);
     if (q == 0 && r < n) {
         const T a = (r > 0) ? fabs(L.E[r - 1]) : T(0);
@@ -309,8 +311,9 @@ __device__ __forceinline__ T bisect_share(LRows<T, Q, CH>& L, BRedN& br, int n, 
     const T pad = T(4) * Lim<T>::eps() * tn + T(2) * pivmin;
     lo -= pad;
     hi += pad;
-    // 257-section, four rounds, narrows the Gershgorin interval 4.4e9-fold
-    // (FP32); 513-section, seven rounds, 9.4e18-fold (FP64)
+    // four rounds in FP32 narrow the Gershgorin interval 2.8e8-fold (129
+    // sections, the fewest threads) to 4.4e9-fold (257); seven in FP64
+    // 7.4e16-fold (257) to 9.4e18-fold (513)
     constexpr int rounds = sizeof(T) == 4 ? 4 : 7;
     const int t = threadIdx.x;
     for (int it = 0; it < rounds; ++it) {
@@ -327,8 +330,8 @@ __device__ __forceinline__ T bisect_share(LRows<T, Q, CH>& L, BRedN& br, int n, 
 }
 
 // ---- eigenvector of tridiag(D, E) for lam: two rounds of inverse iteration
-template <typename T, int Q, int CH>
-__device__ __forceinline__ T tri_eigvec_share(LRows<T, Q, CH>& L, BRedN& br, int n, int r, int q, T lam, T tnorm) {
+template <typename T, int Q, int CH, int RW>
+__device__ __forceinline__ T tri_eigvec_share(LRows<T, Q, CH, RW>& L, BRedN& br, int n, int r, int q, T lam, T tnorm) {
     constexpr int C = Q * CH;
     const T pert = max(Lim<T>::eps() * tnorm, Lim<T>::tiny());
     if (q == 0 && r < C) L.B[r] = T(0.5) + (T)((r * 37 + 11) & 63) * (T(1) / T(64));
@@ -400,8 +403,8 @@ __device__ __forceinline__ T tri_eigvec_share(LRows<T, Q, CH>& L, BRedN& br, int
 }
 
 // y <- Q y with the reflectors parked in h (entry r > j+1 of v_j at column j).
-template <typename T, int Q, int CH>
-__device__ __forceinline__ cx<T> back_transform_share(LRows<T, Q, CH>& L, BRedN& br, const cx<T> (&h)[CH], int n,
+template <typename T, int Q, int CH, int RW>
+__device__ __forceinline__ cx<T> back_transform_share(LRows<T, Q, CH, RW>& L, BRedN& br, const cx<T> (&h)[CH], int n,
                                                       int r, int q, cx<T> y) {
     constexpr int C = Q * CH;
     // j is a compile-time index into h in every step: two fully unrolled
@@ -423,8 +426,8 @@ __device__ __forceinline__ cx<T> back_transform_share(LRows<T, Q, CH>& L, BRedN&
 }
 
 // Certification of an FP32 eigenpair (criteria at kCertScale).
-template <typename T, int Q, int CH>
-__device__ __forceinline__ bool eig_uncertain_share(LRows<T, Q, CH>& L, BRedN& br, int n, int r, int q, T lam, T tn,
+template <typename T, int Q, int CH, int RW>
+__device__ __forceinline__ bool eig_uncertain_share(LRows<T, Q, CH, RW>& L, BRedN& br, int n, int r, int q, T lam, T tn,
                                                     T pivmin, bool largest, cx<T> y) {
     const T a2 = (r < n) ? cabs2(y) : T(0);
     const T s2 = bredn<OpAdd>(L, br, q ? T(0) : a2);
@@ -436,8 +439,8 @@ __device__ __forceinline__ bool eig_uncertain_share(LRows<T, Q, CH>& L, BRedN& b
     return largest ? (c <= n - 2) : (c >= 2);
 }
 
-template <typename T, int Q, int CH, typename Tg>
-__device__ __forceinline__ bool load_share(LRows<T, Q, CH>& L, BRedN& br, cx<T> (&h)[CH], const cx<Tg>* grow, int n,
+template <typename T, int Q, int CH, int RW, typename Tg>
+__device__ __forceinline__ bool load_share(LRows<T, Q, CH, RW>& L, BRedN& br, cx<T> (&h)[CH], const cx<Tg>* grow, int n,
                                            int r, int q, bool act, int band) {
     bool fin = true;
 #pragma unroll
@@ -463,8 +466,8 @@ __device__ __forceinline__ bool load_share(LRows<T, Q, CH>& L, BRedN& br, cx<T> 
     return bredn<OpMin>(L, br, fin ? 1.0f : 0.0f) > 0.5f;
 }
 
-template <typename T, int Q, int CH, typename Tg>
-__device__ __forceinline__ bool hadamard_share(LRows<T, Q, CH>& L, BRedN& br, cx<T> (&h)[CH], const double (&m)[CH],
+template <typename T, int Q, int CH, int RW, typename Tg>
+__device__ __forceinline__ bool hadamard_share(LRows<T, Q, CH, RW>& L, BRedN& br, cx<T> (&h)[CH], const double (&m)[CH],
                                                const cx<Tg>* grow, int n, int r, int q, bool act) {
     bool fin = true;
 #pragma unroll
@@ -492,8 +495,8 @@ __device__ __forceinline__ bool hadamard_share(LRows<T, Q, CH>& L, BRedN& br, cx
 
 // The estimator chain of one pixel (every thread of the workgroup).  CERT:
 // the FP32 result is certified and an uncertified pixel is listed.
-template <typename T, int Q, int CH, int MODE, bool CERT, typename Tg>
-__device__ __forceinline__ void solve_rows_pixel(const SolveArgs& a, LRows<T, Q, CH>& L, const cx<Tg>* gam, int np,
+template <typename T, int Q, int CH, int RW, int MODE, bool CERT, typename Tg>
+__device__ __forceinline__ void solve_rows_pixel(const SolveArgs& a, LRows<T, Q, CH, RW>& L, const cx<Tg>* gam, int np,
                                                  long long pix) {
     constexpr int C = Q * CH;
     BRedN br;
@@ -656,33 +659,34 @@ __device__ __forceinline__ void solve_rows_pixel(const SolveArgs& a, LRows<T, Q,
     }
 }
 
-// Waves per SIMD the register budget is held to: FP32 shares of 40 or 48
-// columns three (a few registers spilled), of 56 and 64 two; FP64 two.
+// Least waves per SIMD the register budget is held to: FP32 shares of up to
+// 48 columns three (a few registers spilled at 48), of 56 and 64 two; FP64
+// two.  The smaller shares take fewer registers and reach more (up to four).
 template <typename T, int CH> constexpr int rows_waves_per_eu() { return sizeof(T) == 4 && CH <= 48 ? 3 : 2; }
 
-// One pixel per workgroup of 128 Q threads; slot s of the round is pixel
+// One pixel per workgroup of RW Q threads; slot s of the round is pixel
 // first + s, its coherence complex Tg.
-template <typename T, int Q, int CH, int MODE, typename Tg>
-__global__ __launch_bounds__(128 * Q) __attribute__((amdgpu_waves_per_eu(rows_waves_per_eu<T, CH>())))
+template <typename T, int Q, int CH, int RW, int MODE, typename Tg>
+__global__ __launch_bounds__(RW * Q) __attribute__((amdgpu_waves_per_eu(rows_waves_per_eu<T, CH>())))
 void solve_rows_kernel(SolveArgs a) {
-    __shared__ LRows<T, Q, CH> L;
+    __shared__ LRows<T, Q, CH, RW> L;
     const int slot = (int)(gridDim.x - 1 - blockIdx.x);   // last-first, as the other solvers
     if (slot >= a.slots) return;                            // the whole workgroup
     const cx<Tg>* gam = (const cx<Tg>*)a.gamma + (size_t)slot * a.n * a.n;
-    solve_rows_pixel<T, Q, CH, MODE, sizeof(T) == 4, Tg>(a, L, gam, a.npix[slot], a.first + slot);
+    solve_rows_pixel<T, Q, CH, RW, MODE, sizeof(T) == 4, Tg>(a, L, gam, a.npix[slot], a.first + slot);
 }
 
 // FP64 re-solve of the listed pixels: each workgroup loops over its share of
 // the list.  The waves compute the listed pixel's FP64 coherence with the
-// plain path's Gram (gram_f64_pixel), wave w the column blocks w, w + 8, ...
-// each synchronising only itself (with the whole workgroup taking part in
-// every block, its two barriers per position over eight waves cost more
-// than the chain), into the workgroup's global area; then the whole
-// workgroup runs the FP64 chain on it.
-template <int Q, int CH, int MODE>
-__global__ __launch_bounds__(128 * Q) __attribute__((amdgpu_waves_per_eu(rows_waves_per_eu<double, CH>())))
+// plain path's Gram (gram_f64_pixel), wave w of NW the column blocks w,
+// w + NW, ..., each synchronising only itself (with the whole workgroup
+// taking part in every block, its two barriers per position over eight waves
+// cost more than the chain), into the workgroup's global area; then the
+// whole workgroup runs the FP64 chain on it.
+template <int Q, int CH, int RW, int MODE>
+__global__ __launch_bounds__(RW * Q) __attribute__((amdgpu_waves_per_eu(rows_waves_per_eu<double, CH>())))
 void resolve_rows_kernel(SolveArgs a) {
-    __shared__ LRows<double, Q, CH> L;
+    __shared__ LRows<double, Q, CH, RW> L;
     extern __shared__ __align__(16) char smem[];
     const int n = a.n;
     cx<double>* gam = (cx<double>*)((char*)a.awork + a.awork_stride * (size_t)blockIdx.x);
@@ -692,16 +696,16 @@ void resolve_rows_kernel(SolveArgs a) {
         const int w = threadIdx.x >> 6;
         const int np = gram_f64_pixel<64, 2, 16, true>(a.g64, true, pix, gam,
                                                        smem + w * gram_f64_group_bytes(n, a.g64.Nx, a.g64.Ny, 16),
-                                                       w, 2 * Q);
+                                                       w, LRows<double, Q, CH, RW>::NW);
         __threadfence();
         __syncthreads();
-        solve_rows_pixel<double, Q, CH, MODE, false, double>(a, L, gam, np, pix);
+        solve_rows_pixel<double, Q, CH, RW, MODE, false, double>(a, L, gam, np, pix);
         __syncthreads();
     }
 }
 
 // Launchers (rows32.hip: T = float, Q = 2; rows64.hip: T = double, Q = 4),
-// Q CH columns per row, the fewest that hold n.
+// Q CH columns per row and RW rows, the fewest that hold n.
 void launch_rows_solver_f32(const SolveArgs& a, hipStream_t s);
 void launch_rows_solver_f64(const SolveArgs& a, hipStream_t s);
 // a.awork / a.awork_stride: the per-workgroup areas (at least n^2 complex

@@ -195,9 +195,11 @@ int fringe_link_batch_run(const fringe_link_batch_item* items, int count, void* 
  * units at a time from a queue.  Unset keys keep the built-in defaults. */
 #define FRINGE_SCHED_GRAM_VARIANT 0
 #define FRINGE_SCHED_SOLVE 1
-#define FRINGE_SCHED_GENERIC_A 2    /* (nbands class 96/128/256, precision): the solver above 64 dates keeps its
-                                       work matrix in LDS (0) or global memory (1), or (<= 128 dates, the default
-                                       there) each row in registers, two threads per row in FP32, four in FP64 (2) */
+#define FRINGE_SCHED_GENERIC_A 2    /* (nbands class 48/64/96/128/256, precision): the solver above 32 dates
+                                       keeps each row in registers, two threads per row in FP32, four in FP64
+                                       (2, up to 128 dates; the default there, except FP32 at 49..64 dates);
+                                       otherwise one row per lane up to 64 dates (0), or its work matrix in LDS
+                                       (0) or global memory (1) above 64 dates */
 #define FRINGE_GRAM_DEFAULT 0
 #define FRINGE_GRAM_STAGED12 1      /* 12 waves, 8 x 16-pixel tiles staged in LDS */
 #define FRINGE_GRAM_STAGED8 2       /* 8 waves, 4 x 16 */

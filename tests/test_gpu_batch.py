@@ -122,8 +122,9 @@ def test_persistent_schedule_equals_grid(n, precision):
     p = tile.link_params(rows, cols, n, Nx, Ny, min_neighbors=5, precision=precision)
     key = (n, 0, 0 if precision == "fp64" else 1)
     old = L.fringe_schedule_get(_lib.FRINGE_SCHED_SOLVE, *key)
-    # above 64 dates the persistent launch serves the LDS work matrix solver
-    place = _Sched(_lib.FRINGE_SCHED_GENERIC_A, (n, key[2], 0), 0 if n > 64 else None)
+    # above 32 dates the persistent launch serves the one-row-per-lane solver
+    # (up to 64 dates) and the LDS work matrix solver, not the register rows
+    place = _Sched(_lib.FRINGE_SCHED_GENERIC_A, (n, key[2], 0), 0 if n > 32 else None)
     place.__enter__()
     try:
         assert L.fringe_schedule_set(_lib.FRINGE_SCHED_SOLVE, *key, 0) == 0

@@ -36,8 +36,9 @@ def _run_both(slc, wts, Nx, Ny, module, method, precision, min_nb=2, ref=0, bw=-
 
 
 class _GenericPlace:
-    """Sets FRINGE_SCHED_GENERIC_A (the solver's work matrix above 64 dates:
-    LDS, global memory, or rows in registers) for one key, restoring it."""
+    """Sets FRINGE_SCHED_GENERIC_A (the solver's layout above 32 dates: rows
+    in registers, one row per lane up to 64 dates, or a work matrix in LDS or
+    global memory above) for one key, restoring it."""
 
     def __init__(self, n, precision, place):
         from fringe_amd import _lib
@@ -127,17 +128,22 @@ def test_nmap_row_range(n, method):
         assert not wg[outside].any() and not cg[outside].any()
 
 
+@pytest.mark.parametrize("place", ["lanes", "registers"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("module,method", [("phase_link", "MLE"), ("evd", "EVD"), ("evd", "MLE")])
 @pytest.mark.parametrize("n", [33, 48, 64])
-def test_link_64row_solver(n, module, method, precision):
-    # 32 < n <= 64: one pixel per wave, one row of every matrix per lane; an
-    # 11 x 9 window holds 99 samples in the interior, fewer at the edges
+def test_link_64row_solver(n, module, method, precision, place):
+    # 32 < n <= 64: one pixel per wave, one row of every matrix per lane
+    # ("lanes"), or one pixel per workgroup of 64 rows of two (FP32) or four
+    # (FP64) threads ("registers"); an 11 x 9 window holds 99 samples in the
+    # interior, fewer at the edges
     rows, cols, Nx, Ny = 14, 16, 5, 4
-    print(f"case 64-row n={n} {module}/{method} {precision}", flush=True)
+    print(f"case 64-row n={n} {module}/{method} {precision} {place}", flush=True)
     slc = synthetic.stack(n, rows, cols, seed=300 + n)
     wts = synthetic.full_mask(rows, cols, Nx, Ny)
-    g, r = _run_both(slc, wts, Nx, Ny, module, method, precision)
+    # the mixed run's FP64 re-solve follows the FP64 layout
+    with _GenericPlace(n, precision, place), _GenericPlace(n, "fp64", place):
+        g, r = _run_both(slc, wts, Nx, Ny, module, method, precision)
     st = compare_link(g, r, r["gap"], stack=slc)
     print(st)
     assert st["untouched_equal"]
@@ -234,17 +240,19 @@ def test_link_generic_solver_modes(n, module, method, precision, place):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("n", [65, 81, 97, 113, 128])
+@pytest.mark.parametrize("n", [33, 49, 65, 81, 97, 113, 128])
 def test_register_rows_column_counts(n, precision):
     """Date counts just past each column count of the register-row solvers
-    (80, 96, 112, 128 columns in FP32; 20..32 per thread in FP64) and at
-    the last one: most of the last block of columns is padding."""
+    (48 .. 128 columns in FP32; 12 .. 32 per thread in FP64) and at the last
+    one: most of the last block of columns is padding, and (33, 65, 97)
+    most rows of the 64-, 96- or 128-row workgroup are idle."""
     rows, cols, Nx, Ny = 16, 20, 7, 5
     print(f"case register rows n={n} {precision}", flush=True)
     slc = synthetic.stack(n, rows, cols, seed=800 + n)
     wts = synthetic.full_mask(rows, cols, Nx, Ny)
     for module, method in (("phase_link", "MLE"), ("evd", "EVD")):
-        g, r = _run_both(slc, wts, Nx, Ny, module, method, precision)
+        with _GenericPlace(n, precision, "registers"), _GenericPlace(n, "fp64", "registers"):
+            g, r = _run_both(slc, wts, Nx, Ny, module, method, precision)
         st = compare_link(g, r, r["gap"], stack=slc)
         print(module, method, st)
         assert st["untouched_equal"] and st["codes_equal"]
@@ -252,24 +260,28 @@ def test_register_rows_column_counts(n, precision):
         assert st["phase_max"] <= PHASE_TOL and st["tcorr_max"] <= TCORR_TOL and st["comp_err"] <= PHASE_TOL
 
 
-@pytest.mark.parametrize("n,Nx,Ny,module,method,place", [
-    (10, 29, 9, "evd", "MLE", None), (30, 7, 5, "phase_link", "MLE", None), (64, 5, 4, "evd", "MLE", None),
-    (64, 5, 4, "evd", "EVD", None), (48, 5, 4, "phase_link", "MLE", None), (96, 7, 5, "phase_link", "MLE", "lds"),
-    (96, 7, 5, "phase_link", "MLE", "registers"), (80, 6, 5, "evd", "MLE", "lds"),
-    (80, 6, 5, "evd", "MLE", "registers"), (88, 6, 5, "evd", "EVD", "registers"),
-    (120, 7, 5, "phase_link", "MLE", "registers")])
-def test_mixed_resolve_equals_fp64(n, Nx, Ny, module, method, place):
+@pytest.mark.parametrize("n,Nx,Ny,module,method,place,place64", [
+    (10, 29, 9, "evd", "MLE", None, None), (30, 7, 5, "phase_link", "MLE", None, None),
+    (64, 5, 4, "evd", "MLE", None, None), (64, 5, 4, "evd", "EVD", None, None),
+    (48, 5, 4, "phase_link", "MLE", None, None), (48, 5, 4, "phase_link", "MLE", None, "lanes"),
+    (64, 5, 4, "evd", "EVD", "registers", "lanes"), (96, 7, 5, "phase_link", "MLE", "lds", None),
+    (96, 7, 5, "phase_link", "MLE", "registers", None), (80, 6, 5, "evd", "MLE", "lds", None),
+    (80, 6, 5, "evd", "MLE", "registers", None), (88, 6, 5, "evd", "EVD", "registers", None),
+    (120, 7, 5, "phase_link", "MLE", "registers", None)])
+def test_mixed_resolve_equals_fp64(n, Nx, Ny, module, method, place, place64):
     """Pixels whose FP32 eigenpair the mixed path cannot certify are
     recomputed by the FP64 path (FP64 Gram and solver) in the same call:
     their outputs equal the plain FP64 run's bit for bit, and every other
-    pixel agrees with the oracle to the tolerances."""
+    pixel agrees with the oracle to the tolerances.  place: the mixed run's
+    solver layout; place64: the FP64 layout, which the re-solve follows."""
     rows, cols = 20, 40
-    print(f"case resolve n={n} {2 * Nx + 1}x{2 * Ny + 1} {module}/{method} {place}", flush=True)
+    print(f"case resolve n={n} {2 * Nx + 1}x{2 * Ny + 1} {module}/{method} {place} {place64}", flush=True)
     slc = synthetic.stack(n, rows, cols, seed=700 + n, patches=6)
     wts, _ = oracle.nmap(slc, Nx, Ny, 0.05)
     out = {}
     for precision in ("fp64", "mixed"):
-        with _GenericPlace(n, precision, place if precision == "mixed" else None):
+        with _GenericPlace(n, precision, place if precision == "mixed" else None), \
+                _GenericPlace(n, "fp64", place64):
             g, r, res = _run_both(slc, wts, Nx, Ny, module, method, precision, min_nb=5, want_resolved=True)
         out[precision] = (g, res)
     (g64, _), (gmx, res) = out["fp64"], out["mixed"]

@@ -371,14 +371,23 @@ def test_schedule_table_loads():
         assert L.fringe_schedule_get(_lib.FRINGE_SCHED_SOLVE, e["n"], e["mode"], e["precision"]) == want
     assert L.fringe_schedule_set(_lib.FRINGE_SCHED_GRAM_VARIANT, 30, 7, 5, 9) == 201
     assert L.fringe_schedule_set(7, 30, 7, 5, 0) == 201
-    # rows in registers: at most 128 dates, either precision
+    # rows in registers: 33..128 dates, either precision; no work matrix in
+    # global memory up to 64 dates
     G = _lib.FRINGE_SCHED_GENERIC_A
     assert L.fringe_schedule_set(G, 160, 1, 0, 2) == 201 and L.fringe_schedule_set(G, 160, 0, 0, 2) == 201
     assert L.fringe_schedule_set(G, 96, 1, 0, 3) == 201 and L.fringe_schedule_set(G, 96, 0, 0, 2) == 0
+    assert L.fringe_schedule_set(G, 64, 1, 0, 1) == 201 and L.fringe_schedule_set(G, 48, 0, 0, 2) == 0
     L.fringe_schedule_clear()
     assert L.fringe_schedule_get(G, 80, 1, 0) == 2 and L.fringe_schedule_get(G, 80, 0, 0) == 2
     assert L.fringe_schedule_get(G, 128, 1, 0) == 2 and L.fringe_schedule_get(G, 160, 1, 0) == 0
+    # defaults up to 64 dates: rows for FP64 above 32 dates and for FP32 at
+    # 33..48; one row per lane for FP32 at 49..64 and for both up to 32
+    assert [L.fringe_schedule_get(G, n, 0, 0) for n in (30, 33, 48, 49, 64)] == [0, 2, 2, 2, 2]
+    assert [L.fringe_schedule_get(G, n, 1, 0) for n in (30, 33, 48, 49, 64)] == [0, 2, 2, 0, 0]
     assert L.fringe_schedule_set(G, 72, 1, 0, 0) == 0 and L.fringe_schedule_get(G, 96, 1, 0) == 0
+    assert L.fringe_schedule_get(G, 64, 1, 0) == 0 and L.fringe_schedule_get(G, 48, 1, 0) == 2
+    assert L.fringe_schedule_set(G, 40, 0, 0, 0) == 0 and L.fringe_schedule_get(G, 48, 0, 0) == 0
+    assert L.fringe_schedule_get(G, 64, 0, 0) == 2
     L.fringe_schedule_clear()
     _lib.load_schedules(L)
 

@@ -683,6 +683,14 @@ void solve_rows_kernel(SolveArgs a) {
 // taking part in every block, its two barriers per position over eight waves
 // cost more than the chain), into the workgroup's global area; then the
 // whole workgroup runs the FP64 chain on it.
+// Rows per lane of that Gram (a wave's 64 lanes hold rows g + 64 q, q < R:
+// one up to 64 dates, which leaves the 64-row kernels without spilled
+// registers) and its column block (each entry's sum is the same whatever
+// the rows and block; 8 instead of 16 above 64 dates spilled no less and
+// was no faster).
+template <int RW> constexpr int resolve_gram_rows() { return RW > 64 ? 2 : 1; }
+constexpr int kResolveRowsCB = 16;
+
 template <int Q, int CH, int RW, int MODE>
 __global__ __launch_bounds__(RW * Q) __attribute__((amdgpu_waves_per_eu(rows_waves_per_eu<double, CH>())))
 void resolve_rows_kernel(SolveArgs a) {
@@ -694,8 +702,8 @@ void resolve_rows_kernel(SolveArgs a) {
     for (int e = blockIdx.x; e < cnt; e += gridDim.x) {
         const long long pix = a.list[e];
         const int w = threadIdx.x >> 6;
-        const int np = gram_f64_pixel<64, 2, 16, true>(a.g64, true, pix, gam,
-                                                       smem + w * gram_f64_group_bytes(n, a.g64.Nx, a.g64.Ny, 16),
+        const int np = gram_f64_pixel<64, resolve_gram_rows<RW>(), kResolveRowsCB, true>(
+            a.g64, true, pix, gam, smem + w * gram_f64_group_bytes(n, a.g64.Nx, a.g64.Ny, kResolveRowsCB),
                                                        w, LRows<double, Q, CH, RW>::NW);
         __threadfence();
         __syncthreads();

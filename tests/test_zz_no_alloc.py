@@ -68,6 +68,28 @@ def test_noalloc_equals_link_run(precision, module, method):
         assert _same(ref, got)
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_noalloc_register_rows(precision):
+    # 33..128 dates: the register-row solvers and (mixed) their FP64 re-solve,
+    # whose areas the fixed scratch bounds; the 72-date tile takes several
+    # rounds, the second call is a smaller 48-date tile
+    cases = [(72, 128, 128, 7, 5, 811), (48, 40, 72, 5, 3, 812)]
+    for n, rows, cols, Nx, Ny, seed in cases:
+        print(f"case noalloc rows n={n} {rows}x{cols} {precision}", flush=True)
+        slc = synthetic.stack(n, rows, cols, seed=seed, patches=6)
+        ds = to_dev(slc, torch)
+        dw = wts_dev(oracle.nmap(slc, Nx, Ny, 0.05)[0], torch)
+        p = tile.link_params(rows, cols, n, Nx, Ny, module="phase_link", method="MLE", min_neighbors=5,
+                             precision=precision)
+        runner = tile.LinkRunner(p)
+        ref = runner.run(ds, dw)
+        torch.cuda.synchronize()
+        ref = tuple(x.cpu().numpy() for x in ref)
+        got = _noalloc(p, ds, dw)
+        assert (ref[1] != 0).sum() > 0.5 * rows * cols
+        assert _same(ref, got)
+
+
 def test_noalloc_refuses_a_tile_beyond_the_scratch():
     p = tile.link_params(2048, 2048, 30, 7, 5, precision="mixed")
     dummy = ctypes.c_void_p(256)

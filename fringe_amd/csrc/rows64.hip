@@ -18,10 +18,11 @@ template <int CH, int RW> static void launch_f64(const SolveArgs& a, hipStream_t
         hipLaunchKernelGGL((solve_rows_kernel<double, 4, CH, RW, MODE_EVD, double>), grid, blk, 0, s, a);
 }
 
-// Rows per workgroup: 64, 96, 112 or 128, the fewest that hold n (a whole
-// number of waves at four threads per row).
+// Rows per workgroup: 48, 64, 96, 112 or 128, the fewest that hold n (a
+// whole number of waves at four threads per row; 48 rows were 1.23x faster
+// than 64 at 33-48 dates, 80 rows no faster than 96 at 65-80).
 void launch_rows_solver_f64(const SolveArgs& a, hipStream_t s) {
-    if (a.n <= 48) launch_f64<12, 64>(a, s);
+    if (a.n <= 48) launch_f64<12, 48>(a, s);
     else if (a.n <= 64) launch_f64<16, 64>(a, s);
     else if (a.n <= 80) launch_f64<20, 96>(a, s);
     else if (a.n <= 96) launch_f64<24, 96>(a, s);
@@ -35,7 +36,7 @@ template <int CH, int RW> static size_t lds_of(int n, int Nx, int Ny) {
 }
 
 size_t resolve_rows_lds_bytes(int n, int Nx, int Ny) {
-    if (n <= 48) return lds_of<12, 64>(n, Nx, Ny);
+    if (n <= 48) return lds_of<12, 48>(n, Nx, Ny);
     if (n <= 64) return lds_of<16, 64>(n, Nx, Ny);
     if (n <= 80) return lds_of<20, 96>(n, Nx, Ny);
     if (n <= 96) return lds_of<24, 96>(n, Nx, Ny);
@@ -56,7 +57,7 @@ template <int CH, int RW> static void resolve_f64(const SolveArgs& a, unsigned b
 }
 
 void launch_resolve_rows(const SolveArgs& a, unsigned blocks, hipStream_t s) {
-    if (a.n <= 48) resolve_f64<12, 64>(a, blocks, s);
+    if (a.n <= 48) resolve_f64<12, 48>(a, blocks, s);
     else if (a.n <= 64) resolve_f64<16, 64>(a, blocks, s);
     else if (a.n <= 80) resolve_f64<20, 96>(a, blocks, s);
     else if (a.n <= 96) resolve_f64<24, 96>(a, blocks, s);

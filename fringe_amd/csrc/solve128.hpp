@@ -1,6 +1,6 @@
 // solve128.hpp -- register-resident estimator chain for 32 < n <= 128 dates,
-// one pixel per workgroup of RW rows (64, 96 or 128: the fewest that hold n)
-// of Q threads each.
+// one pixel per workgroup of RW rows (48 .. 128, the fewest that hold n) of
+// Q threads each.
 //
 // Thread t holds a 1/Q share of row r = t / Q of its pixel's matrices: the
 // columns c = Q k + q (k < CH, q = t % Q), interleaved so that the live
@@ -15,10 +15,11 @@
 // registers and applies the symmetric sweep operator one published pivot row
 // at a time.
 //
-//   T = float, Q = 2 (two to four waves): the mixed precisions, coherence
-//   from the MFMA Gram, FP32 results certified (uncertified pixels are listed
-//   for the FP64 re-solve).  CH = 24 .. 64.
-//   T = double, Q = 4 (four to eight waves): the plain FP64 path, and the
+//   T = float, Q = 4 up to 48 dates (three waves), else Q = 2 (two to four
+//   waves): the mixed precisions, coherence from the MFMA Gram, FP32 results
+//   certified (uncertified pixels are listed for the FP64 re-solve).
+//   CH = 12, 32 .. 64.
+//   T = double, Q = 4 (three to eight waves): the plain FP64 path, and the
 //   FP64 re-solve of the listed pixels (each workgroup computes the listed
 //   pixel's FP64 coherence into its global area first), so that re-solved
 //   pixels equal the FP64 path bit for bit.  CH = 12 .. 32.

@@ -155,13 +155,18 @@ def test_link_64row_solver(n, module, method, precision, place):
     assert st["comp_err"] <= PHASE_TOL
 
 
+@pytest.mark.parametrize("place", ["lanes", "registers"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_link_64row_solver_stbas_reference(precision):
-    n, rows, cols, Nx, Ny = 64, 14, 16, 5, 4
-    print(f"case 64-row stbas {precision}", flush=True)
-    slc = synthetic.stack(n, rows, cols, seed=364)
+@pytest.mark.parametrize("n", [40, 64])
+def test_link_64row_solver_stbas_reference(n, precision, place):
+    # banded coherence (STBAS) and a reference date other than the first, in
+    # both solver layouts of 33..64 dates
+    rows, cols, Nx, Ny = 14, 16, 5, 4
+    print(f"case 64-row stbas n={n} {precision} {place}", flush=True)
+    slc = synthetic.stack(n, rows, cols, seed=300 + n)
     wts = synthetic.full_mask(rows, cols, Nx, Ny)
-    g, r = _run_both(slc, wts, Nx, Ny, "evd", "STBAS", precision, bw=10, ref=3)
+    with _GenericPlace(n, precision, place), _GenericPlace(n, "fp64", place):
+        g, r = _run_both(slc, wts, Nx, Ny, "evd", "STBAS", precision, bw=10, ref=3)
     st = compare_link(g, r, r["gap"], stack=slc, ref_index=3)
     print(st)
     assert st["n_well"] >= 36 and st["codes_equal"]

@@ -11,10 +11,10 @@ stage's launches), and keeps the fastest:
 * Solver launch, per (date class, estimator, precision): one wave per unit,
   or a persistent grid of W waves per compute unit taking G units at a time
   from the work queue.
-* Above 64 dates, per (date class, precision): the solver's work matrix in
-  LDS (where it fits) or in global memory (more pixels in flight), or, for
-  up to 128 dates, each row in registers (two threads per row in FP32, four
-  in FP64).
+* Above 32 dates, per (date class, precision): each row in registers (up to
+  128 dates; two or, up to 48 dates, four threads per row in FP32, four in
+  FP64), or one row per lane (up to 64 dates), or the solver's work matrix
+  in LDS (where it fits) or in global memory (above 64 dates).
 
 The chosen table (with every candidate's time) is written as JSON; the
 library installs it when it is loaded (fringe_amd._lib.load_schedules).

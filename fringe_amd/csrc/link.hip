@@ -61,9 +61,10 @@ static void launch_solve_generic(SolveArgs a, hipStream_t s) {
                        a);
 }
 
-// 32 < n <= 128: the register-row solver (FP32 with two threads per row in
-// the mixed precisions, FP64 with four in the plain path and the re-solve)
-// where the schedule FRINGE_SCHED_GENERIC_A (or its default) selects it;
+// 32 < n <= 128: the register-row solver (FP32 with two threads per row,
+// four up to 48 dates, in the mixed precisions; FP64 with four in the plain
+// path and the re-solve) where the schedule FRINGE_SCHED_GENERIC_A (or its
+// default) selects it;
 // otherwise the one-row-per-lane solver up to 64 dates and a work matrix
 // above.
 bool rows_solver_selected(int n, int precision) {

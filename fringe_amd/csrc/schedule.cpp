@@ -9,8 +9,9 @@
 //   one wave per unit, or a persistent grid of `waves per CU` x CUs waves
 //   that take `grab` units at a time from a queue.
 //   Solver layout above 32 dates, keyed by (date class 48 / 64 / 96 / 128 /
-//   256, precision): the solver that keeps each row in registers, two
-//   (FP32) or four (FP64) threads per row (2, up to 128 dates); otherwise,
+//   256, precision): the solver that keeps each row in registers, two or
+//   (up to 48 dates) four threads per row in FP32, four in FP64 (2, up to
+//   128 dates); otherwise,
 //   up to 64 dates, the solver with one row per lane (0), and above 64 dates
 //   a work matrix in LDS where it fits (0) or in global memory (1).
 //
@@ -41,7 +42,8 @@ std::tuple<int, int, int, int> key_of(int kind, int k0, int k1, int k2) {
 int generic_a_default(int n, int precision_key) {
     if (n <= 32 || n > 128) return 0;
     // 49..64 dates in FP32: the one-row-per-lane solver (rows of two threads
-    // measured 1.35x slower there, 1.05x faster at 33..48)
+    // measured 1.35x slower there; at 33..48 rows of four threads are 1.14x
+    // faster than it)
     if (precision_key != 0 && n > 48 && n <= 64) return 0;
     return kGenericARowsDefault;
 }

@@ -196,7 +196,8 @@ int fringe_link_batch_run(const fringe_link_batch_item* items, int count, void* 
 #define FRINGE_SCHED_GRAM_VARIANT 0
 #define FRINGE_SCHED_SOLVE 1
 #define FRINGE_SCHED_GENERIC_A 2    /* (nbands class 48/64/96/128/256, precision): the solver above 32 dates
-                                       keeps each row in registers, two threads per row in FP32, four in FP64
+                                       keeps each row in registers, two (four up to 48 dates) threads per row
+                                       in FP32, four in FP64
                                        (2, up to 128 dates; the default there, except FP32 at 49..64 dates);
                                        otherwise one row per lane up to 64 dates (0), or its work matrix in LDS
                                        (0) or global memory (1) above 64 dates */

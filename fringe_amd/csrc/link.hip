@@ -126,17 +126,28 @@ size_t resolve_generic_lds_bytes(int n, int Nx, int Ny) {
            solve_static_lds(FRINGE_PREC_FP64);
 }
 
+// Two pixels per wave (n <= 32): the tridiagonal reductions process NC
+// columns, the fewest of 16, 24, 32 that hold n (the same results).
+template <typename T, typename Tg, int NC>
+static void launch_reg32(const SolveArgs& a, hipStream_t s) {
+    const unsigned blocks = (unsigned)((a.slots + 1) / 2);
+    if (a.module == FRINGE_MODULE_PHASE_LINK)
+        hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_EMI, 32, false, NC>), dim3(blocks), dim3(64), 0, s, a);
+    else if (a.method == FRINGE_METHOD_MLE)
+        hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_MLE, 32, false, NC>), dim3(blocks), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_EVD, 32, false, NC>), dim3(blocks), dim3(64), 0, s, a);
+}
+
 // n <= 64: register-resident solver, two pixels per wave (n <= 32) or one
 template <typename T, typename Tg>
 static void dispatch_reg(const SolveArgs& a, hipStream_t s) {
-    if (a.n <= 32) {
-        const unsigned blocks = (unsigned)((a.slots + 1) / 2);
-        if (a.module == FRINGE_MODULE_PHASE_LINK)
-            hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_EMI, 32, false>), dim3(blocks), dim3(64), 0, s, a);
-        else if (a.method == FRINGE_METHOD_MLE)
-            hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_MLE, 32, false>), dim3(blocks), dim3(64), 0, s, a);
-        else
-            hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_EVD, 32, false>), dim3(blocks), dim3(64), 0, s, a);
+    if (a.n <= 16) {
+        launch_reg32<T, Tg, 16>(a, s);
+    } else if (a.n <= 24) {
+        launch_reg32<T, Tg, 24>(a, s);
+    } else if (a.n <= 32) {
+        launch_reg32<T, Tg, 32>(a, s);
     } else {
         const unsigned blocks = (unsigned)a.slots;
         if (a.module == FRINGE_MODULE_PHASE_LINK)
@@ -146,6 +157,16 @@ static void dispatch_reg(const SolveArgs& a, hipStream_t s) {
         else
             hipLaunchKernelGGL((solve_reg_kernel<T, Tg, MODE_EVD, 64, false>), dim3(blocks), dim3(64), 0, s, a);
     }
+}
+
+template <int NC>
+static void launch_resolve32(const SolveArgs& a, dim3 grid, size_t lds, hipStream_t s) {
+    if (a.module == FRINGE_MODULE_PHASE_LINK)
+        hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_EMI, 32, true, NC>), grid, dim3(64), lds, s, a);
+    else if (a.method == FRINGE_METHOD_MLE)
+        hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_MLE, 32, true, NC>), grid, dim3(64), lds, s, a);
+    else
+        hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_EVD, 32, true, NC>), grid, dim3(64), lds, s, a);
 }
 
 // FP64 re-solve (Gram and estimator chain of the plain FP64 path) of the
@@ -177,13 +198,12 @@ static void launch_resolve(const LinkPlan& P, const uint32_t* wts, SolveArgs a, 
     const int gpw = P.n <= 32 ? 2 : 1;
     const size_t lds = a.group_bytes * gpw;
     const dim3 grid((unsigned)(P.resolve_blocks ? P.resolve_blocks : resolve_small_blocks(P.n, kResolveAreaBudget)));
-    if (P.n <= 32) {
-        if (a.module == FRINGE_MODULE_PHASE_LINK)
-            hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_EMI, 32, true>), grid, dim3(64), lds, s, a);
-        else if (a.method == FRINGE_METHOD_MLE)
-            hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_MLE, 32, true>), grid, dim3(64), lds, s, a);
-        else
-            hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_EVD, 32, true>), grid, dim3(64), lds, s, a);
+    if (P.n <= 16) {
+        launch_resolve32<16>(a, grid, lds, s);
+    } else if (P.n <= 24) {
+        launch_resolve32<24>(a, grid, lds, s);
+    } else if (P.n <= 32) {
+        launch_resolve32<32>(a, grid, lds, s);
     } else {
         if (a.module == FRINGE_MODULE_PHASE_LINK)
             hipLaunchKernelGGL((solve_reg_kernel<double, double, MODE_EMI, 64, true>), grid, dim3(64), lds, s, a);

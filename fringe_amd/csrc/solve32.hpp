@@ -234,7 +234,12 @@ __device__ __forceinline__ void blocks_to_rows(LdsR<T, NR>& L, const double (&M)
 // ---- Householder tridiagonalisation of the rows in h ---------------------
 // Step j of the reduction, for C0 - 1 <= j <= C0 + STEP - 2: columns below
 // C0 hold only finished entries and parked reflectors, so they are skipped.
-template <typename T, int NR, int C0, int STEP>
+// Columns from NC on (n <= NC <= NR) are skipped too: there h, V and W hold
+// +0 in every step, so each skipped product is a zero added to a sum that
+// is never -0 (it starts at +0, and an FMA returns -0 only from a -0
+// addend), and each skipped update leaves a +0 that nothing reads: the
+// results are the same bits as with NC = NR.
+template <typename T, int NR, int C0, int STEP, int NC = NR>
 __device__ __forceinline__ void tri_step(LdsR<T, NR>& L, cx<T> (&h)[NR], int j, int n, int r) {
     // row j was published at the end of the previous step (or before the first)
     const cx<T> alpha = cconj(L.R[j + 1]);
@@ -266,7 +271,7 @@ __device__ __forceinline__ void tri_step(LdsR<T, NR>& L, cx<T> (&h)[NR], int j, 
     wsync();
     cx<T> p = {T(0), T(0)};
 #pragma unroll
-    for (int c = C0; c < NR; ++c) cfma(p, h[c], L.V[c]);
+    for (int c = C0; c < NC; ++c) cfma(p, h[c], L.V[c]);
     p = cmul(tau, p);
     const cx<T> s = sumR<NR>(cmulc(vr, p));
     const cx<T> a2 = cscale(cmul(tau, s), T(-0.5));
@@ -274,7 +279,7 @@ __device__ __forceinline__ void tri_step(LdsR<T, NR>& L, cx<T> (&h)[NR], int j, 
     L.W[r] = wr;
     wsync();
 #pragma unroll
-    for (int c = C0; c < NR; ++c) {
+    for (int c = C0; c < NC; ++c) {
         cfms_c(h[c], vr, L.W[c]);
         cfms_c(h[c], wr, L.V[c]);
     }
@@ -282,46 +287,43 @@ __device__ __forceinline__ void tri_step(LdsR<T, NR>& L, cx<T> (&h)[NR], int j, 
     // lane's entry of v_j there for the back-transformation (no later step
     // reads it: V and W vanish at column j from here on)
 #pragma unroll
-    for (int c = (C0 > 0 ? C0 - 1 : 0); c < (C0 + STEP - 1 < NR ? C0 + STEP - 1 : NR); ++c)
+    for (int c = (C0 > 0 ? C0 - 1 : 0); c < (C0 + STEP - 1 < NC ? C0 + STEP - 1 : NC); ++c)
         if (c == j && r > j + 1) h[c] = vr;
-    // publish row j + 1 for the next step (everyone is past reading row j)
+    // publish row j + 1 for the next step (everyone is past reading row j;
+    // entries from NC on are never read)
     if (r == j + 1) {
 #pragma unroll
-        for (int c = 0; c < NR; ++c) L.R[c] = h[c];
+        for (int c = 0; c < NC; ++c) L.R[c] = h[c];
     }
     wsync();
 }
 
-template <typename T, int NR>
+// Steps C0 - 1 .. C0 + STEP - 2 (the last block: up to n - 2), then the
+// next block of STEP columns: one loop per quad (NR = 32) or octet (NR =
+// 64) of columns left to update (separate loops keep the register
+// allocation of each variant to itself).
+template <typename T, int NR, int NC, int C0, int STEP>
+__device__ __forceinline__ void tri_blocks(LdsR<T, NR>& L, cx<T> (&h)[NR], int& j, int n, int r) {
+    if constexpr (C0 + STEP < NC) {
+        for (; j < min(C0 + STEP - 1, n - 1); ++j) tri_step<T, NR, C0, STEP, NC>(L, h, j, n, r);
+        tri_blocks<T, NR, NC, C0 + STEP, STEP>(L, h, j, n, r);
+    } else {
+        for (; j < n - 1; ++j) tri_step<T, NR, C0, STEP, NC>(L, h, j, n, r);
+    }
+}
+
+// NC: the columns processed (n <= NC <= NR, a multiple of the block).
+template <typename T, int NR, int NC = NR>
 __device__ __forceinline__ void tridiag(LdsR<T, NR>& L, cx<T> (&h)[NR], int n, int r) {
+    static_assert(NC % (NR == 32 ? 4 : 8) == 0 && NC <= NR, "whole blocks of columns");
     if (r == 0) {
 #pragma unroll
-        for (int c = 0; c < NR; ++c) L.R[c] = h[c];
+        for (int c = 0; c < NC; ++c) L.R[c] = h[c];
     }
     wsync();
-    // columns below j+1 carry zero reflector entries: one loop per quad (NR =
-    // 32) or octet (NR = 64) of columns left to update (separate loops keep
-    // the register allocation of each variant to itself)
+    // columns below j+1 carry zero reflector entries
     int j = 0;
-    if constexpr (NR == 32) {
-        for (; j < min(3, n - 1); ++j) tri_step<T, NR, 0, 4>(L, h, j, n, r);
-        for (; j < min(7, n - 1); ++j) tri_step<T, NR, 4, 4>(L, h, j, n, r);
-        for (; j < min(11, n - 1); ++j) tri_step<T, NR, 8, 4>(L, h, j, n, r);
-        for (; j < min(15, n - 1); ++j) tri_step<T, NR, 12, 4>(L, h, j, n, r);
-        for (; j < min(19, n - 1); ++j) tri_step<T, NR, 16, 4>(L, h, j, n, r);
-        for (; j < min(23, n - 1); ++j) tri_step<T, NR, 20, 4>(L, h, j, n, r);
-        for (; j < min(27, n - 1); ++j) tri_step<T, NR, 24, 4>(L, h, j, n, r);
-        for (; j < n - 1; ++j) tri_step<T, NR, 28, 4>(L, h, j, n, r);
-    } else {
-        for (; j < min(7, n - 1); ++j) tri_step<T, NR, 0, 8>(L, h, j, n, r);
-        for (; j < min(15, n - 1); ++j) tri_step<T, NR, 8, 8>(L, h, j, n, r);
-        for (; j < min(23, n - 1); ++j) tri_step<T, NR, 16, 8>(L, h, j, n, r);
-        for (; j < min(31, n - 1); ++j) tri_step<T, NR, 24, 8>(L, h, j, n, r);
-        for (; j < min(39, n - 1); ++j) tri_step<T, NR, 32, 8>(L, h, j, n, r);
-        for (; j < min(47, n - 1); ++j) tri_step<T, NR, 40, 8>(L, h, j, n, r);
-        for (; j < min(55, n - 1); ++j) tri_step<T, NR, 48, 8>(L, h, j, n, r);
-        for (; j < n - 1; ++j) tri_step<T, NR, 56, 8>(L, h, j, n, r);
-    }
+    tri_blocks<T, NR, NC, 0, NR == 32 ? 4 : 8>(L, h, j, n, r);
     // last diagonal entry: row n - 1 is the one published last
     if (r == 0) L.D[n - 1] = L.R[n - 1].re;
     wsync();
@@ -559,7 +561,7 @@ enum { MODE_EMI = 0, MODE_EVD = 1, MODE_MLE = 2 };
 // coherence (global memory, or LDS in the FP64 re-solve), np its neighbour
 // count (-1: not linked).  CERT: the FP32 result is checked and, when it is
 // not certified, the pixel is appended to a.cert_list for the FP64 re-solve.
-template <typename T, typename Tg, int MODE, int NR, bool CERT>
+template <typename T, typename Tg, int MODE, int NR, bool CERT, int NC = NR>
 __device__ __forceinline__ void solve_reg_pixel(const SolveArgs& a, LdsR<T, NR>& L, int r, bool inslot, int np,
                                                 const cx<Tg>* gam, long long pix) {
     const int n = a.n;
@@ -575,7 +577,7 @@ __device__ __forceinline__ void solve_reg_pixel(const SolveArgs& a, LdsR<T, NR>&
 
     if constexpr (MODE == MODE_MLE) {
         const bool fin = load_rows<T, Tg, NR>(h, grow, n, r, act, -1);
-        tridiag<T, NR>(L, h, n, r);
+        tridiag<T, NR, NC>(L, h, n, r);
         const T lmin = bisect<T, NR>(L, n, 0, r, tn, pm);
         if (!fin) { code = -1.0f; failed = true; }
         else if (!(lmin >= T(1.0e-6))) { code = -2.0f; failed = true; }
@@ -592,7 +594,7 @@ __device__ __forceinline__ void solve_reg_pixel(const SolveArgs& a, LdsR<T, NR>&
         if constexpr (CERT) unc = unc || !(pmin2 >= kCertPivotMin);
         blocks_to_rows<T, NR>(L, M, m, r);
         const bool hfin = hadamard_rows<T, Tg, NR>(h, m, grow, n, r, act);
-        tridiag<T, NR>(L, h, n, r);
+        tridiag<T, NR, NC>(L, h, n, r);
         const T lam = bisect<T, NR>(L, n, 0, r, tn, pm);
         const T yr = tri_eigvec<T, NR>(L, n, r, lam, tn);
         y = back_transform<T, NR>(L, h, n, r, cx<T>{yr, T(0)});
@@ -606,7 +608,7 @@ __device__ __forceinline__ void solve_reg_pixel(const SolveArgs& a, LdsR<T, NR>&
     } else if constexpr (MODE == MODE_EVD) {
         const int band = (a.method == FRINGE_METHOD_STBAS) ? a.bw : -1;
         const bool fin = load_rows<T, Tg, NR>(h, grow, n, r, act, band);
-        tridiag<T, NR>(L, h, n, r);
+        tridiag<T, NR, NC>(L, h, n, r);
         const T lam = bisect<T, NR>(L, n, n - 1, r, tn, pm);
         const T yr = tri_eigvec<T, NR>(L, n, r, lam, tn);
         y = back_transform<T, NR>(L, h, n, r, cx<T>{yr, T(0)});
@@ -629,7 +631,7 @@ __device__ __forceinline__ void solve_reg_pixel(const SolveArgs& a, LdsR<T, NR>&
         // the EMI eigenpair, unless every pixel of the wave falls back to EVD
         // (a pixel's result never depends on whether this ran for it)
         if (__any(!run_evd)) {
-            tridiag<T, NR>(L, h, n, r);
+            tridiag<T, NR, NC>(L, h, n, r);
             const T lam = bisect<T, NR>(L, n, 0, r, tn, pm);
             const T yr = tri_eigvec<T, NR>(L, n, r, lam, tn);
             y = back_transform<T, NR>(L, h, n, r, cx<T>{yr, T(0)});
@@ -641,7 +643,7 @@ __device__ __forceinline__ void solve_reg_pixel(const SolveArgs& a, LdsR<T, NR>&
         }
         if (__any(act && fin && run_evd)) {
             load_rows<T, Tg, NR>(h, grow, n, r, act, -1);
-            tridiag<T, NR>(L, h, n, r);
+            tridiag<T, NR, NC>(L, h, n, r);
             const T le = bisect<T, NR>(L, n, n - 1, r, tn, pm);
             const T ye = tri_eigvec<T, NR>(L, n, r, le, tn);
             const cx<T> yv = back_transform<T, NR>(L, h, n, r, cx<T>{ye, T(0)});
@@ -749,8 +751,9 @@ __device__ __forceinline__ void solve_reg_pixel(const SolveArgs& a, LdsR<T, NR>&
 // NR = 32 holds two pixels per wave, NR = 64 one.  The FP32 instances
 // certify their results.  LIST = true (T = Tg = double): the FP64 re-solve
 // of the listed pixels; each group computes its pixel's coherence with the
-// FP64 Gram into LDS and solves it there, looping over the list.
-template <typename T, typename Tg, int MODE, int NR, bool LIST>
+// FP64 Gram into LDS and solves it there, looping over the list.  NC: the
+// columns the tridiagonal reductions process (n <= NC; same results).
+template <typename T, typename Tg, int MODE, int NR, bool LIST, int NC = NR>
 __global__ __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && NR == 32 && MODE == MODE_MLE ? 4 : 1)))
 void solve_reg_kernel(SolveArgs a) {
@@ -766,7 +769,7 @@ void solve_reg_kernel(SolveArgs a) {
         const int np = inslot ? a.npix[slot] : -1;
         const cx<Tg>* gam = (const cx<Tg>*)a.gamma + (size_t)(inslot ? slot : 0) * a.n * a.n;
         const long long pix = a.first + (inslot ? slot : 0);
-        solve_reg_pixel<T, Tg, MODE, NR, sizeof(T) == 4>(a, L, r, inslot, np, gam, pix);
+        solve_reg_pixel<T, Tg, MODE, NR, sizeof(T) == 4, NC>(a, L, r, inslot, np, gam, pix);
     } else {
         static_assert(sizeof(T) == 8 && sizeof(Tg) == 8, "the re-solve runs in FP64");
         constexpr int GPW = 64 / NR;
@@ -783,7 +786,7 @@ void solve_reg_kernel(SolveArgs a) {
             const int np = gram_f64_pixel<NR, 1, kResolveSmallCB>(a.g64, valid, pix, gl, gscratch);
             __threadfence();
             __syncthreads();
-            solve_reg_pixel<T, Tg, MODE, NR, false>(a, L, r, valid, np, (const cx<Tg>*)gl, pix);
+            solve_reg_pixel<T, Tg, MODE, NR, false, NC>(a, L, r, valid, np, (const cx<Tg>*)gl, pix);
             __syncthreads();
         }
     }

@@ -127,7 +127,7 @@ size_t resolve_generic_lds_bytes(int n, int Nx, int Ny) {
 }
 
 // Two pixels per wave (n <= 32): the tridiagonal reductions process NC
-// columns, the fewest of 16, 24, 32 that hold n (the same results).
+// columns, the fewest of 16, 24, 28, 30, 32 that hold n (the same results).
 template <typename T, typename Tg, int NC>
 static void launch_reg32(const SolveArgs& a, hipStream_t s) {
     const unsigned blocks = (unsigned)((a.slots + 1) / 2);
@@ -146,6 +146,10 @@ static void dispatch_reg(const SolveArgs& a, hipStream_t s) {
         launch_reg32<T, Tg, 16>(a, s);
     } else if (a.n <= 24) {
         launch_reg32<T, Tg, 24>(a, s);
+    } else if (a.n <= 28) {
+        launch_reg32<T, Tg, 28>(a, s);
+    } else if (a.n <= 30) {
+        launch_reg32<T, Tg, 30>(a, s);
     } else if (a.n <= 32) {
         launch_reg32<T, Tg, 32>(a, s);
     } else {
@@ -202,6 +206,10 @@ static void launch_resolve(const LinkPlan& P, const uint32_t* wts, SolveArgs a, 
         launch_resolve32<16>(a, grid, lds, s);
     } else if (P.n <= 24) {
         launch_resolve32<24>(a, grid, lds, s);
+    } else if (P.n <= 28) {
+        launch_resolve32<28>(a, grid, lds, s);
+    } else if (P.n <= 30) {
+        launch_resolve32<30>(a, grid, lds, s);
     } else if (P.n <= 32) {
         launch_resolve32<32>(a, grid, lds, s);
     } else {

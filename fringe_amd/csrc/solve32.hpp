@@ -312,10 +312,10 @@ __device__ __forceinline__ void tri_blocks(LdsR<T, NR>& L, cx<T> (&h)[NR], int& 
     }
 }
 
-// NC: the columns processed (n <= NC <= NR, a multiple of the block).
+// NC: the columns processed (n <= NC <= NR; the last block may be partial).
 template <typename T, int NR, int NC = NR>
 __device__ __forceinline__ void tridiag(LdsR<T, NR>& L, cx<T> (&h)[NR], int n, int r) {
-    static_assert(NC % (NR == 32 ? 4 : 8) == 0 && NC <= NR, "whole blocks of columns");
+    static_assert(NC > 0 && NC <= NR, "columns of the lanes' rows");
     if (r == 0) {
 #pragma unroll
         for (int c = 0; c < NC; ++c) L.R[c] = h[c];

@@ -111,11 +111,12 @@ def test_batch_across_rounds():
 
 
 @pytest.mark.parametrize("n,precision", [(30, "mixed"), (48, "mixed"), (20, "fp64"), (12, "mixed"), (16, "fp64"),
+                                         (27, "mixed"), (30, "fp64"),
                                          (80, "mixed"), (100, "mixed"), (136, "mixed"), (100, "fp64")])
 def test_persistent_schedule_equals_grid(n, precision):
-    # up to 32 dates the grid launch's tridiagonal reductions process 16, 24
-    # or 32 columns (the fewest that hold n) and the persistent launch's 32:
-    # the same bits
+    # up to 32 dates the grid launch's tridiagonal reductions process 16, 24,
+    # 28, 30 or 32 columns (the fewest that hold n) and the persistent
+    # launch's 32: the same bits
     L = _lib.lib()
     rows, cols, Nx, Ny = 40, 64, 7, 5
     print(f"case persistent schedule n={n} {precision}", flush=True)
